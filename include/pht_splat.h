@@ -1,0 +1,245 @@
+/*
+ * pht_splat.h — C ABI of the MI355X (gfx950) 3D Gaussian Splatting training step.
+ *
+ * This is the drop-in boundary for the hot path of the reference's splat trainer:
+ *   photara::splat::Rasterizer::forward / ::backward
+ *     (photara/include/splat/rasterizer.hpp:128-165, impl photara/src/splat/rasterizer.cu:340-743)
+ *   splat_drender::Rasterizer::forward / ::backward over POD views + WorkspacePools
+ *     (third_party/splat_drender/include/splat_drender/api.h:19-306, src/pipeline.cu:457-712)
+ *   detail::adam_step / adam_step_structure / adam_step_active_prefix
+ *     (photara/src/splat/optimizer.cpp:461-534, photara/src/splat/cuda_ops.cu:1291-1546, 2863-2957)
+ *   loss_kernel, L1 part (photara/src/splat/cuda_ops.cu:846-898)
+ *
+ * Everything crossing this boundary is plain C: device pointers, sizes, POD structs and a
+ * hipStream_t passed as void*. No C++ or torch types. All device pointers are caller-owned;
+ * the library never allocates device memory on the compute path (the caller supplies the
+ * workspace bytes reported by pht_workspace_bytes()).
+ *
+ * Layouts follow the reference: means [N,3], log_scales [N,3], quaternions [N,4] scalar-first,
+ * opacity_logits [N] (reference shape [N,1]), sh [N,K,3] (K = bases per channel, 16 at degree 3),
+ * images planar CHW: color [3,H,W], alpha [H,W]. world_to_camera is the reference's column-major
+ * 4x4 (photara/include/splat/types.hpp:17-34; transform t = v[0..2]x + v[4..6]y + v[8..10]z + v[12..14]).
+ */
+#ifndef PHT_SPLAT_H
+#define PHT_SPLAT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PHT_ABI_VERSION 1
+#define PHT_MAX_VIEWS 32
+
+/* Status codes. The reference throws std::invalid_argument for bad inputs
+ * (third_party/splat_drender/src/pipeline.cu:63-66,460-468), std::runtime_error for device
+ * errors (:68-72) and std::logic_error for a backward without recorded forward state
+ * (:655-656). The host mirrors map PHT_EINVAL -> invalid_argument / ValueError,
+ * PHT_EDEVICE -> runtime_error / RuntimeError, PHT_ESTATE -> logic_error / RuntimeError. */
+enum pht_status {
+    PHT_OK = 0,
+    PHT_EINVAL = 1,    /* invalid argument, shape or pointer */
+    PHT_EDEVICE = 2,   /* HIP runtime error */
+    PHT_ECAPACITY = 3, /* instance list exceeded the workspace capacity (see pht_counts) */
+    PHT_ESTATE = 4,    /* backward without a matching forward in this workspace */
+    PHT_ENOTSUP = 5    /* a reference feature outside this build (depth/normal channels, non-pinhole) */
+};
+
+/* Reduced-precision storage formats for SH / log-scales / opacity logits. */
+enum pht_lp_format { PHT_LP_NONE = 0, PHT_LP_BF16 = 1, PHT_LP_FP16 = 2 };
+
+/* One pinhole camera. Mirrors photara::splat::Camera (types.hpp:17-34) after
+ * camera_view_of() (rasterizer.cu:200-224): cx/cy already include -pixel_center_offset. */
+typedef struct pht_camera {
+    float world_to_camera[16]; /* column-major 4x4 */
+    float position[3];         /* camera centre in world space (SH view direction) */
+    float fx, fy, cx, cy;
+    int32_t width, height;
+    float background[3];
+    int32_t reserved;
+} pht_camera;
+
+/* Raster settings. Mirrors RasterizeOptions / RasterizationPolicy
+ * (rasterizer.hpp:9-54, options.hpp:32-37) for the pinhole RGB training path. */
+typedef struct pht_settings {
+    int32_t active_sh_degree; /* min(requested, model degree), rasterizer.cu:361-362 */
+    int32_t sh_bases;         /* K: coefficients per channel stored per Gaussian */
+    float kernel_size;        /* 2D dilation added to the projected covariance */
+    float scale_modifier;
+    float alpha_clip;         /* 0.99 */
+    float near_plane;         /* 0.2 */
+    int32_t compensate_2d;    /* opacity *= sqrt(det0/det1) */
+    int32_t tile_size;        /* 16 (reference) or 8 */
+    int32_t batch;            /* LDS batch depth of the blend kernels: 32, 64, 128 or 256 */
+    int32_t fp64_conic;       /* route near-singular 2x2 inverses through FP64 */
+    int32_t require_depth;    /* must be 0: median-depth/normal channels are not built */
+    int32_t reserved;
+} pht_settings;
+
+/* Gaussian parameters (raw, pre-activation), reference GaussianModel (types.hpp:36-56)
+ * passed through gaussians_of() (rasterizer.cu:226-251). When lp_format != PHT_LP_NONE the
+ * *_lp arrays are read instead of sh / log_scales / opacity_logits. */
+typedef struct pht_gaussians {
+    int64_t count;
+    const float* means;          /* [N,3] */
+    const float* log_scales;     /* [N,3] */
+    const float* quaternions;    /* [N,4] scalar-first, unnormalised */
+    const float* opacity_logits; /* [N] */
+    const float* sh;             /* [N,K,3] */
+    const float* filter_3d;      /* [N] optional Mip 3D filter, or NULL */
+    const float* colors_precomp; /* [N,3] optional: bypasses SH (RasterizeOptions::colors_precomp) */
+    const uint16_t* sh_lp;             /* [N,K,3] bf16/fp16 bits */
+    const uint16_t* log_scales_lp;     /* [N,3] */
+    const uint16_t* opacity_logits_lp; /* [N] */
+    int32_t lp_format;
+    int32_t reserved;
+} pht_gaussians;
+
+/* Per-view forward outputs (RenderResult, types.hpp:99-111). Optional pointers may be NULL. */
+typedef struct pht_view_outputs {
+    float* color;        /* [3,H,W] required */
+    float* alpha;        /* [H,W]   required: 1 - final transmittance */
+    int32_t* radii;      /* [N]     optional */
+    float* visibility;   /* [N]     optional: 1 where the Gaussian blended into >= 1 pixel */
+} pht_view_outputs;
+
+/* Per-view loss gradients (LossGradients, api.h). grad_alpha may be NULL (treated as 0). */
+typedef struct pht_view_grads {
+    const float* grad_color; /* [3,H,W] */
+    const float* grad_alpha; /* [H,W] or NULL */
+} pht_view_grads;
+
+/* Model gradients wrt raw parameters (ModelGradients, types.hpp:71-87), summed over views.
+ * Arrays are overwritten (not accumulated). Any pointer may be NULL to skip that output. */
+typedef struct pht_model_grads {
+    float* means;          /* [N,3] */
+    float* log_scales;     /* [N,3] */
+    float* quaternions;    /* [N,4] */
+    float* opacity_logits; /* [N]   */
+    float* sh;             /* [N,K,3] */
+    float* colors_precomp; /* [N,3] only when forward used colors_precomp */
+    float* refine_weight;  /* [N] */
+} pht_model_grads;
+
+/* Fused Adam over every Gaussian parameter (StructureAdamUpdate rasterizer.hpp:84-104 plus the
+ * FP32 active-prefix SH Adam, cuda_ops.cu:1506-1546). Masters are FP32 and updated in place;
+ * when lp_format != PHT_LP_NONE the *_lp mirrors are rewritten from the updated masters. */
+typedef struct pht_adam {
+    float* means; float* log_scales; float* quaternions; float* opacity_logits; float* sh;
+    float* m_means; float* v_means;
+    float* m_log_scales; float* v_log_scales;
+    float* m_quaternions; float* v_quaternions;
+    float* m_opacity_logits; float* v_opacity_logits;
+    float* m_sh; float* v_sh;
+    float lr_means, lr_scales, lr_quaternions, lr_opacity, lr_sh0, lr_sh_rest;
+    float beta1, beta2, correction1, correction2, epsilon;
+    float min_log_scale, max_log_scale;
+    /* Optional device float[3] = {correction1, correction2, lr_means}: when non-NULL it
+     * overrides the three fields above, so a captured step graph can be replayed with the
+     * per-step bias corrections and learning-rate schedule written before each launch. */
+    const float* device_hparams;
+    uint16_t* sh_lp; uint16_t* log_scales_lp; uint16_t* opacity_logits_lp;
+    int32_t lp_format;
+    int32_t reserved;
+} pht_adam;
+
+/* Workspace capacities. instances = capacity of the (tile, splat) instance list summed over views. */
+typedef struct pht_capacity {
+    int64_t gaussians;
+    int32_t views;
+    int32_t tile_size;
+    int64_t tiles;     /* sum over views of ceil(W/ts)*ceil(H/ts) */
+    int64_t pixels;    /* sum over views of W*H */
+    int64_t instances;
+} pht_capacity;
+
+/* Device-side counters of the last forward. */
+typedef struct pht_counts {
+    int64_t visible;      /* visible (gaussian, view) splats */
+    int64_t instances;    /* emitted (tile, splat) instances (may exceed capacity -> overflow) */
+    int64_t fp64_conics;  /* splats whose 2x2 inverse was routed through FP64 */
+    int32_t overflow;     /* 1 when instances > capacity: outputs of that forward are invalid */
+    int32_t reserved;
+} pht_counts;
+
+typedef struct pht_workspace pht_workspace; /* opaque host handle over caller-owned device bytes */
+
+/* ---- library ---- */
+int pht_abi_version(void);
+const char* pht_last_error(void);
+/* Number of visible HIP devices (0 without a GPU); never initialises a context on failure paths. */
+int pht_device_count(void);
+
+/* ---- workspace (replaces WorkspacePools grow-callbacks, api.h:227-240, rasterizer.cu:177-185) ---- */
+size_t pht_workspace_bytes(const pht_capacity* cap);
+int pht_workspace_create(void* device_bytes, size_t nbytes, const pht_capacity* cap,
+                         pht_workspace** out);
+void pht_workspace_destroy(pht_workspace* ws);
+
+/* ---- forward: replaces splat_drender::Rasterizer::forward (pipeline.cu:457-615) and the
+ * CudaRasterizer::forward bridge (rasterizer.cu:340-402) for n_views pinhole views at once.
+ * Enqueues on `stream` only. check_overflow=1 synchronises once at the end to report
+ * PHT_ECAPACITY (the reference reads its counters back too, pipeline.cu:206-212);
+ * check_overflow=0 never synchronises (graph capture) — query pht_counts afterwards. ---- */
+int pht_forward(pht_workspace* ws, const pht_gaussians* g, const pht_camera* cams, int n_views,
+                const pht_settings* s, pht_view_outputs* out, int check_overflow, void* stream);
+
+/* ---- backward: replaces splat_drender::Rasterizer::backward (pipeline.cu:651-712) and
+ * CudaRasterizer::backward (rasterizer.cu:495-743). Uses the state recorded by the last
+ * pht_forward on this workspace (PHT_ESTATE if none). With adam != NULL the per-Gaussian
+ * kernel applies the fused Adam update instead of (or in addition to) writing grads. ---- */
+int pht_backward(pht_workspace* ws, const pht_gaussians* g, const pht_view_grads* dl,
+                 pht_model_grads* grads, const pht_adam* adam, void* stream);
+
+/* ---- L1 photometric loss gradient (cuda_ops.cu:886-898, full image, no mask):
+ * grad = weight * sign(color - target) / (3 * W * H); loss_sum (device float, may be NULL)
+ * receives += sum |color - target| * weight / (3 W H). ---- */
+int pht_l1_loss(const float* color, const float* target, float* grad_color, float* loss_sum,
+                int64_t pixels, float weight, void* stream);
+
+/* ---- standalone Adam (adam_kernel / adam_active_prefix_kernel, cuda_ops.cu:1336-1358,1506-1546):
+ * group_stride/active/secondary lr follow the reference signature. ---- */
+int pht_adam_step(float* param, const float* grad, float* m, float* v, int64_t rows,
+                  int64_t row_stride, int64_t active_cols, float lr, float lr_secondary,
+                  int64_t secondary_from_col, float beta1, float beta2, float correction1,
+                  float correction2, float epsilon, float clamp_min, float clamp_max,
+                  int32_t prefix_semantics, void* stream);
+
+/* ---- counters and debug views of the last forward ---- */
+int pht_get_counts(pht_workspace* ws, pht_counts* out, void* stream); /* synchronises */
+/* Copies the sorted instance list as 64-bit keys ((global_tile << 32) | depth_bits) and
+ * values (gaussian index) plus per-tile [start,end) ranges and per-pixel contributor counts
+ * into caller device buffers (any may be NULL). Sizes: keys/values >= instances, ranges
+ * >= 2*tiles, n_contrib >= pixels. */
+int pht_debug_lists(pht_workspace* ws, uint64_t* keys, uint32_t* values, uint32_t* ranges,
+                    uint32_t* n_contrib, float* final_T, void* stream);
+/* Copies the per-(gaussian, view) blend gradients of the last backward (the reference's GradState,
+ * buffers.h): out[(g * n_views + v) * 10 + k], k = d_mean2d.xy, d_conic.xyz, d_opacity, d_color.rgb,
+ * refine; zero for (gaussian, view) pairs that were not visible. out holds N * n_views * 10 floats. */
+int pht_debug_splat_grads(pht_workspace* ws, float* out, void* stream);
+
+/* ---- per-stage timing (the reference's CudaTrainingProfiler, trainer.cpp:216-258, as hipEvents):
+ * with profiling on, forward/backward record an event at each stage boundary on their stream.
+ * Stages: 0 preprocess, 1 visibility compaction + depth sort, 2 tile offsets + instance emission,
+ * 3 tile sort + ranges, 4 blend forward, 5 blend backward, 6 per-Gaussian backward (+ Adam).
+ * pht_stage_times synchronises and writes the last recorded milliseconds of each stage. ---- */
+#define PHT_STAGES 7
+int pht_set_profiling(pht_workspace* ws, int enable);
+int pht_stage_times(pht_workspace* ws, float* ms_out, int max_stages);
+
+/* ---- whole training step as one hipGraph (forward + L1 + backward + fused Adam) ---- */
+typedef struct pht_step_graph pht_step_graph;
+int pht_step_graph_create(pht_workspace* ws, const pht_gaussians* g, const pht_camera* cams,
+                          int n_views, const pht_settings* s, pht_view_outputs* out,
+                          const float* const* targets, float* const* grad_color_bufs,
+                          float* loss_sum, const pht_adam* adam, void* stream,
+                          pht_step_graph** out_graph);
+int pht_step_graph_launch(pht_step_graph* graph, void* stream);
+void pht_step_graph_destroy(pht_step_graph* graph);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PHT_SPLAT_H */

@@ -1,0 +1,254 @@
+"""ctypes binding of the CPU oracle (``oracle/_build/liboracle.so``) — TEST INFRASTRUCTURE ONLY.
+
+Used by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg as the checker or the
+timed CPU baseline. The product package never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from photara_studio_amd import _abi
+from photara_studio_amd.rasterizer import Camera, RasterizeOptions, make_camera
+
+LIB = Path(__file__).resolve().parent / "_build" / "liboracle.so"
+_lib = None
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            from oracle import build as _b
+            _b.build(verbose=False)
+        lib = C.CDLL(str(LIB))
+        vp = C.c_void_p
+        lib.oracle_pexpf.argtypes = [vp, vp, C.c_int64]
+        lib.oracle_plogf.argtypes = [vp, vp, C.c_int64]
+        lib.oracle_gaussian_power.argtypes = [vp, vp, vp, C.c_int64]
+        lib.oracle_lp_roundtrip.argtypes = [vp, vp, vp, C.c_int64, C.c_int]
+        lib.oracle_project.argtypes = [C.c_int, C.POINTER(_abi.pht_gaussians), C.POINTER(_abi.pht_camera),
+                                       C.POINTER(_abi.pht_settings), vp]
+        lib.oracle_enumerate.argtypes = [C.c_float, C.c_float, vp, C.c_float, C.c_int, C.c_int, C.c_int, vp, C.c_int]
+        lib.oracle_enumerate.restype = C.c_int
+        lib.oracle_forward.argtypes = [C.c_int, C.POINTER(_abi.pht_gaussians), C.POINTER(_abi.pht_camera), C.c_int,
+                                       C.POINTER(_abi.pht_settings), C.POINTER(_abi.pht_view_outputs), vp, vp,
+                                       C.c_int64, vp, vp, vp, C.POINTER(_abi.pht_counts), C.c_int]
+        lib.oracle_backward.argtypes = [C.c_int, C.POINTER(_abi.pht_gaussians), C.POINTER(_abi.pht_settings),
+                                        C.POINTER(_abi.pht_view_grads), vp, vp, vp, vp, vp, vp, vp, C.c_int]
+        lib.oracle_l1_loss.argtypes = [vp, vp, vp, C.c_int64, C.c_float]
+        lib.oracle_l1_loss.restype = C.c_double
+        lib.oracle_adam_step.argtypes = [vp, vp, vp, vp, C.c_int64, C.c_int64, C.c_int64, C.c_float, C.c_float,
+                                         C.c_int64, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
+                                         C.c_float, C.c_float, C.c_int]
+        lib.oracle_train_step.argtypes = [C.POINTER(_abi.pht_gaussians), C.POINTER(_abi.pht_camera), C.c_int,
+                                          C.POINTER(_abi.pht_settings), C.POINTER(_abi.pht_view_outputs),
+                                          C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(_abi.pht_adam),
+                                          C.c_int]
+        lib.oracle_train_step.restype = C.c_double
+        lib.oracle_splat_grads.argtypes = [C.c_int, C.c_int64, vp]
+        _lib = lib
+    return _lib
+
+
+def _p(a: Optional[np.ndarray]):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def f32(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+
+
+def threads_default() -> int:
+    return int(os.environ.get("PHT_ORACLE_THREADS", "0")) or (os.cpu_count() or 1)
+
+
+class Scene:
+    """Host copy of one Gaussian set plus the C structs the oracle reads."""
+
+    def __init__(self, arrays: Dict[str, np.ndarray], sh_degree: int = 3, colors_precomp=None, filter_3d=None,
+                 lp: Optional[str] = None):
+        self.a = {k: f32(v) for k, v in arrays.items()}
+        self.n = int(self.a["means"].shape[0])
+        self.sh_bases = int(self.a["sh"].shape[1])
+        self.sh_degree = sh_degree
+        self.colors_precomp = None if colors_precomp is None else f32(colors_precomp)
+        self.filter_3d = None if filter_3d is None else f32(filter_3d).reshape(-1)
+        self.lp = lp
+        self.lp_arrays = {}
+        if lp is not None:
+            import torch
+            dt = torch.bfloat16 if lp == "bf16" else torch.float16
+            for k in ("sh", "log_scales", "opacity_logits"):
+                t = torch.from_numpy(self.a[k]).to(dt)
+                self.lp_arrays[k] = np.ascontiguousarray(t.view(torch.int16).numpy().view(np.uint16))
+
+    def gaussians(self) -> _abi.pht_gaussians:
+        g = _abi.pht_gaussians()
+        g.count = self.n
+        g.means = self.a["means"].ctypes.data
+        g.log_scales = self.a["log_scales"].ctypes.data
+        g.quaternions = self.a["quaternions"].ctypes.data
+        g.opacity_logits = self.a["opacity_logits"].ctypes.data
+        g.sh = self.a["sh"].ctypes.data
+        g.filter_3d = None if self.filter_3d is None else self.filter_3d.ctypes.data
+        g.colors_precomp = None if self.colors_precomp is None else self.colors_precomp.ctypes.data
+        if self.lp is not None:
+            g.lp_format = _abi.PHT_LP_BF16 if self.lp == "bf16" else _abi.PHT_LP_FP16
+            g.sh_lp = self.lp_arrays["sh"].ctypes.data
+            g.log_scales_lp = self.lp_arrays["log_scales"].ctypes.data
+            g.opacity_logits_lp = self.lp_arrays["opacity_logits"].ctypes.data
+        return g
+
+
+def settings_for(scene: Scene, options: RasterizeOptions) -> _abi.pht_settings:
+    s = _abi.pht_settings()
+    s.active_sh_degree = min(int(options.active_sh_degree), scene.sh_degree)
+    s.sh_bases = scene.sh_bases
+    s.kernel_size = float(options.kernel_size)
+    s.scale_modifier = float(options.scale_modifier)
+    s.alpha_clip = float(options.rasterization.alpha_clip)
+    s.near_plane = float(options.rasterization.near_plane)
+    s.compensate_2d = 1 if options.rasterization.compensate_2d else 0
+    s.tile_size = int(options.tile_size)
+    s.batch = int(options.batch)
+    s.fp64_conic = 1 if options.fp64_conic else 0
+    return s
+
+
+class OracleRun:
+    """One oracle forward (and optional backward) over a list of cameras."""
+
+    def __init__(self, scene: Scene, cameras: Sequence[Camera], options: RasterizeOptions, precision: int = 32,
+                 threads: Optional[int] = None, key_cap: int = 1 << 26):
+        self.scene, self.cameras, self.options, self.precision = scene, list(cameras), options, precision
+        self.threads = threads or threads_default()
+        lib = load()
+        nv = len(self.cameras)
+        self.cams = (_abi.pht_camera * nv)(*[make_camera(c, options) for c in self.cameras])
+        self.settings = settings_for(scene, options)
+        self.g = scene.gaussians()
+        n = scene.n
+        self.color = [np.zeros((3, c.height, c.width), np.float32) for c in self.cameras]
+        self.alpha = [np.zeros((c.height, c.width), np.float32) for c in self.cameras]
+        self.radii = [np.zeros((n,), np.int32) for _ in self.cameras]
+        self.visibility = [np.zeros((n,), np.float32) for _ in self.cameras]
+        outs = (_abi.pht_view_outputs * nv)()
+        for k in range(nv):
+            outs[k].color, outs[k].alpha = self.color[k].ctypes.data, self.alpha[k].ctypes.data
+            outs[k].radii, outs[k].visibility = self.radii[k].ctypes.data, self.visibility[k].ctypes.data
+        ts = options.tile_size
+        self.tiles = sum(((c.width + ts - 1) // ts) * ((c.height + ts - 1) // ts) for c in self.cameras)
+        self.pixels = sum(c.width * c.height for c in self.cameras)
+        counts = _abi.pht_counts()
+        if precision == 32:
+            keys = np.zeros((key_cap,), np.uint64)
+            values = np.zeros((key_cap,), np.uint32)
+            ranges = np.zeros((2 * self.tiles,), np.uint32)
+            n_contrib = np.zeros((self.pixels,), np.uint32)
+            final_T = np.zeros((self.pixels,), np.float32)
+            lib.oracle_forward(32, C.byref(self.g), self.cams, nv, C.byref(self.settings), outs, _p(keys), _p(values),
+                               key_cap, _p(ranges), _p(n_contrib), _p(final_T), C.byref(counts), self.threads)
+            ni = int(counts.instances)
+            self.keys, self.values = keys[:ni].copy(), values[:ni].copy()
+            self.ranges, self.n_contrib, self.final_T = ranges, n_contrib, final_T
+        else:
+            lib.oracle_forward(64, C.byref(self.g), self.cams, nv, C.byref(self.settings), outs, None, None, 0, None,
+                               None, None, C.byref(counts), self.threads)
+        self.counts = counts
+        self._outs = outs
+
+    def backward(self, grad_colors: Sequence[np.ndarray], grad_alphas: Optional[Sequence] = None) -> Dict[str, np.ndarray]:
+        lib = load()
+        nv = len(self.cameras)
+        gcs = [f32(g) for g in grad_colors]
+        gas = [None if (grad_alphas is None or grad_alphas[k] is None) else f32(grad_alphas[k]) for k in range(nv)]
+        dl = (_abi.pht_view_grads * nv)()
+        for k in range(nv):
+            dl[k].grad_color = gcs[k].ctypes.data
+            dl[k].grad_alpha = None if gas[k] is None else gas[k].ctypes.data
+        n, s = self.scene.n, self.scene
+        dt = np.float64 if self.precision == 64 else np.float32
+        out = {
+            "means": np.zeros((n, 3), dt), "log_scales": np.zeros((n, 3), dt), "quaternions": np.zeros((n, 4), dt),
+            "opacity_logits": np.zeros((n, 1), dt), "sh": np.zeros((n, s.sh_bases, 3), dt),
+            "colors_precomp": np.zeros((n, 3), dt), "refine_weight": np.zeros((n,), dt),
+        }
+        lib.oracle_backward(self.precision, C.byref(self.g), C.byref(self.settings), dl, _p(out["means"]),
+                            _p(out["log_scales"]), _p(out["quaternions"]), _p(out["opacity_logits"]),
+                            _p(out["sh"]) if s.colors_precomp is None else None,
+                            _p(out["colors_precomp"]) if s.colors_precomp is not None else None,
+                            _p(out["refine_weight"]), self.threads)
+        self._keep = (gcs, gas)
+        return out
+
+    def splat_grads(self) -> np.ndarray:
+        """Per-(gaussian, view) blend gradients of the last backward, [N, views, 10]."""
+        out = np.zeros((self.scene.n, len(self.cameras), 10), np.float64)
+        load().oracle_splat_grads(self.precision, self.scene.n, _p(out))
+        return out
+
+
+def pexpf(x: np.ndarray) -> np.ndarray:
+    x = f32(x)
+    y = np.empty_like(x)
+    load().oracle_pexpf(_p(x), _p(y), x.size)
+    return y
+
+
+def plogf(x: np.ndarray) -> np.ndarray:
+    x = f32(x)
+    y = np.empty_like(x)
+    load().oracle_plogf(_p(x), _p(y), x.size)
+    return y
+
+
+def gaussian_power(conic3: np.ndarray, d2: np.ndarray) -> np.ndarray:
+    conic3, d2 = f32(conic3), f32(d2)
+    out = np.empty((conic3.shape[0],), np.float32)
+    load().oracle_gaussian_power(_p(conic3), _p(d2), _p(out), conic3.shape[0])
+    return out
+
+
+def lp_roundtrip(x: np.ndarray, fmt: int):
+    x = f32(x)
+    bits = np.empty(x.shape, np.uint16)
+    back = np.empty_like(x)
+    load().oracle_lp_roundtrip(_p(x), _p(bits), _p(back), x.size, fmt)
+    return bits, back
+
+
+def project(scene: Scene, camera: Camera, options: RasterizeOptions, precision: int = 32) -> np.ndarray:
+    """Per-Gaussian projection rows: mean2d(2) conic(3) opacity depth radius tiles rgb(3) clamp fp64 visible."""
+    out = np.zeros((scene.n, 16), np.float64)
+    cam = make_camera(camera, options)
+    s = settings_for(scene, options)
+    g = scene.gaussians()
+    load().oracle_project(precision, C.byref(g), C.byref(cam), C.byref(s), _p(out))
+    return out
+
+
+def enumerate_tiles(mean2d, conic3, opacity, grid_x, grid_y, ts, cap=4096) -> List[int]:
+    conic3 = f32(conic3)
+    buf = np.zeros((cap,), np.int32)
+    n = load().oracle_enumerate(float(mean2d[0]), float(mean2d[1]), _p(conic3), float(opacity), grid_x, grid_y,
+                                ts, _p(buf), cap)
+    return buf[:min(n, cap)].tolist()
+
+
+def l1_loss(color: np.ndarray, target: np.ndarray, weight: float = 1.0):
+    color, target = f32(color), f32(target)
+    grad = np.empty_like(color)
+    loss = load().oracle_l1_loss(_p(color), _p(target), _p(grad), color[0].size, weight)
+    return loss, grad
+
+
+def adam_step(p, g, m, v, rows, stride, active, lr, lr2, secondary_from, b1, b2, c1, c2, eps, lo, hi, prefix):
+    for a in (p, g, m, v):
+        assert a.dtype == np.float32 and a.flags.c_contiguous
+    load().oracle_adam_step(_p(p), _p(g), _p(m), _p(v), rows, stride, active, lr, lr2, secondary_from, b1, b2, c1, c2,
+                            eps, lo, hi, prefix)

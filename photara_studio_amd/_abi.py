@@ -1,0 +1,174 @@
+"""ctypes mirror of ``include/pht_splat.h`` and the loader of the in-tree kernel library.
+
+The library is loaded from ``photara_studio_amd/_lib/libpht_splat.so`` (built by
+``photara_studio_amd.build``). There is no fallback: if the library is missing every entry point
+raises, so no CPU path can stand in for the GPU kernels.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libpht_splat.so"
+
+PHT_OK, PHT_EINVAL, PHT_EDEVICE, PHT_ECAPACITY, PHT_ESTATE, PHT_ENOTSUP = range(6)
+PHT_LP_NONE, PHT_LP_BF16, PHT_LP_FP16 = 0, 1, 2
+PHT_MAX_VIEWS = 32
+
+c_float_p = C.POINTER(C.c_float)
+c_u16_p = C.POINTER(C.c_uint16)
+
+
+class pht_camera(C.Structure):
+    _fields_ = [
+        ("world_to_camera", C.c_float * 16),
+        ("position", C.c_float * 3),
+        ("fx", C.c_float), ("fy", C.c_float), ("cx", C.c_float), ("cy", C.c_float),
+        ("width", C.c_int32), ("height", C.c_int32),
+        ("background", C.c_float * 3),
+        ("reserved", C.c_int32),
+    ]
+
+
+class pht_settings(C.Structure):
+    _fields_ = [
+        ("active_sh_degree", C.c_int32), ("sh_bases", C.c_int32),
+        ("kernel_size", C.c_float), ("scale_modifier", C.c_float),
+        ("alpha_clip", C.c_float), ("near_plane", C.c_float),
+        ("compensate_2d", C.c_int32), ("tile_size", C.c_int32), ("batch", C.c_int32),
+        ("fp64_conic", C.c_int32), ("require_depth", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+
+class pht_gaussians(C.Structure):
+    _fields_ = [
+        ("count", C.c_int64),
+        ("means", C.c_void_p), ("log_scales", C.c_void_p), ("quaternions", C.c_void_p),
+        ("opacity_logits", C.c_void_p), ("sh", C.c_void_p), ("filter_3d", C.c_void_p),
+        ("colors_precomp", C.c_void_p),
+        ("sh_lp", C.c_void_p), ("log_scales_lp", C.c_void_p), ("opacity_logits_lp", C.c_void_p),
+        ("lp_format", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+
+class pht_view_outputs(C.Structure):
+    _fields_ = [("color", C.c_void_p), ("alpha", C.c_void_p), ("radii", C.c_void_p), ("visibility", C.c_void_p)]
+
+
+class pht_view_grads(C.Structure):
+    _fields_ = [("grad_color", C.c_void_p), ("grad_alpha", C.c_void_p)]
+
+
+class pht_model_grads(C.Structure):
+    _fields_ = [
+        ("means", C.c_void_p), ("log_scales", C.c_void_p), ("quaternions", C.c_void_p),
+        ("opacity_logits", C.c_void_p), ("sh", C.c_void_p), ("colors_precomp", C.c_void_p),
+        ("refine_weight", C.c_void_p),
+    ]
+
+
+class pht_adam(C.Structure):
+    _fields_ = [
+        ("means", C.c_void_p), ("log_scales", C.c_void_p), ("quaternions", C.c_void_p),
+        ("opacity_logits", C.c_void_p), ("sh", C.c_void_p),
+        ("m_means", C.c_void_p), ("v_means", C.c_void_p),
+        ("m_log_scales", C.c_void_p), ("v_log_scales", C.c_void_p),
+        ("m_quaternions", C.c_void_p), ("v_quaternions", C.c_void_p),
+        ("m_opacity_logits", C.c_void_p), ("v_opacity_logits", C.c_void_p),
+        ("m_sh", C.c_void_p), ("v_sh", C.c_void_p),
+        ("lr_means", C.c_float), ("lr_scales", C.c_float), ("lr_quaternions", C.c_float),
+        ("lr_opacity", C.c_float), ("lr_sh0", C.c_float), ("lr_sh_rest", C.c_float),
+        ("beta1", C.c_float), ("beta2", C.c_float), ("correction1", C.c_float),
+        ("correction2", C.c_float), ("epsilon", C.c_float),
+        ("min_log_scale", C.c_float), ("max_log_scale", C.c_float),
+        ("device_hparams", C.c_void_p),
+        ("sh_lp", C.c_void_p), ("log_scales_lp", C.c_void_p), ("opacity_logits_lp", C.c_void_p),
+        ("lp_format", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+
+class pht_capacity(C.Structure):
+    _fields_ = [
+        ("gaussians", C.c_int64), ("views", C.c_int32), ("tile_size", C.c_int32),
+        ("tiles", C.c_int64), ("pixels", C.c_int64), ("instances", C.c_int64),
+    ]
+
+
+class pht_counts(C.Structure):
+    _fields_ = [
+        ("visible", C.c_int64), ("instances", C.c_int64), ("fp64_conics", C.c_int64),
+        ("overflow", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+
+# Every symbol the header declares, with its ctypes signature.
+SIGNATURES = {
+    "pht_abi_version": (C.c_int, []),
+    "pht_last_error": (C.c_char_p, []),
+    "pht_device_count": (C.c_int, []),
+    "pht_workspace_bytes": (C.c_size_t, [C.POINTER(pht_capacity)]),
+    "pht_workspace_create": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(pht_capacity), C.POINTER(C.c_void_p)]),
+    "pht_workspace_destroy": (None, [C.c_void_p]),
+    "pht_forward": (C.c_int, [C.c_void_p, C.POINTER(pht_gaussians), C.POINTER(pht_camera), C.c_int,
+                              C.POINTER(pht_settings), C.POINTER(pht_view_outputs), C.c_int, C.c_void_p]),
+    "pht_backward": (C.c_int, [C.c_void_p, C.POINTER(pht_gaussians), C.POINTER(pht_view_grads),
+                               C.POINTER(pht_model_grads), C.POINTER(pht_adam), C.c_void_p]),
+    "pht_l1_loss": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_float, C.c_void_p]),
+    "pht_adam_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_int64,
+                                C.c_float, C.c_float, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_float,
+                                C.c_float, C.c_float, C.c_float, C.c_int32, C.c_void_p]),
+    "pht_get_counts": (C.c_int, [C.c_void_p, C.POINTER(pht_counts), C.c_void_p]),
+    "pht_debug_lists": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_void_p]),
+    "pht_debug_splat_grads": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pht_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
+    "pht_stage_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int]),
+    "pht_step_graph_create": (C.c_int, [C.c_void_p, C.POINTER(pht_gaussians), C.POINTER(pht_camera), C.c_int,
+                                        C.POINTER(pht_settings), C.POINTER(pht_view_outputs),
+                                        C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_void_p,
+                                        C.POINTER(pht_adam), C.c_void_p, C.POINTER(C.c_void_p)]),
+    "pht_step_graph_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "pht_step_graph_destroy": (None, [C.c_void_p]),
+}
+
+_lib = None
+
+
+class PhtError(RuntimeError):
+    pass
+
+
+def load() -> C.CDLL:
+    """Load the kernel library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m photara_studio_amd.build` "
+                "(the HIP kernels are the only implementation of this path)")
+        lib = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check(rc: int) -> None:
+    """Map a status code to the reference's exception kinds (invalid_argument -> ValueError,
+    runtime/logic errors -> RuntimeError, unsupported feature -> NotImplementedError)."""
+    if rc == PHT_OK:
+        return
+    msg = load().pht_last_error().decode(errors="replace")
+    if rc == PHT_EINVAL:
+        raise ValueError(msg)
+    if rc == PHT_ENOTSUP:
+        raise NotImplementedError(msg)
+    if rc == PHT_ECAPACITY:
+        raise CapacityError(msg)
+    raise PhtError(msg)
+
+
+class CapacityError(PhtError):
+    pass

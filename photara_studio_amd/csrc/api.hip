@@ -1,0 +1,535 @@
+// C ABI of the gfx950 3DGS training step: workspace carving and launch orchestration.
+//
+// Replaces splat_drender::Rasterizer::{forward, backward} (third_party/splat_drender/src/
+// pipeline.cu:178-312, 457-712) and the CudaRasterizer bridge (photara/src/splat/
+// rasterizer.cu:340-743). Differences by design: every view of a batched step goes through one
+// launch per stage; all counts stay on the device (fixed-capacity buffers + an overflow flag
+// instead of the reference's host readback of the instance count); workspace memory is carved
+// from one caller-owned allocation instead of grow-callbacks; every launch goes to the caller's
+// stream, so the step can be captured as one hipGraph.
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "launch.h"
+
+using namespace pht;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+int check_hip(hipError_t e, const char* what) {
+    if (e != hipSuccess) return fail(PHT_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+    return PHT_OK;
+}
+
+int next_pow2_log2(int n) {
+    int l = 0;
+    while ((1 << l) < n) ++l;
+    return l;
+}
+
+int bits_for(uint64_t max_value) {
+    int b = 0;
+    while (b < 32 && (max_value >> b) != 0) ++b;
+    return b;
+}
+
+struct Carve {
+    char* base;
+    size_t off = 0;
+    template <class T> T* take(size_t count) {
+        off = (off + 255) & ~size_t(255);
+        T* p = reinterpret_cast<T*>(base ? base + off : nullptr);
+        off += count * sizeof(T);
+        return p;
+    }
+};
+
+} // namespace
+
+struct pht_workspace {
+    pht_capacity cap{};
+    int vp_log2 = 0;
+    uint32_t slot_cap = 0, inst_cap = 0;
+    DevView* views = nullptr;
+    Counters* counters = nullptr;
+    float4* rec_a = nullptr;
+    float4* rec_b = nullptr;
+    uint4* rec_c = nullptr;
+    uint32_t* vis_flag = nullptr;
+    float* sgrad = nullptr;
+    uint32_t* view_mask = nullptr;
+    uint32_t* dkeys[2] = {nullptr, nullptr};
+    uint32_t* dvals[2] = {nullptr, nullptr};
+    uint32_t* offsets = nullptr;
+    uint32_t* tkeys[2] = {nullptr, nullptr};
+    uint32_t* tvals[2] = {nullptr, nullptr};
+    uint2* ranges = nullptr;
+    float* final_T = nullptr;
+    uint32_t* n_contrib = nullptr;
+    uint32_t* scan_scratch = nullptr;
+    uint32_t* hist = nullptr;
+    Counters* host_counters = nullptr; // pinned
+    // state of the last forward
+    bool fwd_valid = false;
+    int n_views = 0;
+    int tile_sel = 0;
+    int64_t gaussians = 0;
+    int64_t tiles_total = 0;
+    int64_t pixels_total = 0;
+    pht_settings settings{};
+    std::vector<DevView> host_views;
+    // per-stage timing
+    bool profiling = false;
+    hipEvent_t ev[PHT_STAGES + 1][2] = {};
+    bool ev_recorded[PHT_STAGES] = {};
+
+    void mark(int stage, int edge, hipStream_t st) {
+        if (!profiling) return;
+        if (!ev[stage][edge]) (void)hipEventCreate(&ev[stage][edge]);
+        (void)hipEventRecord(ev[stage][edge], st);
+        if (edge == 1) ev_recorded[stage] = true;
+    }
+};
+
+static void carve_workspace(pht_workspace& w, char* base, const pht_capacity& cap, size_t* total) {
+    Carve c{base};
+    const int vp = 1 << w.vp_log2;
+    const size_t slots = size_t(cap.gaussians) * size_t(vp);
+    const size_t inst = size_t(cap.instances);
+    w.views = c.take<DevView>(PHT_MAX_VIEWS);
+    w.counters = c.take<Counters>(1);
+    w.rec_a = c.take<float4>(slots);
+    w.rec_b = c.take<float4>(slots);
+    w.rec_c = c.take<uint4>(slots);
+    w.vis_flag = c.take<uint32_t>(slots);
+    w.sgrad = c.take<float>(slots * kGradStride);
+    w.view_mask = c.take<uint32_t>(size_t(cap.gaussians));
+    for (int k = 0; k < 2; ++k) {
+        w.dkeys[k] = c.take<uint32_t>(slots);
+        w.dvals[k] = c.take<uint32_t>(slots);
+    }
+    w.offsets = c.take<uint32_t>(slots);
+    for (int k = 0; k < 2; ++k) {
+        w.tkeys[k] = c.take<uint32_t>(inst);
+        w.tvals[k] = c.take<uint32_t>(inst);
+    }
+    w.ranges = c.take<uint2>(size_t(cap.tiles));
+    w.final_T = c.take<float>(size_t(cap.pixels));
+    w.n_contrib = c.take<uint32_t>(size_t(cap.pixels));
+    w.scan_scratch = c.take<uint32_t>(kScanBlocks);
+    w.hist = c.take<uint32_t>(256 * size_t(kScanBlocks));
+    if (total) *total = (c.off + 255) & ~size_t(255);
+}
+
+static int validate_capacity(const pht_capacity* cap) {
+    if (!cap) return fail(PHT_EINVAL, "capacity is null");
+    if (cap->gaussians <= 0 || cap->views <= 0 || cap->views > PHT_MAX_VIEWS || cap->tiles <= 0 ||
+        cap->pixels <= 0 || cap->instances <= 0)
+        return fail(PHT_EINVAL, "capacity fields must be positive (views <= 32)");
+    if (cap->tile_size != 8 && cap->tile_size != 16) return fail(PHT_EINVAL, "tile_size must be 8 or 16");
+    const int vp = 1 << next_pow2_log2(cap->views);
+    if (double(cap->gaussians) * vp >= 2147483647.0) return fail(PHT_EINVAL, "gaussians * views exceeds 2^31");
+    if (cap->instances >= 4294967295LL) return fail(PHT_EINVAL, "instance capacity exceeds 2^32-1");
+    if (cap->tiles >= 2147483647LL || cap->pixels >= 4294967295LL) return fail(PHT_EINVAL, "tiles/pixels too large");
+    return PHT_OK;
+}
+
+extern "C" {
+
+int pht_abi_version(void) { return PHT_ABI_VERSION; }
+
+const char* pht_last_error(void) { return g_last_error.c_str(); }
+
+int pht_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+size_t pht_workspace_bytes(const pht_capacity* cap) {
+    if (validate_capacity(cap) != PHT_OK) return 0;
+    pht_workspace w;
+    w.vp_log2 = next_pow2_log2(cap->views);
+    size_t total = 0;
+    carve_workspace(w, nullptr, *cap, &total);
+    return total;
+}
+
+int pht_workspace_create(void* device_bytes, size_t nbytes, const pht_capacity* cap, pht_workspace** out) {
+    if (!out) return fail(PHT_EINVAL, "out is null");
+    *out = nullptr;
+    if (int r = validate_capacity(cap)) return r;
+    if (!device_bytes) return fail(PHT_EINVAL, "device_bytes is null");
+    if ((reinterpret_cast<uintptr_t>(device_bytes) & 255u) != 0) return fail(PHT_EINVAL, "workspace must be 256-byte aligned");
+    pht_workspace* w = new pht_workspace();
+    w->cap = *cap;
+    w->vp_log2 = next_pow2_log2(cap->views);
+    w->slot_cap = uint32_t(cap->gaussians << w->vp_log2);
+    w->inst_cap = uint32_t(cap->instances);
+    size_t need = 0;
+    carve_workspace(*w, static_cast<char*>(device_bytes), *cap, &need);
+    if (nbytes < need) {
+        delete w;
+        return fail(PHT_EINVAL, "workspace too small: need " + std::to_string(need) + " bytes");
+    }
+    if (hipHostMalloc(reinterpret_cast<void**>(&w->host_counters), sizeof(Counters), hipHostMallocDefault) !=
+        hipSuccess) {
+        delete w;
+        return fail(PHT_EDEVICE, "hipHostMalloc failed for the counter readback buffer");
+    }
+    std::memset(w->host_counters, 0, sizeof(Counters));
+    *out = w;
+    return PHT_OK;
+}
+
+void pht_workspace_destroy(pht_workspace* ws) {
+    if (!ws) return;
+    if (ws->host_counters) (void)hipHostFree(ws->host_counters);
+    for (auto& pair : ws->ev)
+        for (hipEvent_t& e : pair)
+            if (e) (void)hipEventDestroy(e);
+    delete ws;
+}
+
+int pht_forward(pht_workspace* ws, const pht_gaussians* g, const pht_camera* cams, int n_views,
+                const pht_settings* s, pht_view_outputs* out, int check_overflow, void* stream) {
+    if (!ws || !g || !cams || !s || !out) return fail(PHT_EINVAL, "null argument");
+    ws->fwd_valid = false;
+    if (s->require_depth)
+        return fail(PHT_ENOTSUP, "median-depth / normal channels are not built (use require_depth = 0)");
+    if (g->count <= 0) return fail(PHT_EINVAL, "empty model");
+    if (g->count > ws->cap.gaussians) return fail(PHT_EINVAL, "more gaussians than the workspace capacity");
+    if (n_views <= 0 || n_views > ws->cap.views) return fail(PHT_EINVAL, "view count outside the workspace capacity");
+    if (s->tile_size != ws->cap.tile_size) return fail(PHT_EINVAL, "tile_size differs from the workspace's");
+    if (s->active_sh_degree < 0 || s->active_sh_degree > 3) return fail(PHT_EINVAL, "active_sh_degree must be 0..3");
+    if (!g->means || !g->quaternions) return fail(PHT_EINVAL, "means and quaternions are required");
+    const bool lp = g->lp_format != PHT_LP_NONE;
+    if (lp && g->lp_format != PHT_LP_BF16 && g->lp_format != PHT_LP_FP16) return fail(PHT_EINVAL, "bad lp_format");
+    if (lp ? (!g->log_scales_lp || !g->opacity_logits_lp) : (!g->log_scales || !g->opacity_logits))
+        return fail(PHT_EINVAL, "log_scales / opacity_logits are required");
+    if (!g->colors_precomp) {
+        if (lp ? !g->sh_lp : !g->sh) return fail(PHT_EINVAL, "sh (or colors_precomp) is required");
+        if ((s->active_sh_degree + 1) * (s->active_sh_degree + 1) > s->sh_bases)
+            return fail(PHT_EINVAL, "active_sh_degree needs more SH bases than the model stores");
+    }
+    if (s->batch != 32 && s->batch != 64 && s->batch != 128 && s->batch != 256)
+        return fail(PHT_EINVAL, "batch must be 32, 64, 128 or 256");
+    if (s->tile_size == 8 && s->batch > 128) return fail(PHT_EINVAL, "8x8 tiles support batch <= 128");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    // per-view tables
+    std::vector<DevView> hv(static_cast<size_t>(n_views));
+    int64_t tiles = 0, pixels = 0;
+    const int ts = s->tile_size;
+    for (int v = 0; v < n_views; ++v) {
+        const pht_camera& c = cams[v];
+        if (c.width <= 0 || c.height <= 0) return fail(PHT_EINVAL, "camera dimensions must be positive");
+        if (!out[v].color || !out[v].alpha) return fail(PHT_EINVAL, "color and alpha outputs are required");
+        DevView& d = hv[size_t(v)];
+        std::memset(&d, 0, sizeof(d));
+        d.vi.cam = c;
+        d.vi.grid_x = (c.width + ts - 1) / ts;
+        d.vi.grid_y = (c.height + ts - 1) / ts;
+        d.vi.tile_base = int32_t(tiles);
+        d.vi.tile_count = d.vi.grid_x * d.vi.grid_y;
+        d.vi.pixel_base = pixels;
+        d.color = out[v].color;
+        d.alpha = out[v].alpha;
+        d.radii = out[v].radii;
+        d.visibility = out[v].visibility;
+        tiles += d.vi.tile_count;
+        pixels += int64_t(c.width) * c.height;
+    }
+    if (tiles > ws->cap.tiles || pixels > ws->cap.pixels)
+        return fail(PHT_EINVAL, "views exceed the workspace's tile/pixel capacity");
+
+    launch_upload_views(ws->views, hv.data(), n_views, st);
+    ws->mark(0, 0, st);
+    launch_clear(ws->counters, ws->ranges, uint32_t(tiles), st);
+    PreArgs pa;
+    pa.g = *g;
+    pa.s = *s;
+    pa.views = ws->views;
+    pa.n_views = n_views;
+    pa.vp_log2 = ws->vp_log2;
+    pa.rec_a = ws->rec_a;
+    pa.rec_b = ws->rec_b;
+    pa.rec_c = ws->rec_c;
+    pa.vis_flag = ws->vis_flag;
+    pa.sgrad = ws->sgrad;
+    pa.view_mask = ws->view_mask;
+    pa.counters = ws->counters;
+    const int64_t n_slots = g->count << ws->vp_log2;
+    launch_preprocess(pa, n_slots, st);
+    ws->mark(0, 1, st);
+    ws->mark(1, 0, st);
+    launch_compact_visible(ws->vis_flag, ws->rec_c, uint32_t(n_slots), ws->dkeys[0], ws->dvals[0], ws->scan_scratch,
+                           &ws->counters->visible, st);
+    const int dsel = launch_radix_sort(ws->dkeys, ws->dvals, &ws->counters->visible, uint32_t(n_slots), 32, ws->hist, st);
+    ws->mark(1, 1, st);
+    ws->mark(2, 0, st);
+    launch_tile_offsets(ws->dvals[dsel], ws->rec_c, &ws->counters->visible, uint32_t(n_slots), ws->offsets,
+                        ws->scan_scratch, &ws->counters->instances, &ws->counters->overflow, ws->inst_cap, st);
+    launch_emit_instances(ws->dvals[dsel], ws->offsets, ws->counters, uint32_t(n_slots), ws->rec_a, ws->rec_b,
+                          ws->rec_c, ws->views, ws->vp_log2, ts, ws->inst_cap, ws->tkeys[0], ws->tvals[0], st);
+    ws->mark(2, 1, st);
+    ws->mark(3, 0, st);
+    const int tbits = bits_for(uint64_t(tiles - 1));
+    const int tsel = launch_radix_sort(ws->tkeys, ws->tvals, &ws->counters->instances, ws->inst_cap, tbits, ws->hist, st);
+    launch_ranges(ws->tkeys[tsel], ws->counters, ws->inst_cap, uint32_t(tiles), ws->ranges, st);
+    ws->mark(3, 1, st);
+    BlendArgs ba;
+    ba.ranges = ws->ranges;
+    ba.inst_vals = ws->tvals[tsel];
+    ba.rec_a = ws->rec_a;
+    ba.rec_b = ws->rec_b;
+    ba.rec_c = ws->rec_c;
+    ba.views = ws->views;
+    ba.n_views = n_views;
+    ba.vp_log2 = ws->vp_log2;
+    ba.alpha_clip = s->alpha_clip;
+    ba.final_T = ws->final_T;
+    ba.n_contrib = ws->n_contrib;
+    ba.sgrad = ws->sgrad;
+    ba.counters = ws->counters;
+    ba.slot_cap = ws->slot_cap;
+    ws->mark(4, 0, st);
+    launch_blend_forward(ba, int(tiles), ts, s->batch, st);
+    ws->mark(4, 1, st);
+    if (int r = check_hip(hipGetLastError(), "forward launch")) return r;
+
+    ws->fwd_valid = true;
+    ws->n_views = n_views;
+    ws->tile_sel = tsel;
+    ws->gaussians = g->count;
+    ws->tiles_total = tiles;
+    ws->pixels_total = pixels;
+    ws->settings = *s;
+    ws->host_views = hv;
+    if (check_overflow) {
+        if (int r = check_hip(hipMemcpyAsync(ws->host_counters, ws->counters, sizeof(Counters), hipMemcpyDeviceToHost, st),
+                              "counter readback"))
+            return r;
+        if (int r = check_hip(hipStreamSynchronize(st), "forward synchronise")) return r;
+        if (ws->host_counters->overflow) {
+            ws->fwd_valid = false;
+            return fail(PHT_ECAPACITY, "instance list (" + std::to_string(ws->host_counters->instances) +
+                                           ") exceeds the workspace capacity (" + std::to_string(ws->inst_cap) + ")");
+        }
+    }
+    return PHT_OK;
+}
+
+int pht_backward(pht_workspace* ws, const pht_gaussians* g, const pht_view_grads* dl, pht_model_grads* grads,
+                 const pht_adam* adam, void* stream) {
+    if (!ws || !g || !dl) return fail(PHT_EINVAL, "null argument");
+    if (!ws->fwd_valid) return fail(PHT_ESTATE, "backward requires a completed forward on this workspace");
+    if (g->count != ws->gaussians) return fail(PHT_EINVAL, "model size differs from the forward's");
+    if (!grads && !adam) return fail(PHT_EINVAL, "either gradients or an Adam update are required");
+    if (adam) {
+        if (!adam->means || !adam->log_scales || !adam->quaternions || !adam->opacity_logits || !adam->m_means ||
+            !adam->v_means || !adam->m_log_scales || !adam->v_log_scales || !adam->m_quaternions ||
+            !adam->v_quaternions || !adam->m_opacity_logits || !adam->v_opacity_logits)
+            return fail(PHT_EINVAL, "fused Adam requires every structure parameter and moment");
+        if (adam->sh && (!adam->m_sh || !adam->v_sh)) return fail(PHT_EINVAL, "SH Adam requires its moments");
+        if (adam->lp_format != g->lp_format) return fail(PHT_EINVAL, "Adam lp_format differs from the model's");
+    }
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    std::vector<DevView> hv = ws->host_views;
+    for (int v = 0; v < ws->n_views; ++v) {
+        if (!dl[v].grad_color) return fail(PHT_EINVAL, "grad_color is required for every view");
+        hv[size_t(v)].grad_color = dl[v].grad_color;
+        hv[size_t(v)].grad_alpha = dl[v].grad_alpha;
+    }
+    launch_upload_views(ws->views, hv.data(), ws->n_views, st);
+    BlendArgs ba;
+    ba.ranges = ws->ranges;
+    ba.inst_vals = ws->tvals[ws->tile_sel];
+    ba.rec_a = ws->rec_a;
+    ba.rec_b = ws->rec_b;
+    ba.rec_c = ws->rec_c;
+    ba.views = ws->views;
+    ba.n_views = ws->n_views;
+    ba.vp_log2 = ws->vp_log2;
+    ba.alpha_clip = ws->settings.alpha_clip;
+    ba.final_T = ws->final_T;
+    ba.n_contrib = ws->n_contrib;
+    ba.sgrad = ws->sgrad;
+    ba.counters = ws->counters;
+    ba.slot_cap = ws->slot_cap;
+    ws->mark(5, 0, st);
+    launch_blend_backward(ba, int(ws->tiles_total), ws->settings.tile_size, ws->settings.batch, st);
+    ws->mark(5, 1, st);
+    GaussArgs ga;
+    ga.g = *g;
+    ga.s = ws->settings;
+    ga.views = ws->views;
+    ga.n_views = ws->n_views;
+    ga.vp_log2 = ws->vp_log2;
+    ga.rec_c = ws->rec_c;
+    ga.sgrad = ws->sgrad;
+    ga.view_mask = ws->view_mask;
+    ga.counters = ws->counters;
+    std::memset(&ga.out, 0, sizeof(ga.out));
+    if (grads) ga.out = *grads;
+    std::memset(&ga.adam, 0, sizeof(ga.adam));
+    if (adam) ga.adam = *adam;
+    ws->mark(6, 0, st);
+    launch_gaussian_backward(ga, adam != nullptr, st);
+    ws->mark(6, 1, st);
+    return check_hip(hipGetLastError(), "backward launch");
+}
+
+int pht_set_profiling(pht_workspace* ws, int enable) {
+    if (!ws) return fail(PHT_EINVAL, "null workspace");
+    ws->profiling = enable != 0;
+    for (bool& r : ws->ev_recorded) r = false;
+    return PHT_OK;
+}
+
+int pht_stage_times(pht_workspace* ws, float* ms_out, int max_stages) {
+    if (!ws || !ms_out) return fail(PHT_EINVAL, "null argument");
+    const int n = max_stages < PHT_STAGES ? max_stages : PHT_STAGES;
+    for (int s = 0; s < n; ++s) {
+        ms_out[s] = -1.0f;
+        if (!ws->ev_recorded[s]) continue;
+        if (int r = check_hip(hipEventSynchronize(ws->ev[s][1]), "event synchronise")) return r;
+        float ms = 0.0f;
+        if (int r = check_hip(hipEventElapsedTime(&ms, ws->ev[s][0], ws->ev[s][1]), "event elapsed")) return r;
+        ms_out[s] = ms;
+    }
+    return n;
+}
+
+int pht_l1_loss(const float* color, const float* target, float* grad_color, float* loss_sum, int64_t pixels,
+                float weight, void* stream) {
+    if (!color || !target || !grad_color || pixels <= 0) return fail(PHT_EINVAL, "bad L1 arguments");
+    launch_l1_loss(color, target, grad_color, loss_sum, pixels, weight, static_cast<hipStream_t>(stream));
+    return check_hip(hipGetLastError(), "l1 launch");
+}
+
+int pht_adam_step(float* param, const float* grad, float* m, float* v, int64_t rows, int64_t row_stride,
+                  int64_t active_cols, float lr, float lr_secondary, int64_t secondary_from_col, float beta1,
+                  float beta2, float correction1, float correction2, float epsilon, float clamp_min, float clamp_max,
+                  int32_t prefix_semantics, void* stream) {
+    if (!param || !grad || !m || !v || rows < 0 || row_stride <= 0 || active_cols < 0 || active_cols > row_stride)
+        return fail(PHT_EINVAL, "bad Adam arguments");
+    launch_adam_standalone(param, grad, m, v, rows, row_stride, active_cols, lr, lr_secondary, secondary_from_col,
+                           beta1, beta2, correction1, correction2, epsilon, clamp_min, clamp_max, prefix_semantics,
+                           static_cast<hipStream_t>(stream));
+    return check_hip(hipGetLastError(), "adam launch");
+}
+
+int pht_get_counts(pht_workspace* ws, pht_counts* out, void* stream) {
+    if (!ws || !out) return fail(PHT_EINVAL, "null argument");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (int r = check_hip(hipMemcpyAsync(ws->host_counters, ws->counters, sizeof(Counters), hipMemcpyDeviceToHost, st),
+                          "counter readback"))
+        return r;
+    if (int r = check_hip(hipStreamSynchronize(st), "counter synchronise")) return r;
+    out->visible = ws->host_counters->visible;
+    out->instances = ws->host_counters->instances;
+    out->fp64_conics = ws->host_counters->fp64;
+    out->overflow = int32_t(ws->host_counters->overflow);
+    out->reserved = 0;
+    return PHT_OK;
+}
+
+int pht_debug_lists(pht_workspace* ws, uint64_t* keys, uint32_t* values, uint32_t* ranges, uint32_t* n_contrib,
+                    float* final_T, void* stream) {
+    if (!ws) return fail(PHT_EINVAL, "null workspace");
+    if (!ws->fwd_valid) return fail(PHT_ESTATE, "no forward recorded");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (keys || values)
+        launch_debug_keys(ws->tkeys[ws->tile_sel], ws->tvals[ws->tile_sel], ws->rec_c, ws->counters, ws->inst_cap,
+                          ws->vp_log2, keys, values, st);
+    if (ranges)
+        if (int r = check_hip(hipMemcpyAsync(ranges, ws->ranges, size_t(ws->tiles_total) * sizeof(uint2),
+                                             hipMemcpyDeviceToDevice, st), "ranges copy"))
+            return r;
+    if (n_contrib)
+        if (int r = check_hip(hipMemcpyAsync(n_contrib, ws->n_contrib, size_t(ws->pixels_total) * 4,
+                                             hipMemcpyDeviceToDevice, st), "n_contrib copy"))
+            return r;
+    if (final_T)
+        if (int r = check_hip(hipMemcpyAsync(final_T, ws->final_T, size_t(ws->pixels_total) * 4,
+                                             hipMemcpyDeviceToDevice, st), "final_T copy"))
+            return r;
+    return check_hip(hipGetLastError(), "debug lists");
+}
+
+int pht_debug_splat_grads(pht_workspace* ws, float* out, void* stream) {
+    if (!ws || !out) return fail(PHT_EINVAL, "null argument");
+    if (!ws->fwd_valid) return fail(PHT_ESTATE, "no forward recorded");
+    launch_debug_splat_grads(ws->sgrad, ws->view_mask, ws->gaussians, ws->n_views, ws->vp_log2, out,
+                             static_cast<hipStream_t>(stream));
+    return check_hip(hipGetLastError(), "debug splat grads");
+}
+
+// ---------------------------------------------------------------- whole-step graph
+struct pht_step_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+int pht_step_graph_create(pht_workspace* ws, const pht_gaussians* g, const pht_camera* cams, int n_views,
+                          const pht_settings* s, pht_view_outputs* out, const float* const* targets,
+                          float* const* grad_color_bufs, float* loss_sum, const pht_adam* adam, void* stream,
+                          pht_step_graph** out_graph) {
+    if (!out_graph || !targets || !grad_color_bufs || !adam) return fail(PHT_EINVAL, "null argument");
+    *out_graph = nullptr;
+    hipStream_t cap_stream = nullptr;
+    if (int r = check_hip(hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking), "capture stream")) return r;
+    (void)stream;
+    pht_step_graph* sg = new pht_step_graph();
+    const bool was_profiling = ws && ws->profiling;
+    if (ws) ws->profiling = false;
+    int rc = check_hip(hipStreamBeginCapture(cap_stream, hipStreamCaptureModeThreadLocal), "begin capture");
+    if (rc == PHT_OK) {
+        if (loss_sum) launch_zero_f32(loss_sum, 1, cap_stream);
+        rc = pht_forward(ws, g, cams, n_views, s, out, 0, cap_stream);
+        std::vector<pht_view_grads> dl(static_cast<size_t>(n_views));
+        for (int v = 0; rc == PHT_OK && v < n_views; ++v) {
+            const int64_t P = int64_t(cams[v].width) * cams[v].height;
+            rc = pht_l1_loss(out[v].color, targets[v], grad_color_bufs[v], loss_sum, P, 1.0f, cap_stream);
+            dl[size_t(v)].grad_color = grad_color_bufs[v];
+            dl[size_t(v)].grad_alpha = nullptr;
+        }
+        if (rc == PHT_OK) rc = pht_backward(ws, g, dl.data(), nullptr, adam, cap_stream);
+        hipGraph_t graph = nullptr;
+        const int rc_end = check_hip(hipStreamEndCapture(cap_stream, &graph), "end capture");
+        if (rc == PHT_OK) rc = rc_end;
+        sg->graph = graph;
+        if (rc == PHT_OK) rc = check_hip(hipGraphInstantiate(&sg->exec, graph, nullptr, nullptr, 0), "instantiate");
+    }
+    (void)hipStreamDestroy(cap_stream);
+    if (ws) ws->profiling = was_profiling;
+    if (rc != PHT_OK) {
+        pht_step_graph_destroy(sg);
+        return rc;
+    }
+    *out_graph = sg;
+    return PHT_OK;
+}
+
+int pht_step_graph_launch(pht_step_graph* graph, void* stream) {
+    if (!graph || !graph->exec) return fail(PHT_EINVAL, "null graph");
+    return check_hip(hipGraphLaunch(graph->exec, static_cast<hipStream_t>(stream)), "graph launch");
+}
+
+void pht_step_graph_destroy(pht_step_graph* graph) {
+    if (!graph) return;
+    if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
+    if (graph->graph) (void)hipGraphDestroy(graph->graph);
+    delete graph;
+}
+
+} // extern "C"

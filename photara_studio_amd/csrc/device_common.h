@@ -1,0 +1,92 @@
+// Device-side shared definitions for the gfx950 kernels: workspace tables, wave64 helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "splat_math.h"
+
+namespace pht {
+
+// Per-view table the kernels read: geometry + this call's caller-owned output/input pointers.
+struct DevView {
+    ViewInfo vi;
+    float* color;          // [3,H,W]
+    float* alpha;          // [H,W]
+    int32_t* radii;        // [N] or null
+    float* visibility;     // [N] or null
+    const float* grad_color; // [3,H,W] (backward)
+    const float* grad_alpha; // [H,W] or null (backward)
+};
+
+// Device counters, cleared at the start of every forward.
+struct Counters {
+    uint32_t visible;     // compacted visible splats
+    uint32_t instances;   // emitted instances (exact, may exceed capacity)
+    uint32_t overflow;    // instances > capacity
+    uint32_t fp64;        // FP64-routed conics
+    uint32_t scan_total;  // scratch total of the last chunked scan
+    uint32_t pad[3];
+};
+
+// Splat record (one per (gaussian, view) slot; slot = g * vp + v):
+//   a = (mean.x, mean.y, conic.x, conic.y)   b = (conic.z, opacity, r, g)
+//   c = (bits(b), depth_key, tiles, flags)   flags: bits 0-2 SH clamp, bit 3 fp64 conic
+// Splat gradient (12 floats): dmean.xy, dconic.xyz, dopacity, dcolor.rgb, refine, 2 spare.
+constexpr int kGradStride = 12;
+
+// Fixed block count of the chunked scans / radix-sort passes (each block owns one contiguous
+// chunk of the list, so the passes are stable and need no device-wide synchronisation).
+constexpr int kScanBlocks = 1024;
+constexpr int kScanThreads = 256;
+
+// ---------------------------------------------------------------- wave64 helpers
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_move(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL,
+                                                                  ROW_MASK, 0xf, false));
+}
+
+// Sum over the 64 lanes of a wave; every lane must be active. The total lands in lane 63.
+// quad_perm xor1, xor2, row_half_mirror, row_mirror, row_bcast15 (rows 1,3), row_bcast31 (rows 2,3).
+__device__ __forceinline__ float wave_sum_to_last(float v) {
+    v += dpp_move<0xb1, 0xf>(v);
+    v += dpp_move<0x4e, 0xf>(v);
+    v += dpp_move<0x141, 0xf>(v);
+    v += dpp_move<0x140, 0xf>(v);
+    v += dpp_move<0x142, 0xa>(v);
+    v += dpp_move<0x143, 0xc>(v);
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t w = __shfl_xor(v, o, 64);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+// Inclusive prefix sum over the wave (Hillis-Steele on shuffles).
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
+    const int lane = lane_id();
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t w = __shfl_up(v, o, 64);
+        if (lane >= o) v += w;
+    }
+    return v;
+}
+
+__device__ __forceinline__ void atomic_add_f32(float* p, float v) { unsafeAtomicAdd(p, v); }
+
+__device__ __forceinline__ const DevView& view_of_tile(const DevView* views, int n_views, int tile,
+                                                       int& v_out) {
+    int v = 0;
+    while (v + 1 < n_views && views[v + 1].vi.tile_base <= tile) ++v;
+    v_out = v;
+    return views[v];
+}
+
+} // namespace pht

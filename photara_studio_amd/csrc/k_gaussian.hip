@@ -1,0 +1,218 @@
+// Per-Gaussian backward (+ fused Adam), standalone Adam and the L1 loss gradient for gfx950.
+//
+// k_gaussian_bwd runs one thread per Gaussian and loops over the views of the batched step in
+// lockstep (view constants stay wave-uniform): for each view the Gaussian was visible in, it
+// chains the splat's blend gradients through the EWA projection and SH colour
+// (gaussian_backward, third_party/splat_drender/src/render_backward.cu:708-832), sums over views,
+// applies the activation chain (update_structure_adam_row, :650-678) and then either writes the
+// raw-parameter gradients or updates every parameter with Adam in the same pass
+// (adam_scalar :525-550 for structure, the active-prefix SH Adam of cuda_ops.cu:1506-1546),
+// including rows culled in every view, whose moments still decay. With reduced-precision storage
+// the bf16/fp16 mirrors read by the next forward are rewritten from the FP32 masters here.
+#include "launch.h"
+
+namespace pht {
+
+template <int LP> __device__ __forceinline__ float ld_lp(const float* f, const uint16_t* lp, int64_t i) {
+    if constexpr (LP == PHT_LP_NONE) return f[i];
+    else return lp_to_f32(lp[i], LP);
+}
+
+template <int DEG, int LP, bool ADAM>
+__global__ __launch_bounds__(256) void k_gaussian_bwd(GaussArgs a) {
+    constexpr int NB = (DEG + 1) * (DEG + 1);
+    const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= a.g.count) return;
+    if constexpr (ADAM) {
+        if (a.counters->overflow) return;
+    }
+    const float mean[3] = {a.g.means[3 * i], a.g.means[3 * i + 1], a.g.means[3 * i + 2]};
+    float ls[3], q[4];
+    for (int k = 0; k < 3; ++k) ls[k] = ld_lp<LP>(a.g.log_scales, a.g.log_scales_lp, 3 * i + k);
+    for (int k = 0; k < 4; ++k) q[k] = a.g.quaternions[4 * i + k];
+    const float logit = ld_lp<LP>(a.g.opacity_logits, a.g.opacity_logits_lp, i);
+    const bool hf = a.g.filter_3d != nullptr;
+    const float filt = hf ? a.g.filter_3d[i] : 0.0f;
+    const Activated<float> act = activate<float>(ls, q, logit, hf, filt);
+    const uint32_t mask = a.view_mask[i];
+    const bool precomp = a.g.colors_precomp != nullptr;
+
+    SplatGrad<float> sg;
+    for (int k = 0; k < 3; ++k) { sg.mean[k] = 0.0f; sg.scale[k] = 0.0f; }
+    for (int k = 0; k < 4; ++k) sg.rot[k] = 0.0f;
+    sg.opacity = 0.0f;
+    float gsh[NB * 3];
+    float coef[NB * 3];
+#pragma unroll
+    for (int k = 0; k < NB * 3; ++k) gsh[k] = 0.0f;
+    const int64_t row = i * int64_t(a.s.sh_bases) * 3;
+    if (!precomp && mask != 0) {
+#pragma unroll
+        for (int k = 0; k < NB * 3; ++k) coef[k] = ld_lp<LP>(a.g.sh, a.g.sh_lp, row + k);
+    }
+    float gcol[3] = {0.0f, 0.0f, 0.0f};
+    float refine = 0.0f;
+    for (int v = 0; v < a.n_views; ++v) {
+        if (!((mask >> v) & 1u)) continue;
+        const int64_t slot = (i << a.vp_log2) + v;
+        const float4* gp = reinterpret_cast<const float4*>(a.sgrad + slot * kGradStride);
+        const float4 g0 = gp[0], g1 = gp[1], g2 = gp[2];
+        const float dconic[4] = {g0.z, g0.w, g1.x, g1.y};
+        const float dm2[2] = {g0.x, g0.y};
+        const pht_camera& cam = a.views[v].vi.cam;
+        splat_backward_pinhole(mean, act, cam, a.s, dconic, dm2, sg);
+        const float dcol[3] = {g1.z, g1.w, g2.x};
+        if (precomp) {
+            gcol[0] += dcol[0]; gcol[1] += dcol[1]; gcol[2] += dcol[2];
+        } else {
+            float d[3], raw[3];
+            view_dir(mean, cam.position, d, raw);
+            sh_backward(DEG, raw, d, coef, a.rec_c[slot].w & 7u, dcol, gsh, sg.mean);
+        }
+        refine += g2.y;
+    }
+    float gls[3], gq[4], glogit;
+    chain_raw(act, hf, filt, sg, gls, gq, glogit);
+    if (a.out.refine_weight) a.out.refine_weight[i] = refine;
+    if constexpr (!ADAM) {
+        const pht_model_grads& o = a.out;
+        for (int k = 0; k < 3; ++k) {
+            if (o.means) o.means[3 * i + k] = sg.mean[k];
+            if (o.log_scales) o.log_scales[3 * i + k] = gls[k];
+            if (o.colors_precomp) o.colors_precomp[3 * i + k] = gcol[k];
+        }
+        for (int k = 0; k < 4; ++k)
+            if (o.quaternions) o.quaternions[4 * i + k] = gq[k];
+        if (o.opacity_logits) o.opacity_logits[i] = glogit;
+        if (o.sh) {
+#pragma unroll
+            for (int k = 0; k < NB * 3; ++k) o.sh[row + k] = gsh[k];
+            for (int k = NB * 3; k < a.s.sh_bases * 3; ++k) o.sh[row + k] = 0.0f;
+        }
+    } else {
+        const pht_adam& A = a.adam;
+        const float inf = __builtin_huge_valf();
+        float c1 = A.correction1, c2 = A.correction2, lr_means = A.lr_means;
+        if (A.device_hparams) {
+            c1 = A.device_hparams[0];
+            c2 = A.device_hparams[1];
+            lr_means = A.device_hparams[2];
+        }
+        for (int k = 0; k < 3; ++k) {
+            const int64_t j = 3 * i + k;
+            adam_update(A.means[j], A.m_means[j], A.v_means[j], sg.mean[k], lr_means, A.beta1, A.beta2, c1, c2,
+                        A.epsilon, -inf, inf, false);
+            adam_update(A.log_scales[j], A.m_log_scales[j], A.v_log_scales[j], gls[k], A.lr_scales, A.beta1,
+                        A.beta2, c1, c2, A.epsilon, A.min_log_scale, A.max_log_scale, false);
+        }
+        for (int k = 0; k < 4; ++k) {
+            const int64_t j = 4 * i + k;
+            adam_update(A.quaternions[j], A.m_quaternions[j], A.v_quaternions[j], gq[k], A.lr_quaternions,
+                        A.beta1, A.beta2, c1, c2, A.epsilon, -inf, inf, false);
+        }
+        adam_update(A.opacity_logits[i], A.m_opacity_logits[i], A.v_opacity_logits[i], glogit, A.lr_opacity,
+                    A.beta1, A.beta2, c1, c2, A.epsilon, -12.0f, 12.0f, false);
+        if (A.sh && !precomp) {
+#pragma unroll
+            for (int k = 0; k < NB * 3; ++k) {
+                const int64_t j = row + k;
+                adam_update(A.sh[j], A.m_sh[j], A.v_sh[j], gsh[k], k < 3 ? A.lr_sh0 : A.lr_sh_rest, A.beta1,
+                            A.beta2, c1, c2, A.epsilon, -inf, inf, true);
+            }
+        }
+        if constexpr (LP != PHT_LP_NONE) {
+            for (int k = 0; k < 3; ++k)
+                if (A.log_scales_lp) A.log_scales_lp[3 * i + k] = f32_to_lp(A.log_scales[3 * i + k], LP);
+            if (A.opacity_logits_lp) A.opacity_logits_lp[i] = f32_to_lp(A.opacity_logits[i], LP);
+            if (A.sh_lp && A.sh && !precomp) {
+#pragma unroll
+                for (int k = 0; k < NB * 3; ++k) A.sh_lp[row + k] = f32_to_lp(A.sh[row + k], LP);
+            }
+        }
+    }
+}
+
+template <int DEG, int LP> static void gb_launch(const GaussArgs& a, bool adam, hipStream_t st) {
+    const unsigned blocks = unsigned((a.g.count + 255) / 256);
+    if (adam) hipLaunchKernelGGL((k_gaussian_bwd<DEG, LP, true>), dim3(blocks), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((k_gaussian_bwd<DEG, LP, false>), dim3(blocks), dim3(256), 0, st, a);
+}
+template <int LP> static void gb_deg(const GaussArgs& a, bool adam, hipStream_t st) {
+    switch (a.s.active_sh_degree) {
+    case 0: gb_launch<0, LP>(a, adam, st); break;
+    case 1: gb_launch<1, LP>(a, adam, st); break;
+    case 2: gb_launch<2, LP>(a, adam, st); break;
+    default: gb_launch<3, LP>(a, adam, st); break;
+    }
+}
+void launch_gaussian_backward(const GaussArgs& a, bool adam, hipStream_t st) {
+    if (a.g.count <= 0) return;
+    if (a.g.lp_format == PHT_LP_BF16) gb_deg<PHT_LP_BF16>(a, adam, st);
+    else if (a.g.lp_format == PHT_LP_FP16) gb_deg<PHT_LP_FP16>(a, adam, st);
+    else gb_deg<PHT_LP_NONE>(a, adam, st);
+}
+
+// ---------------------------------------------------------------- standalone Adam
+__global__ __launch_bounds__(256) void k_adam(float* p, const float* g, float* m, float* v, int64_t rows,
+                                              int64_t stride, int64_t active, float lr, float lr2,
+                                              int64_t secondary_from, float b1, float b2, float c1, float c2,
+                                              float eps, float lo, float hi, int prefix) {
+    const int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (t >= rows * active) return;
+    const int64_t r = t / active, c = t % active;
+    const int64_t j = r * stride + c;
+    const float lrx = (secondary_from > 0 && c >= secondary_from) ? lr2 : lr;
+    adam_update(p[j], m[j], v[j], g[j], lrx, b1, b2, c1, c2, eps, lo, hi, prefix != 0);
+}
+
+void launch_adam_standalone(float* p, const float* g, float* m, float* v, int64_t rows, int64_t stride,
+                            int64_t active, float lr, float lr2, int64_t secondary_from, float b1, float b2,
+                            float c1, float c2, float eps, float lo, float hi, int prefix, hipStream_t st) {
+    const int64_t n = rows * active;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_adam, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, p, g, m, v, rows, stride, active,
+                       lr, lr2, secondary_from, b1, b2, c1, c2, eps, lo, hi, prefix);
+}
+
+// ---------------------------------------------------------------- L1 loss
+// loss_kernel L1 part (photara/src/splat/cuda_ops.cu:886-898), no mask, full image.
+__global__ __launch_bounds__(256) void k_l1(const float* color, const float* target, float* grad, float* loss_sum,
+                                            int64_t pixels, float weight) {
+    __shared__ float s_w[4];
+    const int64_t p = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    const float inv_p = 1.0f / float(pixels);
+    const float scale = (weight * inv_p) / 3.0f;
+    float part = 0.0f;
+    if (p < pixels) {
+        for (int ch = 0; ch < 3; ++ch) {
+            const int64_t o = ch * pixels + p;
+            const float d = color[o] - target[o];
+            part += fabsf(d);
+            grad[o] = scale * float((d > 0.0f) - (d < 0.0f));
+        }
+    }
+    if (loss_sum) {
+        part = wave_sum_to_last(part);
+        if (lane_id() == 63) s_w[threadIdx.x >> 6] = part;
+        __syncthreads();
+        if (threadIdx.x == 0) atomic_add_f32(loss_sum, ((s_w[0] + s_w[1]) + (s_w[2] + s_w[3])) * scale);
+    }
+}
+
+__global__ void k_zero_f32(float* p, int n) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
+}
+
+void launch_l1_loss(const float* color, const float* target, float* grad, float* loss_sum, int64_t pixels,
+                    float weight, hipStream_t st) {
+    if (pixels <= 0) return;
+    hipLaunchKernelGGL(k_l1, dim3(unsigned((pixels + 255) / 256)), dim3(256), 0, st, color, target, grad, loss_sum,
+                       pixels, weight);
+}
+
+void launch_zero_f32(float* p, int n, hipStream_t st) {
+    hipLaunchKernelGGL(k_zero_f32, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, p, n);
+}
+
+} // namespace pht

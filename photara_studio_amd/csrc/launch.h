@@ -1,0 +1,97 @@
+// Host-side launchers shared between the kernel translation units and the C-ABI layer.
+#pragma once
+
+#include "device_common.h"
+
+namespace pht {
+
+// Arguments of the per-(gaussian, view) preprocess.
+struct PreArgs {
+    pht_gaussians g;
+    pht_settings s;
+    const DevView* views;
+    int n_views;
+    int vp_log2;
+    float4* rec_a;
+    float4* rec_b;
+    uint4* rec_c;
+    uint32_t* vis_flag;
+    float* sgrad;
+    uint32_t* view_mask;
+    Counters* counters;
+};
+
+struct BlendArgs {
+    const uint2* ranges;
+    const uint32_t* inst_vals;
+    const float4* rec_a;
+    const float4* rec_b;
+    const uint4* rec_c;
+    const DevView* views;
+    int n_views;
+    int vp_log2;
+    float alpha_clip;
+    float* final_T;
+    uint32_t* n_contrib;
+    float* sgrad;
+    const Counters* counters;
+    uint32_t slot_cap;
+};
+
+struct GaussArgs {
+    pht_gaussians g;
+    pht_settings s;
+    const DevView* views;
+    int n_views;
+    int vp_log2;
+    const uint4* rec_c;
+    const float* sgrad;
+    const uint32_t* view_mask;
+    const Counters* counters;
+    pht_model_grads out;
+    pht_adam adam;
+};
+
+// k_sort.hip
+int scan_blocks_for(uint32_t capacity);
+void launch_compact_visible(const uint32_t* flags, const uint4* rec_c, uint32_t n_slots, uint32_t* keys,
+                            uint32_t* vals, uint32_t* block_scratch, uint32_t* visible_out, hipStream_t st);
+void launch_tile_offsets(const uint32_t* sorted_slots, const uint4* rec_c, const uint32_t* visible_ptr,
+                         uint32_t slot_cap, uint32_t* offsets, uint32_t* block_scratch,
+                         uint32_t* instances_out, uint32_t* overflow_out, uint32_t inst_cap, hipStream_t st);
+// Sorts (keys, vals) pairs by the low `bits` bits of the keys; returns the buffer index (0/1)
+// holding the result.
+int launch_radix_sort(uint32_t* keys[2], uint32_t* vals[2], const uint32_t* n_ptr, uint32_t cap, int bits,
+                      uint32_t* hist, hipStream_t st);
+
+// k_preprocess.hip
+void launch_upload_views(DevView* dst, const DevView* host_views, int n_views, hipStream_t st);
+void launch_clear(Counters* counters, uint2* ranges, uint32_t n_tiles, hipStream_t st);
+void launch_preprocess(const PreArgs& a, int64_t n_slots, hipStream_t st);
+void launch_emit_instances(const uint32_t* sorted_slots, const uint32_t* offsets, const Counters* counters,
+                           uint32_t slot_cap, const float4* rec_a, const float4* rec_b, const uint4* rec_c,
+                           const DevView* views, int vp_log2, int tile_size, uint32_t inst_cap,
+                           uint32_t* keys_out, uint32_t* vals_out, hipStream_t st);
+void launch_ranges(const uint32_t* keys, const Counters* counters, uint32_t inst_cap, uint32_t n_tiles,
+                   uint2* ranges, hipStream_t st);
+void launch_debug_keys(const uint32_t* tile_keys, const uint32_t* slots, const uint4* rec_c,
+                       const Counters* counters, uint32_t inst_cap, int vp_log2, uint64_t* keys_out,
+                       uint32_t* vals_out, hipStream_t st);
+
+void launch_debug_splat_grads(const float* sgrad, const uint32_t* view_mask, int64_t n, int n_views, int vp_log2,
+                              float* out, hipStream_t st);
+
+// k_blend.hip
+void launch_blend_forward(const BlendArgs& a, int n_tiles, int tile_size, int batch, hipStream_t st);
+void launch_blend_backward(const BlendArgs& a, int n_tiles, int tile_size, int batch, hipStream_t st);
+
+// k_gaussian.hip
+void launch_gaussian_backward(const GaussArgs& a, bool adam, hipStream_t st);
+void launch_adam_standalone(float* p, const float* g, float* m, float* v, int64_t rows, int64_t stride,
+                            int64_t active, float lr, float lr2, int64_t secondary_from, float b1, float b2,
+                            float c1, float c2, float eps, float lo, float hi, int prefix, hipStream_t st);
+void launch_l1_loss(const float* color, const float* target, float* grad, float* loss_sum, int64_t pixels,
+                    float weight, hipStream_t st);
+void launch_zero_f32(float* p, int n, hipStream_t st);
+
+} // namespace pht

@@ -1,0 +1,806 @@
+// splat_math.h — per-element math of the 3DGS training step, shared by the gfx950 kernels
+// (T = float) and the CPU oracle (T = float for bit-exact parity, T = double for accuracy).
+//
+// Determinism contract: every float expression here is evaluated with IEEE-754 single
+// precision, correctly rounded +,-,*,/,sqrt, explicit fmaf where a fused op is wanted, and
+// no contraction (compile with -ffp-contract=off, no fast-math). exp/log are this file's
+// own routines (pexpf/plogf) built only from those operations, so the CPU and the GPU
+// produce identical bits for keys, tile ranges, contributor counts and images.
+//
+// Formulas restate the reference rasterizer (third_party/splat_drender/src/device/*.cuh,
+// src/render_forward.cu, src/render_backward.cu); each function cites what it follows.
+#pragma once
+
+#include <math.h>
+#include <stdint.h>
+
+#include "pht_splat.h"
+
+#if defined(__HIPCC__) || defined(__HIP__)
+#include <hip/hip_runtime.h>
+#define PHT_HD __host__ __device__ inline
+#else
+#define PHT_HD inline
+#endif
+
+namespace pht {
+
+// ---------------------------------------------------------------- constants
+// third_party/splat_drender/include/splat_drender/config.h:35-41
+template <class T> struct K {
+    static constexpr T alpha_floor() { return T(1) / T(255); }
+    static constexpr T transmittance_floor() { return T(1.0e-4); }
+};
+// alpha = min(clip, o * exp(p)) < 1/255 for every o <= 1 once p < -5.6 (exp(-5.6) = 0.0037).
+// Used only to skip the exp of far pixels; the decision it shortcuts is unchanged.
+constexpr float kSkipPower = -5.6f;
+
+// ---------------------------------------------------------------- scalar helpers
+PHT_HD uint32_t f2u(float f) { return __builtin_bit_cast(uint32_t, f); }
+PHT_HD float u2f(uint32_t u) { return __builtin_bit_cast(float, u); }
+
+PHT_HD float fma_(float a, float b, float c) { return fmaf(a, b, c); }
+PHT_HD double fma_(double a, double b, double c) { return fma(a, b, c); }
+PHT_HD float sqrt_(float x) { return sqrtf(x); }
+PHT_HD double sqrt_(double x) { return sqrt(x); }
+PHT_HD float floor_(float x) { return floorf(x); }
+PHT_HD double floor_(double x) { return floor(x); }
+PHT_HD float ceil_(float x) { return ceilf(x); }
+PHT_HD double ceil_(double x) { return ceil(x); }
+PHT_HD float min_(float a, float b) { return fminf(a, b); }
+PHT_HD double min_(double a, double b) { return fmin(a, b); }
+PHT_HD float max_(float a, float b) { return fmaxf(a, b); }
+PHT_HD double max_(double a, double b) { return fmax(a, b); }
+PHT_HD float abs_(float a) { return fabsf(a); }
+PHT_HD double abs_(double a) { return fabs(a); }
+PHT_HD bool finite_(float a) { return (f2u(a) & 0x7f800000u) != 0x7f800000u; }
+PHT_HD bool finite_(double a) { return a - a == 0.0; }
+
+// exp(x), Cody-Waite range reduction x = k*ln2 + r (|r| <= ln2/2) with ln2 split into a
+// 24-bit head and its remainder, then the degree-7 Taylor polynomial of e^r in Horner form
+// (truncation error r^8/8! < 6e-9), then an exact scaling by 2^k. Max error ~1 ulp.
+PHT_HD float pexpf(float x) {
+    if (!(x > -104.0f)) return x != x ? x : 0.0f;
+    if (x > 88.72283935546875f) return u2f(0x7f800000u);
+    const float kf = rintf(x * 1.44269502162933350f);
+    float r = fma_(kf, -0.693147182464599609375f, x);
+    r = fma_(kf, 1.9046542121259336e-09f, r);
+    float p = 1.98412701138295233e-04f;
+    p = fma_(p, r, 1.38888892251998186e-03f);
+    p = fma_(p, r, 8.33333376795053482e-03f);
+    p = fma_(p, r, 4.16666679084300995e-02f);
+    p = fma_(p, r, 1.66666671633720398e-01f);
+    p = fma_(p, r, 0.5f);
+    p = fma_(p, r, 1.0f);
+    p = fma_(p, r, 1.0f);
+    const int k = int(kf);
+    if (k > 127) return (p * u2f(uint32_t(127 + 127) << 23)) * 2.0f;
+    if (k >= -125) return p * u2f(uint32_t(k + 127) << 23);
+    return (p * u2f(uint32_t(k + 127 + 64) << 23)) * 5.42101086242752217e-20f; // 2^-64
+}
+
+// log(x): x = 2^e * m with m in [sqrt(1/2), sqrt(2)), log(m) = 2 atanh(s), s = (m-1)/(m+1),
+// atanh series to s^9 (|s| <= 0.1716, truncation < 1e-9 relative). ~1-2 ulp.
+PHT_HD float plogf(float x) {
+    if (!(x > 0.0f)) return x == 0.0f ? -u2f(0x7f800000u) : u2f(0x7fc00000u);
+    if (x == u2f(0x7f800000u)) return x;
+    uint32_t b = f2u(x);
+    int e = 0;
+    if (b < 0x00800000u) { x = x * 8388608.0f; b = f2u(x); e = -23; }
+    e += int(b >> 23) - 127;
+    float m = u2f((b & 0x007fffffu) | 0x3f800000u);
+    if (m > 1.41421353816986084f) { m = m * 0.5f; e += 1; }
+    const float f = m - 1.0f;
+    const float s = f / (2.0f + f);
+    const float s2 = s * s;
+    float q = 1.11111111938953400e-01f;
+    q = fma_(q, s2, 1.42857149243354797e-01f);
+    q = fma_(q, s2, 2.00000002980232239e-01f);
+    q = fma_(q, s2, 3.33333343267440796e-01f);
+    const float s_2 = 2.0f * s;
+    const float lm = fma_(s_2, s2 * q, s_2);
+    const float ef = float(e);
+    return fma_(ef, 0.693147182464599609375f, fma_(ef, -1.9046542121259336e-09f, lm));
+}
+
+PHT_HD float exp_(float x) { return pexpf(x); }
+PHT_HD double exp_(double x) { return exp(x); }
+PHT_HD float log_(float x) { return plogf(x); }
+PHT_HD double log_(double x) { return log(x); }
+
+// ---------------------------------------------------------------- 3x3 (row-major)
+// third_party/splat_drender/src/device/matrix.cuh:62-193
+template <class T> struct M3 {
+    T m[3][3];
+    PHT_HD T at(int c, int r) const { return m[r][c]; } // column-major accessor
+};
+template <class T> PHT_HD M3<T> mul(const M3<T>& a, const M3<T>& b) {
+    M3<T> o;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            o.m[i][j] = (a.m[i][0] * b.m[0][j] + a.m[i][1] * b.m[1][j]) + a.m[i][2] * b.m[2][j];
+    return o;
+}
+template <class T> PHT_HD M3<T> transpose(const M3<T>& a) {
+    M3<T> o;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) o.m[i][j] = a.m[j][i];
+    return o;
+}
+// Scalar-first quaternion -> rows (matrix.cuh:171-184).
+template <class T> PHT_HD M3<T> quat_rotation(const T q[4]) {
+    const T r = q[0], x = q[1], y = q[2], z = q[3];
+    M3<T> o;
+    o.m[0][0] = T(1) - T(2) * (y * y + z * z);
+    o.m[0][1] = T(2) * (x * y + r * z);
+    o.m[0][2] = T(2) * (x * z - r * y);
+    o.m[1][0] = T(2) * (x * y - r * z);
+    o.m[1][1] = T(1) - T(2) * (x * x + z * z);
+    o.m[1][2] = T(2) * (y * z + r * x);
+    o.m[2][0] = T(2) * (x * z + r * y);
+    o.m[2][1] = T(2) * (y * z - r * x);
+    o.m[2][2] = T(1) - T(2) * (x * x + y * y);
+    return o;
+}
+// Column-major world_to_camera applied to a point (matrix.cuh:142-146).
+template <class T> PHT_HD void xform_point(const T p[3], const float* v, T t[3]) {
+    t[0] = ((T(v[0]) * p[0] + T(v[4]) * p[1]) + T(v[8]) * p[2]) + T(v[12]);
+    t[1] = ((T(v[1]) * p[0] + T(v[5]) * p[1]) + T(v[9]) * p[2]) + T(v[13]);
+    t[2] = ((T(v[2]) * p[0] + T(v[6]) * p[1]) + T(v[10]) * p[2]) + T(v[14]);
+}
+template <class T> PHT_HD void xform_dir_transpose(const T d[3], const float* v, T o[3]) {
+    o[0] = (T(v[0]) * d[0] + T(v[1]) * d[1]) + T(v[2]) * d[2];
+    o[1] = (T(v[4]) * d[0] + T(v[5]) * d[1]) + T(v[6]) * d[2];
+    o[2] = (T(v[8]) * d[0] + T(v[9]) * d[1]) + T(v[10]) * d[2];
+}
+template <class T> PHT_HD M3<T> world_rotation_transposed(const float* v) {
+    M3<T> w;
+    w.m[0][0] = T(v[0]); w.m[0][1] = T(v[1]); w.m[0][2] = T(v[2]);
+    w.m[1][0] = T(v[4]); w.m[1][1] = T(v[5]); w.m[1][2] = T(v[6]);
+    w.m[2][0] = T(v[8]); w.m[2][1] = T(v[9]); w.m[2][2] = T(v[10]);
+    return w;
+}
+
+// ---------------------------------------------------------------- activation
+// activate_row (third_party/splat_drender/src/device/activation.cuh:13-31) ==
+// activate_kernel (photara/src/splat/cuda_ops.cu:208-239).
+template <class T> struct Activated {
+    T scale[3];     // filtered scale sqrt(exp(ls)^2 + f^2)
+    T raw_scale[3]; // exp(ls)
+    T rot[4];       // normalised quaternion
+    T inv_norm;     // 1 / max(|q|, 1e-10)
+    T sigmoid;      // sigmoid(logit)
+    T opacity;      // sigmoid * prod(raw/filtered)
+};
+template <class T>
+PHT_HD Activated<T> activate(const T ls[3], const T q[4], T logit, bool has_filter, T filter) {
+    Activated<T> a;
+    T ratio = T(1);
+    const T f2 = has_filter ? filter * filter : T(0);
+    for (int k = 0; k < 3; ++k) {
+        const T v = exp_(ls[k]);
+        a.raw_scale[k] = v;
+        if (has_filter) {
+            a.scale[k] = sqrt_(v * v + f2);
+            ratio = ratio * (v / a.scale[k]);
+        } else {
+            a.scale[k] = v;
+        }
+    }
+    const T n2 = ((q[0] * q[0] + q[1] * q[1]) + q[2] * q[2]) + q[3] * q[3];
+    const T inv = T(1) / sqrt_(max_(n2, T(1e-20)));
+    a.inv_norm = inv;
+    for (int k = 0; k < 4; ++k) a.rot[k] = q[k] * inv;
+    a.sigmoid = T(1) / (T(1) + exp_(-logit));
+    a.opacity = a.sigmoid * ratio;
+    return a;
+}
+
+// ---------------------------------------------------------------- camera (internal)
+// Per-view constants as kernels see them: pht_camera plus the derived tile grid and the
+// view's offsets into the global tile / pixel arrays of a batched launch.
+struct ViewInfo {
+    pht_camera cam;
+    int32_t grid_x, grid_y;
+    int32_t tile_base;  // first global tile id of this view
+    int32_t tile_count;
+    int64_t pixel_base; // first global pixel of this view
+};
+
+// ---------------------------------------------------------------- screen footprint cap
+// footprint::policy / apply / conic_backward (third_party/splat_drender/src/device/footprint.cuh:13-48).
+// The eigen-radius uses sqrt(h^2 + b^2) where the reference calls hypotf.
+template <class T> struct Cap { T q, lambda, da, db, dc; };
+template <class T> PHT_HD Cap<T> cap_policy(T a, T b, T c, int width, int height) {
+    const T half = T(0.5) * (a - c);
+    const T radius = sqrt_(half * half + b * b);
+    const T lambda = (T(0.5) * a + T(0.5) * c) + radius;
+    const int mn = width < height ? width : height;
+    const T limit = min_(T(1024), T(mn));
+    const T variance = (T(0.125) * limit) * limit;
+    Cap<T> cp;
+    cp.lambda = lambda;
+    if (!(lambda > variance) || !finite_(lambda)) {
+        cp.q = T(1); cp.da = T(0); cp.db = T(0); cp.dc = T(0);
+        return cp;
+    }
+    const T h = radius > T(0) ? half / radius : T(0);
+    cp.q = variance / lambda;
+    cp.da = T(0.5) + T(0.5) * h;
+    cp.db = radius > T(0) ? b / radius : T(0);
+    cp.dc = T(0.5) - T(0.5) * h;
+    return cp;
+}
+template <class T>
+PHT_HD void conic_backward_capped(T a, T b, T c, const Cap<T>& cap, const T dconic[3], T out[3]) {
+    const T x = a * cap.q, y = b * cap.q, z = c * cap.q;
+    const T det = x * z - y * y;
+    const T u = z / det, v = -y / det, w = x / det;
+    const T ga = -((u * u * dconic[0] + T(2) * u * v * dconic[1]) + v * v * dconic[2]);
+    const T gb = T(-2) * ((u * v * dconic[0] + (u * w + v * v) * dconic[1]) + v * w * dconic[2]);
+    const T gc = -((v * v * dconic[0] + T(2) * v * w * dconic[1]) + w * w * dconic[2]);
+    const T pull = ((ga * a + gb * b) + gc * c) / cap.lambda;
+    out[0] = cap.q * (ga - pull * cap.da);
+    out[1] = cap.q * (gb - pull * cap.db);
+    out[2] = cap.q * (gc - pull * cap.dc);
+}
+
+// ---------------------------------------------------------------- EWA projection (pinhole)
+// geo::project_splat pinhole branch (geometry.cuh:127-163,178-198,248-253) fused with the
+// conic / radius part of preprocess_gaussians (render_forward.cu:69-83).
+// Screen covariance: M = S R (W J), cov = M^T M (only the 2x2 block is live since J's third
+// column is zero), + kernel_size on the diagonal, capped, then inverted.
+template <class T> struct Projected {
+    T mean2d[2];
+    T conic[3];
+    T opacity;   // activated opacity * compensation
+    T depth;     // |t| (radial distance)
+    int radius;
+    bool fp64;   // 2x2 inverse went through the FP64 route
+};
+
+// Relative determinant below which the float 2x2 inverse loses more than 8 bits to
+// cancellation; such splats are inverted in FP64 when fp64_conic is set.
+constexpr float kNearSingular = 1.0f / 256.0f;
+
+template <class T>
+PHT_HD bool project_pinhole(const T mean[3], const Activated<T>& act, const pht_camera& cam,
+                            const pht_settings& s, Projected<T>& out) {
+    T t[3];
+    xform_point(mean, cam.world_to_camera, t);
+    if (!(t[2] > T(s.near_plane))) return false; // camera.h:241-246
+    const T fx = T(cam.fx), fy = T(cam.fy);
+    const T limx = T(1.3) * (T(cam.width) / (T(2) * fx));
+    const T limy = T(1.3) * (T(cam.height) / (T(2) * fy));
+    const T u = t[0] / t[2], v = t[1] / t[2];
+    const T txc = min_(limx, max_(-limx, u)) * t[2];
+    const T tyc = min_(limy, max_(-limy, v)) * t[2];
+    const T tz2 = t[2] * t[2];
+    const T j00 = fx / t[2], j11 = fy / t[2];
+    const T j20 = -(fx * txc) / tz2, j21 = -(fy * tyc) / tz2;
+    // T = W J: row i = (W[i][0] J00 + W[i][2] J20, W[i][1] J11 + W[i][2] J21, 0)
+    const M3<T> W = world_rotation_transposed<T>(cam.world_to_camera);
+    T tm[3][2];
+    for (int i = 0; i < 3; ++i) {
+        tm[i][0] = W.m[i][0] * j00 + W.m[i][2] * j20;
+        tm[i][1] = W.m[i][1] * j11 + W.m[i][2] * j21;
+    }
+    const M3<T> R = quat_rotation<T>(act.rot);
+    const T sm = T(s.scale_modifier);
+    T mm[3][2]; // M = (S R) T
+    for (int i = 0; i < 3; ++i) {
+        const T si = sm * act.scale[i];
+        const T r0 = si * R.m[i][0], r1 = si * R.m[i][1], r2 = si * R.m[i][2];
+        mm[i][0] = (r0 * tm[0][0] + r1 * tm[1][0]) + r2 * tm[2][0];
+        mm[i][1] = (r0 * tm[0][1] + r1 * tm[1][1]) + r2 * tm[2][1];
+    }
+    const T c00 = (mm[0][0] * mm[0][0] + mm[1][0] * mm[1][0]) + mm[2][0] * mm[2][0];
+    const T c01 = (mm[0][0] * mm[0][1] + mm[1][0] * mm[1][1]) + mm[2][0] * mm[2][1];
+    const T c11 = (mm[0][1] * mm[0][1] + mm[1][1] * mm[1][1]) + mm[2][1] * mm[2][1];
+    const T ks = T(s.kernel_size);
+    T a = c00 + ks, b = c01, c = c11 + ks;
+    T coef = T(1);
+    if (s.compensate_2d) {
+        const T det0 = max_(T(1.0e-6), c00 * c11 - c01 * c01);
+        const T det1 = max_(T(1.0e-6), a * c - b * b);
+        coef = sqrt_(det0 / det1);
+    }
+    // footprint::apply (footprint.cuh:30-37)
+    const Cap<T> cp = cap_policy(a, b, c, cam.width, cam.height);
+    if (cp.q < T(1)) { a = a * cp.q; b = b * cp.q; c = c * cp.q; }
+    // conic (render_forward.cu:74-79), FP64 for near-singular 2x2 blocks
+    const T ac = a * c;
+    const T det = ac - b * b;
+    if (det == T(0)) return false;
+    out.fp64 = false;
+    if (s.fp64_conic && sizeof(T) == 4 && !(det > T(kNearSingular) * ac)) {
+        const double dd = double(a) * double(c) - double(b) * double(b);
+        if (!(dd > 0.0)) return false;
+        const double di = 1.0 / dd;
+        out.conic[0] = T(double(c) * di);
+        out.conic[1] = T(-double(b) * di);
+        out.conic[2] = T(double(a) * di);
+        out.fp64 = true;
+    } else {
+        const T det_inv = T(1) / det;
+        out.conic[0] = c * det_inv;
+        out.conic[1] = -b * det_inv;
+        out.conic[2] = a * det_inv;
+    }
+    out.opacity = act.opacity * coef;
+    // radius (render_forward.cu:81-83)
+    const T mid = T(0.5) * (a + c);
+    const T root = sqrt_(max_(T(0.1), mid * mid - det));
+    out.radius = int(ceil_(T(3) * sqrt_(max_(mid + root, mid - root))));
+    // pixel centre from the unclamped point (camera.h:101-116)
+    const T iz = T(1) / t[2];
+    out.mean2d[0] = (t[0] * iz) * fx + T(cam.cx);
+    out.mean2d[1] = (t[1] * iz) * fy + T(cam.cy);
+    out.depth = sqrt_((t[0] * t[0] + t[1] * t[1]) + t[2] * t[2]); // render_forward.cu:104
+    return true;
+}
+
+// ---------------------------------------------------------------- tile enumeration
+// Opacity-aware AccuTile footprint (third_party/splat_drender/src/device/tiles.cuh:18-179,
+// non-wrapping path): the ellipse where alpha falls to 1/255, walked along the shorter screen
+// axis with an exact per-slice interval. emit(tx, ty) is called in walk order.
+template <class T, class Emit>
+PHT_HD uint32_t enumerate_tiles(T mx, T my, const T conic[3], T op, int grid_x, int grid_y, int ts,
+                                Emit&& emit) {
+    const T a = conic[0], b = conic[1], c = conic[2];
+    const T det = a * c - b * b;
+    if (!(a > T(0) && c > T(0)) || !(op >= K<T>::alpha_floor()) || !(det > T(0))) return 0;
+    const T threshold = T(2) * log_(op * T(255));
+    if (!(threshold >= T(0))) return 0;
+    const T ex = sqrt_(threshold * c / det);
+    const T ey = sqrt_(threshold * a / det);
+    const T tsf = T(ts);
+    const bool iterate_x = (T(2) * ex) / tsf <= (T(2) * ey) / tsf;
+    const T cu = iterate_x ? mx : my;
+    const T cv = iterate_x ? my : mx;
+    const T au = iterate_x ? a : c;
+    const T av = iterate_x ? c : a;
+    const int gu = iterate_x ? grid_x : grid_y;
+    const int gv = iterate_x ? grid_y : grid_x;
+    const T eu = sqrt_(threshold * av / det);
+    const T ev = sqrt_(threshold * au / det);
+    int u_min = int(floor_((cu - eu) / tsf));
+    u_min = u_min < gu ? u_min : gu;
+    u_min = u_min > 0 ? u_min : 0;
+    int u_max = int(floor_((cu + eu) / tsf)) + 1;
+    u_max = u_max < gu ? u_max : gu;
+    if (u_min >= u_max) return 0;
+    const T arg_v_min = cu + b * ev / au;
+    const T arg_v_max = cu - b * ev / au;
+    uint32_t count = 0;
+    for (int tu = u_min; tu < u_max; ++tu) {
+        const T s0 = max_(cu - eu, T(tu) * tsf);
+        const T s1 = min_(cu + eu, T(tu + 1) * tsf);
+        const T d0 = s0 - cu, d1 = s1 - cu;
+        const T r0 = sqrt_(max_(T(0), av * threshold - det * d0 * d0));
+        const T r1 = sqrt_(max_(T(0), av * threshold - det * d1 * d1));
+        const T lo0 = cv + (-b * d0 - r0) / av;
+        const T lo1 = cv + (-b * d1 - r1) / av;
+        const T hi0 = cv + (-b * d0 + r0) / av;
+        const T hi1 = cv + (-b * d1 + r1) / av;
+        const T sv_min = (arg_v_min >= s0 && arg_v_min <= s1) ? cv - ev : min_(lo0, lo1);
+        const T sv_max = (arg_v_max >= s0 && arg_v_max <= s1) ? cv + ev : max_(hi0, hi1);
+        int v_min = int(floor_(sv_min / tsf));
+        v_min = v_min < gv ? v_min : gv;
+        v_min = v_min > 0 ? v_min : 0;
+        int v_max = int(floor_(sv_max / tsf)) + 1;
+        v_max = v_max < gv ? v_max : gv;
+        for (int tv = v_min; tv < v_max; ++tv) {
+            emit(iterate_x ? tu : tv, iterate_x ? tv : tu);
+            ++count;
+        }
+    }
+    return count;
+}
+
+// ---------------------------------------------------------------- spherical harmonics
+// Real SH up to degree 3 (third_party/splat_drender/src/device/sh.cuh:8-65): the standard
+// normalisation constants, rgb = max(sum_k Y_k c_k + 0.5, 0) per channel.
+constexpr double kShC0 = 0.28209479177387814;
+constexpr double kShC1 = 0.4886025119029199;
+constexpr double kShC2[5] = {1.0925484305920792, -1.0925484305920792, 0.31539156525252005,
+                             -1.0925484305920792, 0.5462742152960396};
+constexpr double kShC3[7] = {-0.5900435899266435, 2.890611442640554, -0.4570457994644658,
+                             0.3731763325901154, -0.4570457994644658, 1.445305721320277,
+                             -0.5900435899266435};
+
+PHT_HD int sh_count(int degree) { return (degree + 1) * (degree + 1); }
+
+// Unit view direction from the camera centre (sh.cuh:20-23).
+template <class T> PHT_HD void view_dir(const T mean[3], const float pos[3], T d[3], T raw[3]) {
+    raw[0] = mean[0] - T(pos[0]);
+    raw[1] = mean[1] - T(pos[1]);
+    raw[2] = mean[2] - T(pos[2]);
+    const T inv = T(1) / sqrt_((raw[0] * raw[0] + raw[1] * raw[1]) + raw[2] * raw[2]);
+    d[0] = raw[0] * inv; d[1] = raw[1] * inv; d[2] = raw[2] * inv;
+}
+
+template <class T> PHT_HD void sh_basis(int degree, const T d[3], T Y[16]) {
+    const T x = d[0], y = d[1], z = d[2];
+    Y[0] = T(kShC0);
+    if (degree < 1) return;
+    Y[1] = -T(kShC1) * y;
+    Y[2] = T(kShC1) * z;
+    Y[3] = -T(kShC1) * x;
+    if (degree < 2) return;
+    const T xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
+    Y[4] = T(kShC2[0]) * xy;
+    Y[5] = T(kShC2[1]) * yz;
+    Y[6] = T(kShC2[2]) * ((T(2) * zz - xx) - yy);
+    Y[7] = T(kShC2[3]) * xz;
+    Y[8] = T(kShC2[4]) * (xx - yy);
+    if (degree < 3) return;
+    Y[9] = T(kShC3[0]) * (y * (T(3) * xx - yy));
+    Y[10] = T(kShC3[1]) * (xy * z);
+    Y[11] = T(kShC3[2]) * (y * ((T(4) * zz - xx) - yy));
+    Y[12] = T(kShC3[3]) * (z * ((T(2) * zz - T(3) * xx) - T(3) * yy));
+    Y[13] = T(kShC3[4]) * (x * ((T(4) * zz - xx) - yy));
+    Y[14] = T(kShC3[5]) * (z * (xx - yy));
+    Y[15] = T(kShC3[6]) * (x * (xx - T(3) * yy));
+}
+
+// coef[k*3 + ch]; writes rgb and the per-channel clamp flags (bit ch set when clamped).
+template <class T>
+PHT_HD void sh_eval(int degree, const T d[3], const T* coef, T rgb[3], uint32_t& clamp_bits) {
+    T Y[16];
+    sh_basis(degree, d, Y);
+    const int nb = sh_count(degree);
+    clamp_bits = 0;
+    for (int ch = 0; ch < 3; ++ch) {
+        T acc = Y[0] * coef[ch];
+        for (int k = 1; k < nb; ++k) acc = fma_(Y[k], coef[k * 3 + ch], acc);
+        acc = acc + T(0.5);
+        if (acc < T(0)) clamp_bits |= 1u << ch;
+        rgb[ch] = max_(acc, T(0));
+    }
+}
+
+// d(unit dir)/d(raw dir) applied to dv (matrix.cuh:187-193).
+template <class T> PHT_HD void dnormalize(const T v[3], const T dv[3], T o[3]) {
+    const T inv = T(1) / sqrt_((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+    const T n[3] = {v[0] * inv, v[1] * inv, v[2] * inv};
+    o[0] = (((T(1) - n[0] * n[0]) * dv[0] - n[1] * n[0] * dv[1]) - n[2] * n[0] * dv[2]) * inv;
+    o[1] = ((-n[0] * n[1] * dv[0] + (T(1) - n[1] * n[1]) * dv[1]) - n[2] * n[1] * dv[2]) * inv;
+    o[2] = ((-n[0] * n[2] * dv[0] - n[1] * n[2] * dv[1]) + (T(1) - n[2] * n[2]) * dv[2]) * inv;
+}
+
+// Backward of sh_eval (sh.cuh:75-171): accumulates dL/dcoef into gsh[k*3+ch] and returns the
+// view-direction contribution to dL/dmean in gmean (added).
+template <class T>
+PHT_HD void sh_backward(int degree, const T raw_dir[3], const T d[3], const T* coef,
+                        uint32_t clamp_bits, const T dl_rgb[3], T* gsh, T gmean[3]) {
+    T dl[3];
+    for (int ch = 0; ch < 3; ++ch) dl[ch] = (clamp_bits >> ch) & 1u ? T(0) : dl_rgb[ch];
+    T Y[16];
+    sh_basis(degree, d, Y);
+    const int nb = sh_count(degree);
+    for (int k = 0; k < nb; ++k)
+        for (int ch = 0; ch < 3; ++ch) gsh[k * 3 + ch] += Y[k] * dl[ch];
+    if (degree < 1) return;
+    const T x = d[0], y = d[1], z = d[2];
+    // dY_k/d(x,y,z) for k >= 1
+    T dYx[16], dYy[16], dYz[16];
+    dYx[1] = T(0); dYy[1] = -T(kShC1); dYz[1] = T(0);
+    dYx[2] = T(0); dYy[2] = T(0); dYz[2] = T(kShC1);
+    dYx[3] = -T(kShC1); dYy[3] = T(0); dYz[3] = T(0);
+    if (degree > 1) {
+        const T c0 = T(kShC2[0]), c1 = T(kShC2[1]), c2 = T(kShC2[2]), c3 = T(kShC2[3]), c4 = T(kShC2[4]);
+        dYx[4] = c0 * y;            dYy[4] = c0 * x;            dYz[4] = T(0);
+        dYx[5] = T(0);              dYy[5] = c1 * z;            dYz[5] = c1 * y;
+        dYx[6] = c2 * (T(-2) * x);  dYy[6] = c2 * (T(-2) * y);  dYz[6] = c2 * (T(4) * z);
+        dYx[7] = c3 * z;            dYy[7] = T(0);              dYz[7] = c3 * x;
+        dYx[8] = c4 * (T(2) * x);   dYy[8] = c4 * (T(-2) * y);  dYz[8] = T(0);
+    }
+    if (degree > 2) {
+        const T xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
+        const T* C = nullptr; (void)C;
+        const T k0 = T(kShC3[0]), k1 = T(kShC3[1]), k2 = T(kShC3[2]), k3 = T(kShC3[3]);
+        const T k4 = T(kShC3[4]), k5 = T(kShC3[5]), k6 = T(kShC3[6]);
+        dYx[9] = k0 * (T(6) * xy);                    dYy[9] = k0 * (T(3) * (xx - yy));
+        dYz[9] = T(0);
+        dYx[10] = k1 * yz;                            dYy[10] = k1 * xz;
+        dYz[10] = k1 * xy;
+        dYx[11] = k2 * (T(-2) * xy);                  dYy[11] = k2 * ((T(-3) * yy + T(4) * zz) - xx);
+        dYz[11] = k2 * (T(8) * yz);
+        dYx[12] = k3 * (T(-6) * xz);                  dYy[12] = k3 * (T(-6) * yz);
+        dYz[12] = k3 * (T(3) * ((T(2) * zz - xx) - yy));
+        dYx[13] = k4 * ((T(-3) * xx + T(4) * zz) - yy); dYy[13] = k4 * (T(-2) * xy);
+        dYz[13] = k4 * (T(8) * xz);
+        dYx[14] = k5 * (T(2) * xz);                   dYy[14] = k5 * (T(-2) * yz);
+        dYz[14] = k5 * (xx - yy);
+        dYx[15] = k6 * (T(3) * (xx - yy));            dYy[15] = k6 * (T(-6) * xy);
+        dYz[15] = T(0);
+    }
+    T dd[3] = {T(0), T(0), T(0)};
+    for (int ch = 0; ch < 3; ++ch) {
+        T sx = T(0), sy = T(0), sz = T(0);
+        for (int k = 1; k < nb; ++k) {
+            const T ck = coef[k * 3 + ch];
+            sx = sx + dYx[k] * ck;
+            sy = sy + dYy[k] * ck;
+            sz = sz + dYz[k] * ck;
+        }
+        dd[0] = dd[0] + sx * dl[ch];
+        dd[1] = dd[1] + sy * dl[ch];
+        dd[2] = dd[2] + sz * dl[ch];
+    }
+    T dm[3];
+    dnormalize(raw_dir, dd, dm);
+    gmean[0] += dm[0]; gmean[1] += dm[1]; gmean[2] += dm[2];
+}
+
+// ---------------------------------------------------------------- per-pixel blend term
+// geo::gaussian_power (geometry.cuh:26-33) with its fixed rounding order.
+template <class T> PHT_HD T gaussian_power(T ca, T cb, T cc, T dx, T dy) {
+    const T xx = ca * dx;
+    const T yy = (cc * dy) * dy;
+    const T xy = (cb * dx) * dy;
+    return T(-0.5) * fma_(dx, xx, yy) - xy;
+}
+
+// Alpha of one (instance, pixel) pair as in blend_tile (render_forward.cu:380-388).
+// Returns false when the pair is skipped (power > 0 or alpha < 1/255); G = exp(power).
+template <class T>
+PHT_HD bool pair_alpha(T ca, T cb, T cc, T op, T dx, T dy, T alpha_clip, T& alpha, T& G) {
+    const T power = gaussian_power(ca, cb, cc, dx, dy);
+    if (power > T(0)) return false;
+    if (power < T(kSkipPower)) return false;
+    G = exp_(power);
+    alpha = min_(alpha_clip, op * G);
+    return !(alpha < K<T>::alpha_floor());
+}
+
+// ---------------------------------------------------------------- per-Gaussian backward
+// Pinhole RGB chain of geo::splat_backward<PinholeRGB=true> (geometry.cuh:422-912 with no
+// ray-plane / normal gradients: dL_dVrk = 0, dL_dr = 0, dL_du = dL_dv = dL_dz = 0),
+// detail::cov3d_backward (:317-379) and add_mean2d_gradient (:385-417).
+// Inputs: blend gradients d_conic (xyz conic, w opacity), d_mean2d. Outputs are added.
+template <class T> struct SplatGrad {
+    T mean[3];
+    T scale[3];   // wrt activated (filtered) scale
+    T rot[4];     // wrt normalised quaternion
+    T opacity;    // wrt activated opacity
+};
+
+template <class T>
+PHT_HD void splat_backward_pinhole(const T mean[3], const Activated<T>& act, const pht_camera& cam,
+                                   const pht_settings& s, const T d_conic[4], const T d_mean2d[2],
+                                   SplatGrad<T>& io) {
+    const float* view = cam.world_to_camera;
+    const T hx = T(cam.fx), hy = T(cam.fy);
+    T t[3];
+    xform_point(mean, view, t);
+    const T limx = T(1.3) * (T(cam.width) / (T(2) * hx));
+    const T limy = T(1.3) * (T(cam.height) / (T(2) * hy));
+    const T u0 = t[0] / t[2], v0 = t[1] / t[2];
+    const T txc = min_(limx, max_(-limx, u0)) * t[2];
+    const T tyc = min_(limy, max_(-limy, v0)) * t[2];
+    const T x_mul = (u0 < -limx || u0 > limx) ? T(0) : T(1);
+    const T y_mul = (v0 < -limy || v0 > limy) ? T(0) : T(1);
+    const T tz2c = t[2] * t[2];
+    M3<T> J;
+    J.m[0][0] = hx / t[2]; J.m[0][1] = T(0); J.m[0][2] = T(0);
+    J.m[1][0] = T(0); J.m[1][1] = hy / t[2]; J.m[1][2] = T(0);
+    J.m[2][0] = -(hx * txc) / tz2c; J.m[2][1] = -(hy * tyc) / tz2c; J.m[2][2] = T(0);
+    const M3<T> W = world_rotation_transposed<T>(view);
+    const M3<T> Tm = mul(W, J);
+    const T sm = T(s.scale_modifier);
+    const T sl[3] = {sm * act.scale[0], sm * act.scale[1], sm * act.scale[2]};
+    const M3<T> R = quat_rotation<T>(act.rot);
+    M3<T> SR;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) SR.m[i][j] = sl[i] * R.m[i][j];
+    const M3<T> M = mul(SR, Tm);
+    const M3<T> cov2D = mul(transpose(M), M);
+    const M3<T> Vrk = mul(transpose(SR), SR);
+    unsigned min_id = 0;
+    if (sl[1] < sl[min_id]) min_id = 1;
+    if (sl[2] < sl[min_id]) min_id = 2;
+
+    const T ks = T(s.kernel_size);
+    T coef = T(1);
+    if (s.compensate_2d) {
+        const T det_0 = max_(T(1.0e-6), cov2D.m[0][0] * cov2D.m[1][1] - cov2D.m[0][1] * cov2D.m[0][1]);
+        const T det_1 = max_(T(1.0e-6), (cov2D.m[0][0] + ks) * (cov2D.m[1][1] + ks) -
+                                            cov2D.m[0][1] * cov2D.m[0][1]);
+        coef = sqrt_(det_0 / det_1);
+    }
+    // conic -> 2D covariance (geometry.cuh:759-809)
+    const T a = cov2D.m[0][0] + ks, b = cov2D.m[0][1], c = cov2D.m[1][1] + ks;
+    const T denom = a * c - b * b;
+    const T denom2inv = T(1) / (denom * denom + T(1.0e-7));
+    T dL_da = T(0), dL_db = T(0), dL_dc = T(0);
+    T dcov[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+    const Cap<T> cp = cap_policy(a, b, c, cam.width, cam.height);
+    if (denom2inv != T(0) || cp.q < T(1)) {
+        dL_da = denom2inv * ((-c * c * d_conic[0] + T(2) * b * c * d_conic[1]) + (denom - a * c) * d_conic[2]);
+        dL_dc = denom2inv * ((-a * a * d_conic[2] + T(2) * a * b * d_conic[1]) + (denom - a * c) * d_conic[0]);
+        dL_db = denom2inv * T(2) * ((b * c * d_conic[0] - (denom + T(2) * b * b) * d_conic[1]) + a * b * d_conic[2]);
+        if (cp.q < T(1)) {
+            T g3[3];
+            conic_backward_capped(a, b, c, cp, d_conic, g3);
+            dL_da = g3[0]; dL_db = g3[1]; dL_dc = g3[2];
+        }
+        io.opacity += d_conic[3] * coef;
+        const T t00 = Tm.at(0, 0), t10 = Tm.at(1, 0), t01 = Tm.at(0, 1), t11 = Tm.at(1, 1);
+        const T t02 = Tm.at(0, 2), t12 = Tm.at(1, 2);
+        dcov[0] = (t00 * t00 * dL_da + t00 * t10 * dL_db) + t10 * t10 * dL_dc;
+        dcov[3] = (t01 * t01 * dL_da + t01 * t11 * dL_db) + t11 * t11 * dL_dc;
+        dcov[5] = (t02 * t02 * dL_da + t02 * t12 * dL_db) + t12 * t12 * dL_dc;
+        dcov[1] = (T(2) * t00 * t01 * dL_da + (t00 * t11 + t01 * t10) * dL_db) + T(2) * t10 * t11 * dL_dc;
+        dcov[2] = (T(2) * t00 * t02 * dL_da + (t00 * t12 + t02 * t10) * dL_db) + T(2) * t10 * t12 * dL_dc;
+        dcov[4] = (T(2) * t02 * t01 * dL_da + (t01 * t12 + t02 * t11) * dL_db) + T(2) * t11 * t12 * dL_dc;
+    }
+
+    // dSigma -> d(scale, rotation) (geometry.cuh:317-379), dL_dr = 0
+    {
+        const T dsig[3][3] = {{dcov[0], T(0.5) * dcov[1], T(0.5) * dcov[2]},
+                              {T(0.5) * dcov[1], dcov[3], T(0.5) * dcov[4]},
+                              {T(0.5) * dcov[2], T(0.5) * dcov[4], dcov[5]}};
+        M3<T> Ms; // S R with the local scales
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) Ms.m[i][j] = sl[i] * R.m[i][j];
+        M3<T> dM;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                T acc = T(0);
+                for (int k = 0; k < 3; ++k) acc = acc + T(2) * Ms.m[i][k] * dsig[k][j];
+                dM.m[i][j] = acc;
+            }
+        M3<T> dMt = transpose(dM);
+        const T gs0 = (R.m[0][0] * dMt.m[0][0] + R.m[0][1] * dMt.m[1][0]) + R.m[0][2] * dMt.m[2][0];
+        const T gs1 = (R.m[1][0] * dMt.m[0][1] + R.m[1][1] * dMt.m[1][1]) + R.m[1][2] * dMt.m[2][1];
+        const T gs2 = (R.m[2][0] * dMt.m[0][2] + R.m[2][1] * dMt.m[1][2]) + R.m[2][2] * dMt.m[2][2];
+        io.scale[0] += gs0 * sm;
+        io.scale[1] += gs1 * sm;
+        io.scale[2] += gs2 * sm;
+        for (int r = 0; r < 3; ++r) {
+            dMt.m[r][0] = dMt.m[r][0] * sl[0];
+            dMt.m[r][1] = dMt.m[r][1] * sl[1];
+            dMt.m[r][2] = dMt.m[r][2] * sl[2];
+        }
+        (void)min_id;
+        const T r_ = act.rot[0], x = act.rot[1], y = act.rot[2], z = act.rot[3];
+        const T a01 = dMt.at(0, 1), a10 = dMt.at(1, 0);
+        const T a02 = dMt.at(0, 2), a20 = dMt.at(2, 0);
+        const T a12 = dMt.at(1, 2), a21 = dMt.at(2, 1);
+        const T a00 = dMt.at(0, 0), a11 = dMt.at(1, 1), a22 = dMt.at(2, 2);
+        io.rot[0] += (T(2) * z * (a01 - a10) + T(2) * y * (a20 - a02)) + T(2) * x * (a12 - a21);
+        io.rot[1] += ((T(2) * y * (a10 + a01) + T(2) * z * (a20 + a02)) + T(2) * r_ * (a12 - a21)) -
+                     T(4) * x * (a22 + a11);
+        io.rot[2] += ((T(2) * x * (a10 + a01) + T(2) * r_ * (a20 - a02)) + T(2) * z * (a12 + a21)) -
+                     T(4) * y * (a22 + a00);
+        io.rot[3] += ((T(2) * r_ * (a01 - a10) + T(2) * x * (a20 + a02)) + T(2) * y * (a12 + a21)) -
+                     T(4) * z * (a11 + a00);
+    }
+
+    // dL/dT -> dL/dJ -> dL/dt (geometry.cuh:826-903, pinhole)
+    const T dT00 = T(2) * ((Tm.at(0, 0) * Vrk.at(0, 0) + Tm.at(0, 1) * Vrk.at(0, 1)) + Tm.at(0, 2) * Vrk.at(0, 2)) * dL_da +
+                   ((Tm.at(1, 0) * Vrk.at(0, 0) + Tm.at(1, 1) * Vrk.at(0, 1)) + Tm.at(1, 2) * Vrk.at(0, 2)) * dL_db;
+    const T dT01 = T(2) * ((Tm.at(0, 0) * Vrk.at(1, 0) + Tm.at(0, 1) * Vrk.at(1, 1)) + Tm.at(0, 2) * Vrk.at(1, 2)) * dL_da +
+                   ((Tm.at(1, 0) * Vrk.at(1, 0) + Tm.at(1, 1) * Vrk.at(1, 1)) + Tm.at(1, 2) * Vrk.at(1, 2)) * dL_db;
+    const T dT02 = T(2) * ((Tm.at(0, 0) * Vrk.at(2, 0) + Tm.at(0, 1) * Vrk.at(2, 1)) + Tm.at(0, 2) * Vrk.at(2, 2)) * dL_da +
+                   ((Tm.at(1, 0) * Vrk.at(2, 0) + Tm.at(1, 1) * Vrk.at(2, 1)) + Tm.at(1, 2) * Vrk.at(2, 2)) * dL_db;
+    const T dT10 = T(2) * ((Tm.at(1, 0) * Vrk.at(0, 0) + Tm.at(1, 1) * Vrk.at(0, 1)) + Tm.at(1, 2) * Vrk.at(0, 2)) * dL_dc +
+                   ((Tm.at(0, 0) * Vrk.at(0, 0) + Tm.at(0, 1) * Vrk.at(0, 1)) + Tm.at(0, 2) * Vrk.at(0, 2)) * dL_db;
+    const T dT11 = T(2) * ((Tm.at(1, 0) * Vrk.at(1, 0) + Tm.at(1, 1) * Vrk.at(1, 1)) + Tm.at(1, 2) * Vrk.at(1, 2)) * dL_dc +
+                   ((Tm.at(0, 0) * Vrk.at(1, 0) + Tm.at(0, 1) * Vrk.at(1, 1)) + Tm.at(0, 2) * Vrk.at(1, 2)) * dL_db;
+    const T dT12 = T(2) * ((Tm.at(1, 0) * Vrk.at(2, 0) + Tm.at(1, 1) * Vrk.at(2, 1)) + Tm.at(1, 2) * Vrk.at(2, 2)) * dL_dc +
+                   ((Tm.at(0, 0) * Vrk.at(2, 0) + Tm.at(0, 1) * Vrk.at(2, 1)) + Tm.at(0, 2) * Vrk.at(2, 2)) * dL_db;
+    const T dJ00 = (W.at(0, 0) * dT00 + W.at(0, 1) * dT01) + W.at(0, 2) * dT02;
+    const T dJ02 = (W.at(2, 0) * dT00 + W.at(2, 1) * dT01) + W.at(2, 2) * dT02;
+    const T dJ11 = (W.at(1, 0) * dT10 + W.at(1, 1) * dT11) + W.at(1, 2) * dT12;
+    const T dJ12 = (W.at(2, 0) * dT10 + W.at(2, 1) * dT11) + W.at(2, 2) * dT12;
+    const T tz = T(1) / t[2];
+    const T tz2 = tz * tz;
+    const T tz3 = tz2 * tz;
+    const T dt[3] = {
+        x_mul * (-hx * tz2 * dJ02),
+        y_mul * (-hy * tz2 * dJ12),
+        ((-hx * tz2 * dJ00 - hy * tz2 * dJ11) + ((T(1) + x_mul) * hx * txc) * tz3 * dJ02) +
+            ((T(1) + y_mul) * hy * tyc) * tz3 * dJ12};
+    T gm[3];
+    xform_dir_transpose(dt, view, gm);
+    io.mean[0] += gm[0]; io.mean[1] += gm[1]; io.mean[2] += gm[2];
+
+    // mean2d -> mean3d (geometry.cuh:385-399), unclamped point
+    {
+        const T rz = T(1) / (t[2] + T(1.0e-7));
+        const T sx = t[0] * rz, sy = t[1] * rz;
+        const T dmx = d_mean2d[0], dmy = d_mean2d[1];
+        io.mean[0] += (hx * (T(view[0]) - sx * T(view[2])) * dmx + hy * (T(view[1]) - sy * T(view[2])) * dmy) * rz;
+        io.mean[1] += (hx * (T(view[4]) - sx * T(view[6])) * dmx + hy * (T(view[5]) - sy * T(view[6])) * dmy) * rz;
+        io.mean[2] += (hx * (T(view[8]) - sx * T(view[10])) * dmx + hy * (T(view[9]) - sy * T(view[10])) * dmy) * rz;
+    }
+}
+
+// Activation chain to raw parameters (update_structure_adam_row grad part,
+// render_backward.cu:650-678 == chain_gradient_kernel, cuda_ops.cu:304-348), no regularisers.
+template <class T>
+PHT_HD void chain_raw(const Activated<T>& act, bool has_filter, T filter, const SplatGrad<T>& g,
+                      T g_log_scale[3], T g_quat[4], T& g_logit) {
+    const T f2 = has_filter ? filter * filter : T(0);
+    for (int k = 0; k < 3; ++k) {
+        const T fs = act.scale[k], rs = act.raw_scale[k];
+        g_log_scale[k] = g.scale[k] * rs * rs / fs + g.opacity * act.opacity * f2 / (fs * fs);
+    }
+    const T qdot = ((act.rot[0] * g.rot[0] + act.rot[1] * g.rot[1]) + act.rot[2] * g.rot[2]) + act.rot[3] * g.rot[3];
+    for (int k = 0; k < 4; ++k) g_quat[k] = act.inv_norm * (g.rot[k] - act.rot[k] * qdot);
+    g_logit = g.opacity * act.opacity * (T(1) - act.sigmoid);
+}
+
+// ---------------------------------------------------------------- Adam element
+// adam_element / adam_scalar (photara/src/splat/cuda_ops.cu:1291-1334,
+// render_backward.cu:525-550). prefix_semantics selects the active-prefix SH variant
+// (cuda_ops.cu:1506-1546): no clamping, non-finite moments leave the parameter untouched.
+PHT_HD void adam_update(float& p, float& m, float& v, float g, float lr, float beta1, float beta2,
+                        float c1, float c2, float eps, float lo, float hi, bool prefix_semantics) {
+    const float prev = p;
+    if (!finite_(prev) || !finite_(g)) {
+        m = 0.0f;
+        v = 0.0f;
+        if (prefix_semantics) p = finite_(prev) ? prev : 0.0f;
+        else p = finite_(prev) ? fminf(fmaxf(prev, lo), hi) : fminf(fmaxf(0.0f, lo), hi);
+        return;
+    }
+    const float nm = beta1 * m + (1.0f - beta1) * g;
+    const float nv = beta2 * v + ((1.0f - beta2) * g) * g;
+    if (!finite_(nm) || !finite_(nv)) {
+        m = 0.0f;
+        v = 0.0f;
+        if (!prefix_semantics) p = fminf(fmaxf(prev, lo), hi);
+        return;
+    }
+    m = nm;
+    v = nv;
+    const float cand = prev - (lr * (nm / c1)) / (sqrtf(nv / c2) + eps);
+    const float upd = finite_(cand) ? cand : prev;
+    p = prefix_semantics ? upd : fminf(fmaxf(upd, lo), hi);
+}
+
+// ---------------------------------------------------------------- reduced precision
+PHT_HD float bf16_to_f32(uint16_t h) { return u2f(uint32_t(h) << 16); }
+PHT_HD uint16_t f32_to_bf16(float f) { // round to nearest even, NaN kept quiet
+    uint32_t u = f2u(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return uint16_t((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return uint16_t(u >> 16);
+}
+PHT_HD float fp16_to_f32(uint16_t h) {
+    const uint32_t sign = uint32_t(h & 0x8000u) << 16;
+    uint32_t e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
+    if (e == 0x1fu) return u2f(sign | 0x7f800000u | (m << 13));
+    if (e == 0) {
+        if (m == 0) return u2f(sign);
+        const float f = float(m) * 5.9604644775390625e-08f; // 2^-24
+        return sign ? -f : f;
+    }
+    return u2f(sign | ((e + 112u) << 23) | (m << 13));
+}
+PHT_HD uint16_t f32_to_fp16(float f) { // round to nearest even
+    const uint32_t u = f2u(f);
+    const uint32_t sign = (u >> 16) & 0x8000u;
+    const uint32_t au = u & 0x7fffffffu;
+    if (au > 0x7f800000u) return uint16_t(sign | 0x7e00u);
+    if (au >= 0x477ff000u) return uint16_t(sign | 0x7c00u); // >= 65520 rounds to inf
+    if (au < 0x38800000u) { // subnormal half (or zero)
+        const float af = u2f(au) * 16777216.0f; // scale by 2^24: value in half-ulp units
+        const float r = rintf(af);
+        return uint16_t(sign | uint32_t(r));
+    }
+    uint32_t e = (au >> 23) - 112u;
+    uint32_t m = au & 0x7fffffu;
+    uint32_t h = (e << 10) | (m >> 13);
+    const uint32_t rem = m & 0x1fffu;
+    if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) ++h;
+    return uint16_t(sign | h);
+}
+PHT_HD float lp_to_f32(uint16_t h, int fmt) { return fmt == PHT_LP_FP16 ? fp16_to_f32(h) : bf16_to_f32(h); }
+PHT_HD uint16_t f32_to_lp(float f, int fmt) { return fmt == PHT_LP_FP16 ? f32_to_fp16(f) : f32_to_bf16(f); }
+
+} // namespace pht

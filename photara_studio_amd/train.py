@@ -1,0 +1,144 @@
+"""Training-step driver: forward + L1 loss + backward + fused Adam over a set of views.
+
+Reproduces the per-iteration call sequence of the reference trainer (Trainer::train_impl,
+photara/src/splat/trainer.cpp:1922-2491: rasterizer.forward, compute_step_loss, rasterizer.backward
+with the fused structure Adam) for a fixed batch of views, either launched eagerly or replayed as
+one captured hipGraph (the whole step, no host synchronisation inside it).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _abi
+from ._abi import check
+from .rasterizer import (Camera, GaussianModel, RasterizeOptions, StructureAdamUpdate, Workspace,
+                         default_instance_capacity, make_adam, make_camera, make_gaussians, make_settings,
+                         tiles_of, validate_model)
+
+
+@dataclass
+class LearningRates:
+    """TrainingOptions defaults (photara/include/splat/options.hpp:242-258)."""
+    means: float = 1.6e-4
+    scales: float = 5e-3
+    opacities: float = 5e-2
+    quaternions: float = 1e-3
+    sh0: float = 2.5e-3
+    sh_rest: float = 1.25e-4
+
+    @staticmethod
+    def uniform(lr: float) -> "LearningRates":
+        return LearningRates(lr, lr, lr, lr, lr, lr)
+
+
+def to_low_precision(model: GaussianModel, dtype: torch.dtype) -> GaussianModel:
+    """Attach bf16/fp16 copies of SH, log-scales and opacity logits (FP32 masters stay)."""
+    model.sh_lp = model.sh.to(dtype).contiguous()
+    model.log_scales_lp = model.log_scales.to(dtype).contiguous()
+    model.opacity_logits_lp = model.opacity_logits.to(dtype).contiguous()
+    return model
+
+
+class TrainStep:
+    def __init__(self, model: GaussianModel, cameras: Sequence[Camera], targets: Sequence[torch.Tensor],
+                 options: RasterizeOptions, lrs: Optional[LearningRates] = None,
+                 instance_capacity: Optional[int] = None):
+        validate_model(model, options)
+        self.model, self.cameras, self.options = model, list(cameras), options
+        self.targets = [t.contiguous() for t in targets]
+        self.lrs = lrs or LearningRates()
+        dev = model.means.device
+        n, nv = model.size(), len(self.cameras)
+        self.adam = StructureAdamUpdate.zeros_like(
+            model, means_lr=self.lrs.means, scales_lr=self.lrs.scales, quaternions_lr=self.lrs.quaternions,
+            opacities_lr=self.lrs.opacities, sh_lr=self.lrs.sh0, sh_rest_lr=self.lrs.sh_rest)
+        self.cams = (_abi.pht_camera * nv)(*[make_camera(c, options) for c in self.cameras])
+        self.settings = make_settings(model, options)
+        self.gauss = make_gaussians(model, options)
+        ts = int(options.tile_size)
+        self.tiles = tiles_of(self.cameras, ts)
+        self.pixels = sum(int(c.width) * int(c.height) for c in self.cameras)
+        self.capacity = instance_capacity or default_instance_capacity(n, nv)
+        self.ws = Workspace(dev, n, nv, ts, self.tiles, self.pixels, self.capacity)
+        self.colors = [torch.empty((3, c.height, c.width), device=dev) for c in self.cameras]
+        self.alphas = [torch.empty((c.height, c.width), device=dev) for c in self.cameras]
+        self.radii = [torch.empty((n,), dtype=torch.int32, device=dev) for _ in self.cameras]
+        self.visibility = [torch.empty((n,), device=dev) for _ in self.cameras]
+        self.grad_colors = [torch.empty_like(c) for c in self.colors]
+        self.outs = (_abi.pht_view_outputs * nv)()
+        for k in range(nv):
+            self.outs[k].color, self.outs[k].alpha = self.colors[k].data_ptr(), self.alphas[k].data_ptr()
+            self.outs[k].radii, self.outs[k].visibility = self.radii[k].data_ptr(), self.visibility[k].data_ptr()
+        self.dl = (_abi.pht_view_grads * nv)()
+        for k in range(nv):
+            self.dl[k].grad_color = self.grad_colors[k].data_ptr()
+        self.loss_sum = torch.zeros((1,), device=dev)
+        self.hparams = torch.zeros((3,), device=dev)
+        self.step_index = 0
+        self.graph = None
+        self.lib = _abi.load()
+
+    def _corrections(self, step: int):
+        b1, b2 = np.float32(self.adam.beta1), np.float32(self.adam.beta2)
+        c1 = float(np.float32(1.0) - np.power(b1, np.float32(step)))
+        c2 = float(np.float32(1.0) - np.power(b2, np.float32(step)))
+        return c1, c2
+
+    def step_eager(self) -> None:
+        """One iteration launched kernel by kernel on the current stream (no host sync)."""
+        self.step_index += 1
+        c1, c2 = self._corrections(self.step_index)
+        self.adam.correction1, self.adam.correction2 = c1, c2
+        stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        nv = len(self.cameras)
+        check(self.lib.pht_forward(self.ws.handle, C.byref(self.gauss), self.cams, nv, C.byref(self.settings),
+                                   self.outs, 0, stream))
+        self.loss_sum.zero_()
+        for k in range(nv):
+            check(self.lib.pht_l1_loss(C.c_void_p(self.colors[k].data_ptr()), C.c_void_p(self.targets[k].data_ptr()),
+                                       C.c_void_p(self.grad_colors[k].data_ptr()),
+                                       C.c_void_p(self.loss_sum.data_ptr()), self.colors[k][0].numel(), 1.0, stream))
+        adam = make_adam(self.model, self.adam)
+        check(self.lib.pht_backward(self.ws.handle, C.byref(self.gauss), self.dl, None, C.byref(adam), stream))
+
+    def capture(self) -> None:
+        """Capture forward + L1 + backward + Adam as one hipGraph (hyper-parameters on device)."""
+        nv = len(self.cameras)
+        self._adam_graph = make_adam(self.model, self.adam)
+        self._adam_graph.device_hparams = self.hparams.data_ptr()
+        tgt = (C.c_void_p * nv)(*[t.data_ptr() for t in self.targets])
+        grd = (C.c_void_p * nv)(*[g.data_ptr() for g in self.grad_colors])
+        handle = C.c_void_p()
+        stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        check(self.lib.pht_step_graph_create(self.ws.handle, C.byref(self.gauss), self.cams, nv,
+                                             C.byref(self.settings), self.outs, tgt, grd,
+                                             C.c_void_p(self.loss_sum.data_ptr()), C.byref(self._adam_graph), stream,
+                                             C.byref(handle)))
+        self.graph = handle
+
+    def step_graph(self) -> None:
+        if self.graph is None:
+            self.capture()
+        self.step_index += 1
+        c1, c2 = self._corrections(self.step_index)
+        self.hparams.copy_(torch.tensor([c1, c2, self.adam.means_lr], dtype=torch.float32))
+        check(self.lib.pht_step_graph_launch(self.graph, C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+
+    def counts(self) -> _abi.pht_counts:
+        return self.ws.counts()
+
+    def loss(self) -> float:
+        return float(self.loss_sum.item())
+
+    def __del__(self):
+        try:
+            if self.graph is not None:
+                self.lib.pht_step_graph_destroy(self.graph)
+                self.graph = None
+        except Exception:
+            pass

@@ -1,0 +1,251 @@
+"""GPU parity of the HIP training step against the CPU oracle (float32 restatement).
+
+Bar: sort keys, gaussian order, tile ranges, contributor counts, final transmittance, radii and
+visibility bit-exact; images within 1e-5 max-abs (bit-exact expected: both sides evaluate the same
+IEEE float operations in the same order); gradients within 1e-4 relative (max-abs error over the
+tensor's max-abs: the GPU sums per-pixel terms with wave reductions and atomics, the oracle in a
+fixed order).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import small_scene, training_options
+
+pytestmark = pytest.mark.gpu
+
+IMG_TOL = 1e-5
+GRAD_TOL = 1e-4
+
+
+def _gpu_render(arrays, cams, opts, cuda, **kw):
+    from photara_studio_amd.rasterizer import Rasterizer, debug_lists
+    from photara_studio_amd.scenes import model_from
+    model = model_from(arrays, cuda)
+    for k, v in kw.items():
+        setattr(model, k, v)
+    r = Rasterizer()
+    res = r.forward_views(model, cams, opts)
+    lists = debug_lists(res[0])
+    return r, model, res, lists
+
+
+def _assert_forward_equal(res, lists, orc, label=""):
+    assert int(lists["counts"].instances) == int(orc.counts.instances), label
+    assert int(lists["counts"].visible) == int(orc.counts.visible), label
+    assert int(lists["counts"].fp64_conics) == int(orc.counts.fp64_conics), label
+    np.testing.assert_array_equal(lists["keys"], orc.keys, err_msg=f"sort keys {label}")
+    np.testing.assert_array_equal(lists["values"], orc.values, err_msg=f"gaussian order {label}")
+    np.testing.assert_array_equal(lists["ranges"], orc.ranges, err_msg=f"tile ranges {label}")
+    np.testing.assert_array_equal(lists["n_contrib"], orc.n_contrib, err_msg=f"n_contrib {label}")
+    np.testing.assert_array_equal(lists["final_T"], orc.final_T, err_msg=f"final T {label}")
+    for k, r in enumerate(res):
+        col = r.color.cpu().numpy()
+        alp = r.alpha.cpu().numpy()
+        assert np.abs(col - orc.color[k]).max() <= IMG_TOL, label
+        assert np.abs(alp - orc.alpha[k]).max() <= IMG_TOL, label
+        np.testing.assert_array_equal(col, orc.color[k], err_msg=f"color view {k} {label}")
+        np.testing.assert_array_equal(r.radii.cpu().numpy(), orc.radii[k], err_msg=f"radii {label}")
+        np.testing.assert_array_equal(r.visibility.cpu().numpy(), orc.visibility[k], err_msg=f"visibility {label}")
+
+
+def _rel(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+def _grad_images(res, seed=1):
+    rng = np.random.default_rng(seed)
+    return [rng.normal(0, 1e-3, size=tuple(r.color.shape)).astype(np.float32) for r in res]
+
+
+def test_blend_backward_splat_grads(cuda, oracle):
+    """Per-(gaussian, view) blend gradients (the GradState of the reference) vs the oracle."""
+    from photara_studio_amd.rasterizer import debug_splat_grads
+    from photara_studio_amd.scenes import workload_cameras
+    arrays = small_scene(2000, 1.0, 0)
+    cams = workload_cameras("c0")
+    opts = training_options()
+    r, model, res, lists = _gpu_render(arrays, cams, opts, cuda)
+    orc = oracle.OracleRun(oracle.Scene(arrays), cams, opts, 32)
+    gimg = _grad_images(res, 0)
+    r.backward_views(model, res, [torch.from_numpy(g).to(cuda) for g in gimg])
+    orc.backward(gimg)
+    got, want = debug_splat_grads(res[0]), orc.splat_grads()
+    names = ["dmean.x", "dmean.y", "dconic.x", "dconic.y", "dconic.z", "dopacity", "dcol.r", "dcol.g", "dcol.b",
+             "refine"]
+    errs = {}
+    for k, nm in enumerate(names):
+        e = _rel(got[..., k], want[..., k])
+        errs[nm] = e
+        i = int(np.argmax(np.abs(got[..., k] - want[..., k]).reshape(-1)))
+        print(f"  {nm}: rel err {e:.2e} (worst gaussian {i}: gpu {got.reshape(-1, 10)[i, k]:.6e} "
+              f"oracle {want.reshape(-1, 10)[i, k]:.6e})")
+    assert all(e <= GRAD_TOL for e in errs.values()), errs
+
+
+@pytest.mark.parametrize("n,seed", [(2000, 0), (10000, 7)])
+def test_forward_backward_parity_single_view(cuda, oracle, n, seed):
+    from photara_studio_amd.scenes import workload_cameras
+    arrays = small_scene(n, 1.0, seed)
+    cams = workload_cameras("c0")
+    opts = training_options()
+    print(f"[parity] single view n={n} seed={seed}", flush=True)
+    r, model, res, lists = _gpu_render(arrays, cams, opts, cuda)
+    orc = oracle.OracleRun(oracle.Scene(arrays), cams, opts, 32)
+    _assert_forward_equal(res, lists, orc, f"n={n}")
+    gimg = _grad_images(res, seed)
+    galpha = [np.random.default_rng(seed + 5).normal(0, 1e-4, size=tuple(x.alpha.shape)).astype(np.float32) for x in res]
+    grads = r.backward_views(model, res, [torch.from_numpy(g).to(cuda) for g in gimg],
+                             [torch.from_numpy(g).to(cuda) for g in galpha])
+    og = orc.backward(gimg, galpha)
+    errs = {}
+    for name in ("means", "log_scales", "quaternions", "opacity_logits", "sh", "refine_weight"):
+        got = getattr(grads, name).cpu().numpy().reshape(og[name].shape)
+        assert np.isfinite(got).all(), name
+        errs[name] = _rel(got, og[name])
+        print(f"  grad {name}: rel err {errs[name]:.2e}")
+    assert all(e <= GRAD_TOL for e in errs.values()), errs
+
+
+def test_batched_mixed_resolution_views(cuda, oracle):
+    from photara_studio_amd.scenes import look_at
+    arrays = small_scene(3000, 1.0, 3)
+    cams = [look_at((0.0, 0.0, 4.0), width=160, height=96), look_at((3.0, 0.5, 2.5), width=128, height=128),
+            look_at((-2.0, -1.0, 3.0), width=97, height=61)]
+    opts = training_options()
+    print("[parity] batched 3 views", flush=True)
+    r, model, res, lists = _gpu_render(arrays, cams, opts, cuda)
+    orc = oracle.OracleRun(oracle.Scene(arrays), cams, opts, 32)
+    _assert_forward_equal(res, lists, orc, "batched")
+    gimg = _grad_images(res, 11)
+    grads = r.backward_views(model, res, [torch.from_numpy(g).to(cuda) for g in gimg])
+    og = orc.backward(gimg)
+    for name in ("means", "log_scales", "quaternions", "opacity_logits", "sh"):
+        err = _rel(getattr(grads, name).cpu().numpy().reshape(og[name].shape), og[name])
+        assert err <= GRAD_TOL, (name, err)
+    # per-view slices equal single-view renders; summed single-view gradients equal the batched one
+    total = None
+    for k, cam in enumerate(cams):
+        r1, m1, res1, _ = _gpu_render(arrays, [cam], opts, cuda)
+        np.testing.assert_array_equal(res1[0].color.cpu().numpy(), res[k].color.cpu().numpy())
+        g1 = r1.backward_views(m1, res1, [torch.from_numpy(gimg[k]).to(cuda)])
+        total = g1.means.cpu().numpy() if total is None else total + g1.means.cpu().numpy()
+    assert _rel(total, grads.means.cpu().numpy()) <= GRAD_TOL
+
+
+@pytest.mark.parametrize("ts,batch", [(16, 32), (16, 64), (16, 128), (8, 32), (8, 128)])
+def test_tile_and_batch_shapes(cuda, oracle, ts, batch):
+    from photara_studio_amd.scenes import workload_cameras
+    arrays = small_scene(4000, 1.0, 5)
+    cams = workload_cameras("c0")
+    opts = training_options(tile_size=ts, batch=batch)
+    print(f"[parity] tile {ts} batch {batch}", flush=True)
+    r, model, res, lists = _gpu_render(arrays, cams, opts, cuda)
+    orc = oracle.OracleRun(oracle.Scene(arrays), cams, opts, 32)
+    _assert_forward_equal(res, lists, orc, f"ts={ts} batch={batch}")
+    gimg = _grad_images(res, 2)
+    grads = r.backward_views(model, res, [torch.from_numpy(g).to(cuda) for g in gimg])
+    og = orc.backward(gimg)
+    assert _rel(grads.means.cpu().numpy(), og["means"]) <= GRAD_TOL
+    assert _rel(grads.sh.cpu().numpy(), og["sh"]) <= GRAD_TOL
+
+
+@pytest.mark.parametrize("lp", ["bf16", "fp16"])
+def test_reduced_precision_storage(cuda, oracle, lp):
+    from photara_studio_amd.scenes import workload_cameras
+    from photara_studio_amd.train import to_low_precision
+    arrays = small_scene(3000, 1.0, 9)
+    cams = workload_cameras("c0")
+    opts = training_options()
+    print(f"[parity] storage {lp}", flush=True)
+    from photara_studio_amd.rasterizer import Rasterizer, debug_lists
+    from photara_studio_amd.scenes import model_from
+    model = to_low_precision(model_from(arrays, cuda), torch.bfloat16 if lp == "bf16" else torch.float16)
+    r = Rasterizer()
+    res = r.forward_views(model, cams, opts)
+    lists = debug_lists(res[0])
+    orc = oracle.OracleRun(oracle.Scene(arrays, lp=lp), cams, opts, 32)
+    _assert_forward_equal(res, lists, orc, lp)
+    o64 = oracle.OracleRun(oracle.Scene(arrays), cams, opts, 64)
+    err = np.abs(res[0].color.cpu().numpy() - o64.color[0]).max()
+    print(f"  {lp} image vs FP64 oracle of the FP32 parameters: {err:.2e}")
+
+
+def test_l1_loss_parity(cuda, oracle):
+    from photara_studio_amd import _abi
+    import ctypes as C
+    rng = np.random.default_rng(0)
+    color = rng.random((3, 37, 53), dtype=np.float32)
+    target = rng.random((3, 37, 53), dtype=np.float32)
+    target[0, :3, :3] = color[0, :3, :3]  # zero differences -> zero gradient
+    loss, grad = oracle.l1_loss(color, target)
+    c, t = torch.from_numpy(color).to(cuda), torch.from_numpy(target).to(cuda)
+    g = torch.empty_like(c)
+    ls = torch.zeros(1, device=cuda)
+    _abi.check(_abi.load().pht_l1_loss(C.c_void_p(c.data_ptr()), C.c_void_p(t.data_ptr()), C.c_void_p(g.data_ptr()),
+                                       C.c_void_p(ls.data_ptr()), 37 * 53, 1.0,
+                                       C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+    np.testing.assert_array_equal(g.cpu().numpy(), grad)
+    assert abs(float(ls.item()) - loss) <= 1e-5 * max(1.0, loss)
+
+
+def test_standalone_adam_bit_exact(cuda, oracle):
+    """Standalone Adam kernel == oracle Adam on the same gradients, incl. non-finite handling."""
+    from photara_studio_amd import _abi
+    import ctypes as C
+    rows, stride = 257, 48
+    rng = np.random.default_rng(3)
+    p = rng.normal(size=(rows, stride)).astype(np.float32)
+    g = rng.normal(size=(rows, stride)).astype(np.float32) * 0.03
+    g[0, 0], g[5, 7], p[9, 3] = np.nan, np.inf, np.inf
+    m = np.zeros_like(p)
+    v = np.zeros_like(p)
+    for prefix, active, lo, hi in ((0, stride, -8.0, 2.0), (1, 27, -np.inf, np.inf)):
+        pc, mc, vc = p.copy(), m.copy(), v.copy()
+        tp, tg, tm, tv = (torch.from_numpy(x.copy()).to(cuda) for x in (p, g, m, v))
+        for step in (1, 2, 3):
+            c1 = float(np.float32(1) - np.float32(0.9) ** np.float32(step))
+            c2 = float(np.float32(1) - np.float32(0.999) ** np.float32(step))
+            oracle.adam_step(pc, g, mc, vc, rows, stride, active, 1e-3, 1e-4, 3, 0.9, 0.999, c1, c2, 1e-15, lo, hi,
+                             prefix)
+            _abi.check(_abi.load().pht_adam_step(
+                C.c_void_p(tp.data_ptr()), C.c_void_p(tg.data_ptr()), C.c_void_p(tm.data_ptr()),
+                C.c_void_p(tv.data_ptr()), rows, stride, active, 1e-3, 1e-4, 3, 0.9, 0.999, c1, c2, 1e-15, lo, hi,
+                prefix, C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+        np.testing.assert_array_equal(tp.cpu().numpy(), pc)
+        np.testing.assert_array_equal(tm.cpu().numpy(), mc)
+        np.testing.assert_array_equal(tv.cpu().numpy(), vc)
+
+
+def test_capacity_overflow_grows_and_matches(cuda, oracle):
+    from photara_studio_amd.rasterizer import Rasterizer, debug_lists
+    from photara_studio_amd.scenes import model_from, workload_cameras
+    arrays = small_scene(3000, 1.0, 4)
+    cams = workload_cameras("c0")
+    opts = training_options()
+    model = model_from(arrays, cuda)
+    r = Rasterizer()
+    res = r.forward_views(model, cams, opts, instance_capacity=1000)  # too small: grows and re-runs
+    lists = debug_lists(res[0])
+    orc = oracle.OracleRun(oracle.Scene(arrays), cams, opts, 32)
+    _assert_forward_equal(res, lists, orc, "after growth")
+
+
+def test_backward_repeatable_no_stale_state(cuda):
+    """Two forward+backward passes give the same gradients (splat_test.cpp:2480-2496)."""
+    from photara_studio_amd.rasterizer import Rasterizer
+    from photara_studio_amd.scenes import model_from, workload_cameras
+    arrays = small_scene(2000, 1.0, 8)
+    cams = workload_cameras("c0")
+    opts = training_options()
+    model = model_from(arrays, cuda)
+    r = Rasterizer()
+    out = []
+    for _ in range(2):
+        res = r.forward_views(model, cams, opts)
+        gimg = torch.full_like(res[0].color, 1.0 / res[0].color[0].numel())
+        out.append(r.backward_views(model, res, [gimg]).opacity_logits.cpu().numpy())
+    assert np.abs(out[0] - out[1]).max() <= 1e-6 * max(1.0, np.abs(out[0]).max())
