@@ -162,9 +162,12 @@ def main():
 
     # plain FP32 step: one view per step, eager launches
     plain = TrainStep(model_from(arrays, dev), [cams[0]], [targets[0]], opts, LearningRates())
+    plain.ensure_capacity()
     plain_steps = args.plain_steps or 4 * args.steps
     plain_s = timed(plain.step_eager, plain_steps, args.warmup)
     pc = plain.counts()
+    if pc.overflow:
+        raise RuntimeError("plain step overflowed its instance capacity")
     print(f"[bench] plain fp32: {plain_s * 1e3:.3f} ms/step ({pc.instances} instances)", flush=True)
     del plain
     torch.cuda.empty_cache()
@@ -172,10 +175,7 @@ def main():
     # batched BF16 graph step: all views per step
     model = to_low_precision(model_from(arrays, dev), torch.bfloat16)
     batched = TrainStep(model, cams, targets, opts, LearningRates())
-    batched.step_eager()  # first step eagerly: sizes the instance capacity check
-    torch.cuda.synchronize()
-    if batched.counts().overflow:
-        raise RuntimeError("instance capacity exceeded; raise default_instance_capacity")
+    batched.ensure_capacity()
     batched.capture()
     step_s = timed(batched.step_graph, args.steps, args.warmup)
     bc = batched.counts()

@@ -77,11 +77,34 @@ template <class T> T load_in(const float* f, const uint16_t* lp, int fmt, int64_
     return T(f[i]);
 }
 
+// Optional double-precision parameters for the accuracy oracle (finite differences need inputs
+// that are not rounded to float). When set, the double instantiation reads these arrays.
+struct Inputs64 {
+    const double* means = nullptr;
+    const double* log_scales = nullptr;
+    const double* quaternions = nullptr;
+    const double* opacity_logits = nullptr;
+    const double* sh = nullptr;
+};
+Inputs64 g_in64;
+
+template <class T> T mean_in(const pht_gaussians& g, int64_t j) {
+    if (sizeof(T) == 8 && g_in64.means) return T(g_in64.means[j]);
+    return T(g.means[j]);
+}
+
 template <class T> Activated<T> activate_row(const pht_gaussians& g, int64_t i) {
     T ls[3], q[4];
-    for (int k = 0; k < 3; ++k) ls[k] = load_in<T>(g.log_scales, g.log_scales_lp, g.lp_format, 3 * i + k);
-    for (int k = 0; k < 4; ++k) q[k] = T(g.quaternions[4 * i + k]);
-    const T logit = load_in<T>(g.opacity_logits, g.opacity_logits_lp, g.lp_format, i);
+    T logit;
+    if (sizeof(T) == 8 && g_in64.log_scales) {
+        for (int k = 0; k < 3; ++k) ls[k] = T(g_in64.log_scales[3 * i + k]);
+        for (int k = 0; k < 4; ++k) q[k] = T(g_in64.quaternions[4 * i + k]);
+        logit = T(g_in64.opacity_logits[i]);
+    } else {
+        for (int k = 0; k < 3; ++k) ls[k] = load_in<T>(g.log_scales, g.log_scales_lp, g.lp_format, 3 * i + k);
+        for (int k = 0; k < 4; ++k) q[k] = T(g.quaternions[4 * i + k]);
+        logit = load_in<T>(g.opacity_logits, g.opacity_logits_lp, g.lp_format, i);
+    }
     const bool hf = g.filter_3d != nullptr;
     return activate<T>(ls, q, logit, hf, hf ? T(g.filter_3d[i]) : T(0));
 }
@@ -90,7 +113,9 @@ template <class T> void load_sh(const pht_gaussians& g, const pht_settings& s, i
     const int nb = sh_count(s.active_sh_degree);
     for (int k = 0; k < nb; ++k)
         for (int ch = 0; ch < 3; ++ch)
-            coef[k * 3 + ch] = load_in<T>(g.sh, g.sh_lp, g.lp_format, (i * s.sh_bases + k) * 3 + ch);
+            coef[k * 3 + ch] = (sizeof(T) == 8 && g_in64.sh)
+                                   ? T(g_in64.sh[(i * s.sh_bases + k) * 3 + ch])
+                                   : load_in<T>(g.sh, g.sh_lp, g.lp_format, (i * s.sh_bases + k) * 3 + ch);
 }
 
 // Per (gaussian, view) preprocess. Returns true when visible (touches >= 1 tile).
@@ -98,7 +123,7 @@ template <class T>
 bool preprocess_one(const pht_gaussians& g, const pht_settings& s, const ViewInfo& vi, int64_t i,
                     SplatRec<T>& rec, int32_t& radius_out) {
     radius_out = 0;
-    const T mean[3] = {T(g.means[3 * i]), T(g.means[3 * i + 1]), T(g.means[3 * i + 2])};
+    const T mean[3] = {mean_in<T>(g, 3 * i), mean_in<T>(g, 3 * i + 1), mean_in<T>(g, 3 * i + 2)};
     const Activated<T> act = activate_row<T>(g, i);
     Projected<T> p;
     if (!project_pinhole(mean, act, vi.cam, s, p)) return false;
@@ -133,6 +158,8 @@ template <class T> struct State {
     std::vector<uint32_t> ranges;        // [2*tiles]
     std::vector<T> final_T;              // [pixels]
     std::vector<uint32_t> n_contrib;     // [pixels]
+    std::vector<T> color;                // per view [3,H,W], concatenated
+    std::vector<T> alpha;                // per view [H,W], concatenated
     int64_t fp64_count = 0;
 };
 
@@ -199,6 +226,8 @@ void run_forward(const pht_gaussians& g, const pht_camera* cams, int n_views, co
     // Blend, one tile at a time (tiles are independent).
     st.final_T.assign(size_t(vs.pixels), T(1));
     st.n_contrib.assign(size_t(vs.pixels), 0);
+    st.color.assign(size_t(3 * vs.pixels), T(0));
+    st.alpha.assign(size_t(vs.pixels), T(0));
     const T floorT = K<T>::transmittance_floor();
     const T clip = T(s.alpha_clip);
     const int ts = s.tile_size;
@@ -233,6 +262,9 @@ void run_forward(const pht_gaussians& g, const pht_camera* cams, int n_views, co
                 const int64_t pid = int64_t(py) * W + px;
                 st.final_T[size_t(vi.pixel_base + pid)] = Tr;
                 st.n_contrib[size_t(vi.pixel_base + pid)] = last;
+                for (int ch = 0; ch < 3; ++ch)
+                    st.color[size_t(3 * vi.pixel_base + ch * P + pid)] = C[ch] + Tr * T(vi.cam.background[ch]);
+                st.alpha[size_t(vi.pixel_base + pid)] = T(1) - Tr;
                 if (outs) {
                     for (int ch = 0; ch < 3; ++ch)
                         outs[v].color[ch * P + pid] = float(C[ch] + Tr * T(vi.cam.background[ch]));
@@ -338,7 +370,7 @@ void gaussian_row_grad(const pht_gaussians& g, const pht_settings& s, const Stat
     std::memset((void*)&out, 0, sizeof(out));
     const int64_t N = g.count;
     const int n_views = int(st.views.info.size());
-    const T mean[3] = {T(g.means[3 * i]), T(g.means[3 * i + 1]), T(g.means[3 * i + 2])};
+    const T mean[3] = {mean_in<T>(g, 3 * i), mean_in<T>(g, 3 * i + 1), mean_in<T>(g, 3 * i + 2)};
     const Activated<T> act = activate_row<T>(g, i);
     SplatGrad<T> sg;
     std::memset((void*)&sg, 0, sizeof(sg));
@@ -429,6 +461,7 @@ State<float> g_state32;
 State<double> g_state64;
 std::vector<SplatGradAcc<float>> g_acc32;
 std::vector<SplatGradAcc<double>> g_acc64;
+float g_alpha_clip = 0.99f; // alpha clip of the last forward (for the tile-free check)
 
 template <class T>
 void export_splat_grads(const State<T>& st, const std::vector<SplatGradAcc<T>>& acc, int64_t n, double* out) {
@@ -453,6 +486,16 @@ void export_splat_grads(const State<T>& st, const std::vector<SplatGradAcc<T>>& 
 extern "C" {
 
 int oracle_abi_version(void) { return PHT_ABI_VERSION; }
+
+// Double-precision parameters read by the precision-64 runs (NULL means: use the float inputs).
+void oracle_set_inputs64(const double* means, const double* log_scales, const double* quaternions,
+                         const double* opacity_logits, const double* sh) {
+    g_in64.means = means;
+    g_in64.log_scales = log_scales;
+    g_in64.quaternions = quaternions;
+    g_in64.opacity_logits = opacity_logits;
+    g_in64.sh = sh;
+}
 
 // Deterministic math primitives, exported for the error-bound tests.
 void oracle_pexpf(const float* x, float* y, int64_t n) {
@@ -526,6 +569,7 @@ int oracle_forward(int precision, const pht_gaussians* g, const pht_camera* cams
                    const pht_settings* s, pht_view_outputs* outs, uint64_t* keys, uint32_t* values,
                    int64_t key_cap, uint32_t* ranges, uint32_t* n_contrib, float* final_T,
                    pht_counts* counts, int threads) {
+    g_alpha_clip = s->alpha_clip;
     if (precision == 64) {
         run_forward<double>(*g, cams, n_views, *s, g_state64, outs, threads);
         if (counts) {
@@ -552,6 +596,78 @@ int oracle_forward(int precision, const pht_gaussians* g, const pht_camera* cams
     if (ranges) std::copy(st.ranges.begin(), st.ranges.end(), ranges);
     if (n_contrib) std::copy(st.n_contrib.begin(), st.n_contrib.end(), n_contrib);
     if (final_T) std::copy(st.final_T.begin(), st.final_T.end(), final_T);
+    return 0;
+}
+
+// SH colour of unit directions (n x 3) for one coefficient row (coef[k*3+ch]), in double; rgb
+// before the max(., 0) clamp is returned in out[n x 3].
+void oracle_sh_eval(int degree, const double* dirs, const double* coef, double* out, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) {
+        const double d[3] = {dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]};
+        double Y[16];
+        sh_basis(degree, d, Y);
+        for (int ch = 0; ch < 3; ++ch) {
+            double acc = Y[0] * coef[ch];
+            for (int k = 1; k < sh_count(degree); ++k) acc = fma(Y[k], coef[k * 3 + ch], acc);
+            out[3 * i + ch] = acc + 0.5;
+        }
+    }
+}
+
+// Tile-free render of the last float forward: every pixel walks all visible splats of its view in
+// (depth bits, gaussian index) order with the same per-pair arithmetic. Equal to the tiled render
+// bit for bit iff tile enumeration, sorting and ranges lose nothing.
+int oracle_bruteforce(float* color_concat, float* alpha_concat, float* final_T) {
+    const State<float>& st = g_state32;
+    for (size_t vi_idx = 0; vi_idx < st.views.info.size(); ++vi_idx) {
+        const ViewInfo& vi = st.views.info[vi_idx];
+        std::vector<uint32_t> order;
+        for (size_t si = 0; si < st.splats.size(); ++si)
+            if (st.splats[si].v == int(vi_idx)) order.push_back(uint32_t(si));
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+            const SplatRec<float>& x = st.splats[a];
+            const SplatRec<float>& y = st.splats[b];
+            if (x.depth_key != y.depth_key) return x.depth_key < y.depth_key;
+            return x.g < y.g;
+        });
+        const int W = vi.cam.width, H = vi.cam.height;
+        const int64_t P = int64_t(W) * H;
+#pragma omp parallel for schedule(dynamic, 8)
+        for (int py = 0; py < H; ++py)
+            for (int px = 0; px < W; ++px) {
+                float Tr = 1.0f, C[3] = {0.0f, 0.0f, 0.0f};
+                for (uint32_t si : order) {
+                    const SplatRec<float>& r = st.splats[si];
+                    const float dx = r.mean2d[0] - float(px), dy = r.mean2d[1] - float(py);
+                    float alpha, G;
+                    if (!pair_alpha(r.conic[0], r.conic[1], r.conic[2], r.opacity, dx, dy, g_alpha_clip, alpha, G))
+                        continue;
+                    const float test_t = Tr * (1.0f - alpha);
+                    if (test_t < K<float>::transmittance_floor()) break;
+                    const float a_t = alpha * Tr;
+                    for (int ch = 0; ch < 3; ++ch) C[ch] = fma_(r.rgb[ch], a_t, C[ch]);
+                    Tr = test_t;
+                }
+                const int64_t pid = int64_t(py) * W + px;
+                for (int ch = 0; ch < 3; ++ch)
+                    color_concat[3 * vi.pixel_base + ch * P + pid] = C[ch] + Tr * vi.cam.background[ch];
+                alpha_concat[vi.pixel_base + pid] = 1.0f - Tr;
+                final_T[vi.pixel_base + pid] = Tr;
+            }
+    }
+    return 0;
+}
+
+// Images of the last forward at its own precision (double for precision 64): colour per view
+// [3,H,W] and alpha per view [H,W], each concatenated over views.
+int oracle_images(int precision, double* color, double* alpha) {
+    if (precision == 64) {
+        std::copy(g_state64.color.begin(), g_state64.color.end(), color);
+        std::copy(g_state64.alpha.begin(), g_state64.alpha.end(), alpha);
+    } else {
+        std::copy(g_state32.color.begin(), g_state32.color.end(), color);
+        std::copy(g_state32.alpha.begin(), g_state32.alpha.end(), alpha);
+    }
     return 0;
 }
 

@@ -51,6 +51,10 @@ def load() -> C.CDLL:
                                           C.c_int]
         lib.oracle_train_step.restype = C.c_double
         lib.oracle_splat_grads.argtypes = [C.c_int, C.c_int64, vp]
+        lib.oracle_set_inputs64.argtypes = [vp, vp, vp, vp, vp]
+        lib.oracle_images.argtypes = [C.c_int, vp, vp]
+        lib.oracle_sh_eval.argtypes = [C.c_int, vp, vp, vp, C.c_int64]
+        lib.oracle_bruteforce.argtypes = [vp, vp, vp]
         _lib = lib
     return _lib
 
@@ -161,6 +165,17 @@ class OracleRun:
                                None, None, C.byref(counts), self.threads)
         self.counts = counts
         self._outs = outs
+        # images at the run's own precision (double for the accuracy oracle)
+        c = np.zeros((3 * self.pixels,), np.float64)
+        a = np.zeros((self.pixels,), np.float64)
+        lib.oracle_images(precision, _p(c), _p(a))
+        self.color_hp, self.alpha_hp = [], []
+        off = 0
+        for cam in self.cameras:
+            P = cam.width * cam.height
+            self.color_hp.append(c[3 * off:3 * (off + P)].reshape(3, cam.height, cam.width))
+            self.alpha_hp.append(a[off:off + P].reshape(cam.height, cam.width))
+            off += P
 
     def backward(self, grad_colors: Sequence[np.ndarray], grad_alphas: Optional[Sequence] = None) -> Dict[str, np.ndarray]:
         lib = load()
@@ -186,11 +201,52 @@ class OracleRun:
         self._keep = (gcs, gas)
         return out
 
+    def bruteforce(self):
+        """Tile-free render of this (float) run: per-view colour, alpha, final T."""
+        c = np.zeros((3 * self.pixels,), np.float32)
+        a = np.zeros((self.pixels,), np.float32)
+        t = np.zeros((self.pixels,), np.float32)
+        load().oracle_bruteforce(_p(c), _p(a), _p(t))
+        out, off = [], 0
+        for cam in self.cameras:
+            P = cam.width * cam.height
+            out.append((c[3 * off:3 * (off + P)].reshape(3, cam.height, cam.width),
+                        a[off:off + P].reshape(cam.height, cam.width), t[off:off + P]))
+            off += P
+        return out
+
     def splat_grads(self) -> np.ndarray:
         """Per-(gaussian, view) blend gradients of the last backward, [N, views, 10]."""
         out = np.zeros((self.scene.n, len(self.cameras), 10), np.float64)
         load().oracle_splat_grads(self.precision, self.scene.n, _p(out))
         return out
+
+
+class inputs64:
+    """Context manager: precision-64 runs read these double parameters instead of the float ones."""
+
+    def __init__(self, arrays: Dict[str, np.ndarray]):
+        self.a = {k: np.ascontiguousarray(np.asarray(arrays[k], dtype=np.float64))
+                  for k in ("means", "log_scales", "quaternions", "opacity_logits", "sh")}
+
+    def __enter__(self):
+        a = self.a
+        load().oracle_set_inputs64(_p(a["means"]), _p(a["log_scales"]), _p(a["quaternions"]),
+                                   _p(a["opacity_logits"]), _p(a["sh"]))
+        return self
+
+    def __exit__(self, *exc):
+        load().oracle_set_inputs64(None, None, None, None, None)
+        return False
+
+
+def sh_eval(degree: int, dirs: np.ndarray, coef: np.ndarray) -> np.ndarray:
+    """Unclamped SH colour (double) of unit directions [n,3] for one coefficient row [16,3]."""
+    d = np.ascontiguousarray(dirs, dtype=np.float64)
+    c = np.ascontiguousarray(np.asarray(coef, dtype=np.float64).reshape(-1))
+    out = np.zeros((d.shape[0], 3), np.float64)
+    load().oracle_sh_eval(degree, _p(d), _p(c), _p(out), d.shape[0])
+    return out
 
 
 def pexpf(x: np.ndarray) -> np.ndarray:

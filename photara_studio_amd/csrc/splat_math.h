@@ -308,20 +308,36 @@ PHT_HD bool project_pinhole(const T mean[3], const Activated<T>& act, const pht_
     // footprint::apply (footprint.cuh:30-37)
     const Cap<T> cp = cap_policy(a, b, c, cam.width, cam.height);
     if (cp.q < T(1)) { a = a * cp.q; b = b * cp.q; c = c * cp.q; }
-    // conic (render_forward.cu:74-79), FP64 for near-singular 2x2 blocks
-    const T ac = a * c;
-    const T det = ac - b * b;
-    if (det == T(0)) return false;
+    // conic (render_forward.cu:74-79)
+    T ac = a * c;
+    T det = ac - b * b;
     out.fp64 = false;
     if (s.fp64_conic && sizeof(T) == 4 && !(det > T(kNearSingular) * ac)) {
-        const double dd = double(a) * double(c) - double(b) * double(b);
+        // Near-singular 2x2 block: the float products cancel. Form the 2x2 covariance, its cap,
+        // determinant and inverse in FP64 from the same float inputs, then round back.
+        double md[3][2];
+        for (int i = 0; i < 3; ++i) {
+            const double si = double(sm) * double(act.scale[i]);
+            const double r0 = si * double(R.m[i][0]), r1 = si * double(R.m[i][1]), r2 = si * double(R.m[i][2]);
+            md[i][0] = (r0 * double(tm[0][0]) + r1 * double(tm[1][0])) + r2 * double(tm[2][0]);
+            md[i][1] = (r0 * double(tm[0][1]) + r1 * double(tm[1][1])) + r2 * double(tm[2][1]);
+        }
+        double da = ((md[0][0] * md[0][0] + md[1][0] * md[1][0]) + md[2][0] * md[2][0]) + double(ks);
+        double db = (md[0][0] * md[0][1] + md[1][0] * md[1][1]) + md[2][0] * md[2][1];
+        double dc = ((md[0][1] * md[0][1] + md[1][1] * md[1][1]) + md[2][1] * md[2][1]) + double(ks);
+        const Cap<double> cpd = cap_policy(da, db, dc, cam.width, cam.height);
+        if (cpd.q < 1.0) { da = da * cpd.q; db = db * cpd.q; dc = dc * cpd.q; }
+        const double dd = da * dc - db * db;
         if (!(dd > 0.0)) return false;
         const double di = 1.0 / dd;
-        out.conic[0] = T(double(c) * di);
-        out.conic[1] = T(-double(b) * di);
-        out.conic[2] = T(double(a) * di);
+        out.conic[0] = T(dc * di);
+        out.conic[1] = T(-db * di);
+        out.conic[2] = T(da * di);
+        a = T(da); b = T(db); c = T(dc);
+        det = T(dd);
         out.fp64 = true;
     } else {
+        if (det == T(0)) return false;
         const T det_inv = T(1) / det;
         out.conic[0] = c * det_inv;
         out.conic[1] = -b * det_inv;

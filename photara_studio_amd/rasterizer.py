@@ -345,7 +345,7 @@ class _Context:
 
 
 def default_instance_capacity(n: int, views: int) -> int:
-    return max(1 << 20, 6 * n * views)
+    return max(1 << 20, 8 * n * views)
 
 
 def tiles_of(cameras: Sequence[Camera], ts: int) -> int:

@@ -132,6 +132,27 @@ class TrainStep:
     def counts(self) -> _abi.pht_counts:
         return self.ws.counts()
 
+    def ensure_capacity(self, headroom: float = 1.25) -> int:
+        """Run forward passes until the instance list fits the workspace (growing it by the
+        measured count x headroom); returns the instance count. Parameters are not updated."""
+        lib = self.lib
+        stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        while True:
+            check(lib.pht_forward(self.ws.handle, C.byref(self.gauss), self.cams, len(self.cameras),
+                                  C.byref(self.settings), self.outs, 0, stream))
+            c = self.ws.counts()
+            if not c.overflow:
+                return int(c.instances)
+            if self.graph is not None:
+                lib.pht_step_graph_destroy(self.graph)
+                self.graph = None
+            self.capacity = int(c.instances * headroom) + 4096
+            dev = self.model.means.device
+            self.ws = None
+            torch.cuda.empty_cache()
+            self.ws = Workspace(dev, self.model.size(), len(self.cameras), int(self.options.tile_size), self.tiles,
+                                self.pixels, self.capacity)
+
     def loss(self) -> float:
         return float(self.loss_sum.item())
 

@@ -1,0 +1,75 @@
+"""Pins of the oracle's scalar primitives (CPU): exp/log written from Cody-Waite reduction and
+series, the fixed-order footprint power, and the bf16/fp16 storage conversions."""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+
+def ulp_err(got: np.ndarray, want64: np.ndarray) -> np.ndarray:
+    """Error in units of the float32 ulp at the exact value."""
+    want32 = want64.astype(np.float32)
+    ulp = np.spacing(np.abs(want32)).astype(np.float64)
+    return np.abs(got.astype(np.float64) - want64) / ulp
+
+
+def test_pexpf_error_bound(oracle):
+    rng = np.random.default_rng(0)
+    x = np.concatenate([np.linspace(-87.0, 88.5, 400001, dtype=np.float32),
+                        rng.uniform(-15.0, 0.0, 200000).astype(np.float32),
+                        np.array([0.0, -0.0, 1e-30, -1e-30, 1.0, -1.0], np.float32)])
+    y = oracle.pexpf(x)
+    err = ulp_err(y, np.exp(x.astype(np.float64)))
+    assert err.max() <= 2.0, err.max()
+    assert oracle.pexpf(np.array([0.0], np.float32))[0] == 1.0
+    special = oracle.pexpf(np.array([-np.inf, np.nan, 100.0, -200.0], np.float32))
+    assert special[0] == 0.0 and np.isnan(special[1]) and special[2] == np.inf and special[3] == 0.0
+    # subnormal results stay accurate (denormals are not flushed)
+    xs = np.linspace(-103.0, -88.0, 20001, dtype=np.float32)
+    ys = oracle.pexpf(xs)
+    want = np.exp(xs.astype(np.float64))
+    assert np.abs(ys.astype(np.float64) - want).max() <= 2 * 1.4e-45
+
+
+def test_plogf_error_bound(oracle):
+    rng = np.random.default_rng(1)
+    x = np.concatenate([np.exp(rng.uniform(-87, 88, 400000)).astype(np.float32),
+                        np.linspace(1.0, 255.0, 200001, dtype=np.float32),
+                        np.array([1.0, 2.0, 0.5, 255.0, 1e-40, 1.17549435e-38], np.float32)])
+    y = oracle.plogf(x)
+    want = np.log(x.astype(np.float64))
+    err = np.abs(y.astype(np.float64) - want) / np.spacing(np.maximum(np.abs(want), 1e-30).astype(np.float32))
+    # near x = 1 the result is tiny; bound it absolutely there
+    assert err[np.abs(want) > 1e-3].max() <= 2.0
+    assert np.abs(y.astype(np.float64) - want)[np.abs(want) <= 1e-3].max() <= 1e-10
+    assert oracle.plogf(np.array([1.0], np.float32))[0] == 0.0
+    sp = oracle.plogf(np.array([0.0, -1.0, np.inf], np.float32))
+    assert sp[0] == -np.inf and np.isnan(sp[1]) and sp[2] == np.inf
+
+
+def test_gaussian_power_matches_exact(oracle):
+    """-1/2 (a dx^2 + c dy^2) - b dx dy, fixed rounding order (geometry.cuh:26-33)."""
+    rng = np.random.default_rng(2)
+    n = 100000
+    a = rng.uniform(0.01, 2.0, n)
+    c = rng.uniform(0.01, 2.0, n)
+    b = rng.uniform(-1, 1, n) * np.sqrt(a * c) * 0.99
+    conic = np.stack([a, b, c], 1).astype(np.float32)
+    d = rng.uniform(-8, 8, (n, 2)).astype(np.float32)
+    got = oracle.gaussian_power(conic, d).astype(np.float64)
+    c64, d64 = conic.astype(np.float64), d.astype(np.float64)
+    want = -0.5 * (c64[:, 0] * d64[:, 0] ** 2 + c64[:, 2] * d64[:, 1] ** 2) - c64[:, 1] * d64[:, 0] * d64[:, 1]
+    scale = 0.5 * (c64[:, 0] * d64[:, 0] ** 2 + c64[:, 2] * d64[:, 1] ** 2) + np.abs(c64[:, 1] * d64[:, 0] * d64[:, 1])
+    assert (np.abs(got - want) <= 4e-7 * scale + 1e-30).all()
+
+
+def test_reduced_precision_conversions_match_torch(oracle):
+    from photara_studio_amd import _abi
+    rng = np.random.default_rng(3)
+    x = np.concatenate([rng.normal(0, 3, 100000), rng.normal(0, 1e-6, 10000), rng.normal(0, 3e4, 10000),
+                        [0.0, -0.0, 65504.0, 65520.0, 1e9, -1e9, 6e-8, 1e-7, np.inf, -np.inf]]).astype(np.float32)
+    for fmt, dt in ((_abi.PHT_LP_BF16, torch.bfloat16), (_abi.PHT_LP_FP16, torch.float16)):
+        bits, back = oracle.lp_roundtrip(x, fmt)
+        ref = torch.from_numpy(x).to(dt)
+        np.testing.assert_array_equal(bits, ref.view(torch.int16).numpy().view(np.uint16))
+        np.testing.assert_array_equal(back, ref.float().numpy())
