@@ -50,6 +50,19 @@ def algorithmic_bytes(n, views, visible, instances, pixels, tiles, sh_bytes, lp)
     return out
 
 
+def profiled_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` from the rocprofv3 FETCH_SIZE / WRITE_SIZE summaries under
+    profiles/ (already scaled by the calibration copy), or None when no summary covers it."""
+    for f in sorted((ROOT / "profiles").glob("*.md")):
+        for line in f.read_text().splitlines():
+            if not line.startswith("traffic:"):
+                continue
+            kv = dict(x.split("=", 1) for x in line[len("traffic:"):].split())
+            if kv.get("kernel") == kernel and kv.get("workload") == workload:
+                return int(kv["fetch_bytes"]) + int(kv["write_bytes"]), f.name
+    return None, None
+
+
 def make_targets(n, cams, opts, device, seed):
     from photara_studio_amd.rasterizer import Rasterizer
     from photara_studio_amd.scenes import model_from, synthetic_torch
@@ -139,6 +152,8 @@ def main():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--tile", type=int, default=16)
     p.add_argument("--batch", type=int, default=256)
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="first copy 1 GiB device-to-device (a kernel of known traffic for counter scaling)")
     args = p.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only (multi-GPU view sharding is a design note, "
@@ -153,6 +168,13 @@ def main():
 
     dev = torch.device("cuda:0")
     n, nv = args.gaussians, args.views
+    if args.calibrate_copy:
+        src = torch.ones((1 << 28,), dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print("[bench] calibration copy: 1073741824 bytes read + written", flush=True)
+        del src, dst
     cams = workload_cameras(f"bench{nv}")
     opts = RasterizeOptions(active_sh_degree=3, kernel_size=0.3, tile_size=args.tile, batch=args.batch)
     opts.rasterization.compensate_2d = False
@@ -197,6 +219,8 @@ def main():
     dominant = max(acc, key=acc.get)
     dom_gbps = ab[dominant] / (acc[dominant] * 1e-3) / 1e9
     step_bytes = sum(ab.values())
+    workload_key = f"{n // 1_000_000}M-{nv}x1080p-bf16" if n % 1_000_000 == 0 else f"{n}-{nv}x1080p-bf16"
+    traffic, traffic_src = profiled_traffic(dominant, workload_key)
     print(f"[bench] stages ms: " + ", ".join(f"{k}={v:.3f}" for k, v in acc.items()), flush=True)
 
     cpu = None
@@ -229,8 +253,9 @@ def main():
         "counts": {"visible": int(bc.visible), "instances": int(bc.instances), "fp64_conics": int(bc.fp64_conics)},
         "stage_ms": {k: round(v, 4) for k, v in acc.items()},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(dom_gbps, 1), "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": round(dom_gbps / HBM_PEAK_GBPS, 4), "traffic": None,
-                     "algorithmic_bytes": int(ab[dominant])},
+                     "unit": "GB/s", "frac": round(dom_gbps / HBM_PEAK_GBPS, 4), "traffic": traffic,
+                     "algorithmic_bytes": int(ab[dominant]),
+                     "traffic_source": f"profiles/{traffic_src}" if traffic_src else None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
