@@ -69,6 +69,7 @@ struct pht_workspace {
     uint32_t* view_mask = nullptr;
     uint32_t* dkeys[2] = {nullptr, nullptr};
     uint32_t* dvals[2] = {nullptr, nullptr};
+    uint32_t* dtiles[2] = {nullptr, nullptr};
     uint32_t* offsets = nullptr;
     uint32_t* tkeys[2] = {nullptr, nullptr};
     uint32_t* tvals[2] = {nullptr, nullptr};
@@ -77,6 +78,7 @@ struct pht_workspace {
     uint32_t* n_contrib = nullptr;
     uint32_t* scan_scratch = nullptr;
     uint32_t* hist = nullptr;
+    uint32_t* digit_total = nullptr;
     Counters* host_counters = nullptr; // pinned
     // state of the last forward
     bool fwd_valid = false;
@@ -116,6 +118,7 @@ static void carve_workspace(pht_workspace& w, char* base, const pht_capacity& ca
     for (int k = 0; k < 2; ++k) {
         w.dkeys[k] = c.take<uint32_t>(slots);
         w.dvals[k] = c.take<uint32_t>(slots);
+        w.dtiles[k] = c.take<uint32_t>(slots);
     }
     w.offsets = c.take<uint32_t>(slots);
     for (int k = 0; k < 2; ++k) {
@@ -127,6 +130,7 @@ static void carve_workspace(pht_workspace& w, char* base, const pht_capacity& ca
     w.n_contrib = c.take<uint32_t>(size_t(cap.pixels));
     w.scan_scratch = c.take<uint32_t>(kScanBlocks);
     w.hist = c.take<uint32_t>(256 * size_t(kScanBlocks));
+    w.digit_total = c.take<uint32_t>(256);
     if (total) *total = (c.off + 255) & ~size_t(255);
 }
 
@@ -271,19 +275,21 @@ int pht_forward(pht_workspace* ws, const pht_gaussians* g, const pht_camera* cam
     launch_preprocess(pa, n_slots, st);
     ws->mark(0, 1, st);
     ws->mark(1, 0, st);
-    launch_compact_visible(ws->vis_flag, ws->rec_c, uint32_t(n_slots), ws->dkeys[0], ws->dvals[0], ws->scan_scratch,
-                           &ws->counters->visible, st);
-    const int dsel = launch_radix_sort(ws->dkeys, ws->dvals, &ws->counters->visible, uint32_t(n_slots), 32, ws->hist, st);
+    launch_compact_visible(ws->vis_flag, ws->rec_c, uint32_t(n_slots), ws->dkeys[0], ws->dvals[0], ws->dtiles[0],
+                           ws->scan_scratch, &ws->counters->visible, st);
+    const int dsel = launch_radix_sort(ws->dkeys, ws->dvals, ws->dtiles, &ws->counters->visible, uint32_t(n_slots),
+                                       32, ws->hist, ws->digit_total, st);
     ws->mark(1, 1, st);
     ws->mark(2, 0, st);
-    launch_tile_offsets(ws->dvals[dsel], ws->rec_c, &ws->counters->visible, uint32_t(n_slots), ws->offsets,
-                        ws->scan_scratch, &ws->counters->instances, &ws->counters->overflow, ws->inst_cap, st);
+    launch_tile_offsets(ws->dtiles[dsel], &ws->counters->visible, uint32_t(n_slots), ws->offsets, ws->scan_scratch,
+                        &ws->counters->instances, &ws->counters->overflow, ws->inst_cap, st);
     launch_emit_instances(ws->dvals[dsel], ws->offsets, ws->counters, uint32_t(n_slots), ws->rec_a, ws->rec_b,
                           ws->rec_c, ws->views, ws->vp_log2, ts, ws->inst_cap, ws->tkeys[0], ws->tvals[0], st);
     ws->mark(2, 1, st);
     ws->mark(3, 0, st);
     const int tbits = bits_for(uint64_t(tiles - 1));
-    const int tsel = launch_radix_sort(ws->tkeys, ws->tvals, &ws->counters->instances, ws->inst_cap, tbits, ws->hist, st);
+    const int tsel = launch_radix_sort(ws->tkeys, ws->tvals, nullptr, &ws->counters->instances, ws->inst_cap, tbits,
+                                       ws->hist, ws->digit_total, st);
     launch_ranges(ws->tkeys[tsel], ws->counters, ws->inst_cap, uint32_t(tiles), ws->ranges, st);
     ws->mark(3, 1, st);
     BlendArgs ba;

@@ -18,76 +18,118 @@ template <int LP> __device__ __forceinline__ float ld_lp(const float* f, const u
     else return lp_to_f32(lp[i], LP);
 }
 
+// SH rows of the block's 256 Gaussians live in LDS with an odd row pitch, so the per-thread row
+// reads of the backward are bank-conflict free; loads and stores of the SH arrays (values,
+// moments, reduced-precision mirror) are done cooperatively over the block's contiguous range,
+// one coalesced pass each, instead of one 192-byte row per lane.
+constexpr int kShPitch = 49;
+
+// Walks e = t, t + 256, ... over [0, rows * stride) keeping (row, k) = divmod(e, stride)
+// incrementally; calls f(e, row, k) for the active coefficients k < active.
+template <class F>
+__device__ __forceinline__ void for_block_rows(int rows, int stride, int active, F&& f) {
+    const int total = rows * stride;
+    const int step_row = kScanThreads / stride, step_k = kScanThreads % stride;
+    int row = int(threadIdx.x) / stride, k = int(threadIdx.x) % stride;
+    for (int e = threadIdx.x; e < total; e += kScanThreads) {
+        if (k < active) f(e, row, k);
+        row += step_row;
+        k += step_k;
+        if (k >= stride) {
+            k -= stride;
+            ++row;
+        }
+    }
+}
+
 template <int DEG, int LP, bool ADAM>
 __global__ __launch_bounds__(256) void k_gaussian_bwd(GaussArgs a) {
     constexpr int NB = (DEG + 1) * (DEG + 1);
-    const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
-    if (i >= a.g.count) return;
+    __shared__ float s_sh[kScanThreads * kShPitch];
     if constexpr (ADAM) {
-        if (a.counters->overflow) return;
+        if (a.counters->overflow) return; // uniform over the grid
     }
-    const float mean[3] = {a.g.means[3 * i], a.g.means[3 * i + 1], a.g.means[3 * i + 2]};
-    float ls[3], q[4];
-    for (int k = 0; k < 3; ++k) ls[k] = ld_lp<LP>(a.g.log_scales, a.g.log_scales_lp, 3 * i + k);
-    for (int k = 0; k < 4; ++k) q[k] = a.g.quaternions[4 * i + k];
-    const float logit = ld_lp<LP>(a.g.opacity_logits, a.g.opacity_logits_lp, i);
-    const bool hf = a.g.filter_3d != nullptr;
-    const float filt = hf ? a.g.filter_3d[i] : 0.0f;
-    const Activated<float> act = activate<float>(ls, q, logit, hf, filt);
-    const uint32_t mask = a.view_mask[i];
+    const int64_t i0 = int64_t(blockIdx.x) * kScanThreads;
+    const int64_t i = i0 + threadIdx.x;
+    const bool live = i < a.g.count;
+    const int rows = int(a.g.count - i0 < kScanThreads ? a.g.count - i0 : kScanThreads);
+    const int stride = a.s.sh_bases * 3;
+    const int64_t row0 = i0 * int64_t(stride);
     const bool precomp = a.g.colors_precomp != nullptr;
+    const bool use_sh = !precomp;
+    if (use_sh) {
+        for_block_rows(rows, stride, NB * 3, [&](int e, int row, int k) {
+            s_sh[row * kShPitch + k] = ld_lp<LP>(a.g.sh, a.g.sh_lp, row0 + e);
+        });
+    }
+    __syncthreads();
+    float* my_sh = s_sh + threadIdx.x * kShPitch;
 
     SplatGrad<float> sg;
     for (int k = 0; k < 3; ++k) { sg.mean[k] = 0.0f; sg.scale[k] = 0.0f; }
     for (int k = 0; k < 4; ++k) sg.rot[k] = 0.0f;
     sg.opacity = 0.0f;
     float gsh[NB * 3];
-    float coef[NB * 3];
 #pragma unroll
     for (int k = 0; k < NB * 3; ++k) gsh[k] = 0.0f;
-    const int64_t row = i * int64_t(a.s.sh_bases) * 3;
-    if (!precomp && mask != 0) {
-#pragma unroll
-        for (int k = 0; k < NB * 3; ++k) coef[k] = ld_lp<LP>(a.g.sh, a.g.sh_lp, row + k);
-    }
     float gcol[3] = {0.0f, 0.0f, 0.0f};
     float refine = 0.0f;
-    for (int v = 0; v < a.n_views; ++v) {
-        if (!((mask >> v) & 1u)) continue;
-        const int64_t slot = (i << a.vp_log2) + v;
-        const float4* gp = reinterpret_cast<const float4*>(a.sgrad + slot * kGradStride);
-        const float4 g0 = gp[0], g1 = gp[1], g2 = gp[2];
-        const float dconic[4] = {g0.z, g0.w, g1.x, g1.y};
-        const float dm2[2] = {g0.x, g0.y};
-        const pht_camera& cam = a.views[v].vi.cam;
-        splat_backward_pinhole(mean, act, cam, a.s, dconic, dm2, sg);
-        const float dcol[3] = {g1.z, g1.w, g2.x};
-        if (precomp) {
-            gcol[0] += dcol[0]; gcol[1] += dcol[1]; gcol[2] += dcol[2];
-        } else {
-            float d[3], raw[3];
-            view_dir(mean, cam.position, d, raw);
-            sh_backward(DEG, raw, d, coef, a.rec_c[slot].w & 7u, dcol, gsh, sg.mean);
+    float gls[3] = {0.0f, 0.0f, 0.0f}, gq[4] = {0.0f, 0.0f, 0.0f, 0.0f}, glogit = 0.0f;
+    if (live) {
+        const float mean[3] = {a.g.means[3 * i], a.g.means[3 * i + 1], a.g.means[3 * i + 2]};
+        float ls[3], q[4];
+        for (int k = 0; k < 3; ++k) ls[k] = ld_lp<LP>(a.g.log_scales, a.g.log_scales_lp, 3 * i + k);
+        for (int k = 0; k < 4; ++k) q[k] = a.g.quaternions[4 * i + k];
+        const float logit = ld_lp<LP>(a.g.opacity_logits, a.g.opacity_logits_lp, i);
+        const bool hf = a.g.filter_3d != nullptr;
+        const float filt = hf ? a.g.filter_3d[i] : 0.0f;
+        const Activated<float> act = activate<float>(ls, q, logit, hf, filt);
+        const uint32_t mask = a.view_mask[i];
+        for (int v = 0; v < a.n_views; ++v) {
+            if (!((mask >> v) & 1u)) continue;
+            const int64_t slot = (i << a.vp_log2) + v;
+            const float4* gp = reinterpret_cast<const float4*>(a.sgrad + slot * kGradStride);
+            const float4 g0 = gp[0], g1 = gp[1], g2 = gp[2];
+            const float dconic[4] = {g0.z, g0.w, g1.x, g1.y};
+            const float dm2[2] = {g0.x, g0.y};
+            const pht_camera& cam = a.views[v].vi.cam;
+            splat_backward_pinhole(mean, act, cam, a.s, dconic, dm2, sg);
+            const float dcol[3] = {g1.z, g1.w, g2.x};
+            if (precomp) {
+                gcol[0] += dcol[0]; gcol[1] += dcol[1]; gcol[2] += dcol[2];
+            } else {
+                float d[3], raw[3];
+                view_dir(mean, cam.position, d, raw);
+                sh_backward(DEG, raw, d, my_sh, a.rec_c[slot].w & 7u, dcol, gsh, sg.mean);
+            }
+            refine += g2.y;
         }
-        refine += g2.y;
+        chain_raw(act, hf, filt, sg, gls, gq, glogit);
+        if (a.out.refine_weight) a.out.refine_weight[i] = refine;
     }
-    float gls[3], gq[4], glogit;
-    chain_raw(act, hf, filt, sg, gls, gq, glogit);
-    if (a.out.refine_weight) a.out.refine_weight[i] = refine;
+    // own row only: the coefficients just read are replaced by their gradients
+    if (use_sh) {
+#pragma unroll
+        for (int k = 0; k < NB * 3; ++k) my_sh[k] = gsh[k];
+    }
+    __syncthreads();
     if constexpr (!ADAM) {
         const pht_model_grads& o = a.out;
-        for (int k = 0; k < 3; ++k) {
-            if (o.means) o.means[3 * i + k] = sg.mean[k];
-            if (o.log_scales) o.log_scales[3 * i + k] = gls[k];
-            if (o.colors_precomp) o.colors_precomp[3 * i + k] = gcol[k];
+        if (live) {
+            for (int k = 0; k < 3; ++k) {
+                if (o.means) o.means[3 * i + k] = sg.mean[k];
+                if (o.log_scales) o.log_scales[3 * i + k] = gls[k];
+                if (o.colors_precomp) o.colors_precomp[3 * i + k] = gcol[k];
+            }
+            for (int k = 0; k < 4; ++k)
+                if (o.quaternions) o.quaternions[4 * i + k] = gq[k];
+            if (o.opacity_logits) o.opacity_logits[i] = glogit;
         }
-        for (int k = 0; k < 4; ++k)
-            if (o.quaternions) o.quaternions[4 * i + k] = gq[k];
-        if (o.opacity_logits) o.opacity_logits[i] = glogit;
         if (o.sh) {
-#pragma unroll
-            for (int k = 0; k < NB * 3; ++k) o.sh[row + k] = gsh[k];
-            for (int k = NB * 3; k < a.s.sh_bases * 3; ++k) o.sh[row + k] = 0.0f;
+            float* dst = o.sh + row0;
+            for_block_rows(rows, stride, stride, [&](int e, int row, int k) {
+                dst[e] = (use_sh && k < NB * 3) ? s_sh[row * kShPitch + k] : 0.0f;
+            });
         }
     } else {
         const pht_adam& A = a.adam;
@@ -98,36 +140,43 @@ __global__ __launch_bounds__(256) void k_gaussian_bwd(GaussArgs a) {
             c2 = A.device_hparams[1];
             lr_means = A.device_hparams[2];
         }
-        for (int k = 0; k < 3; ++k) {
-            const int64_t j = 3 * i + k;
-            adam_update(A.means[j], A.m_means[j], A.v_means[j], sg.mean[k], lr_means, A.beta1, A.beta2, c1, c2,
-                        A.epsilon, -inf, inf, false);
-            adam_update(A.log_scales[j], A.m_log_scales[j], A.v_log_scales[j], gls[k], A.lr_scales, A.beta1,
-                        A.beta2, c1, c2, A.epsilon, A.min_log_scale, A.max_log_scale, false);
+        if (live) {
+            for (int k = 0; k < 3; ++k) {
+                const int64_t j = 3 * i + k;
+                adam_update(A.means[j], A.m_means[j], A.v_means[j], sg.mean[k], lr_means, A.beta1, A.beta2, c1, c2,
+                            A.epsilon, -inf, inf, false);
+                adam_update(A.log_scales[j], A.m_log_scales[j], A.v_log_scales[j], gls[k], A.lr_scales, A.beta1,
+                            A.beta2, c1, c2, A.epsilon, A.min_log_scale, A.max_log_scale, false);
+            }
+            for (int k = 0; k < 4; ++k) {
+                const int64_t j = 4 * i + k;
+                adam_update(A.quaternions[j], A.m_quaternions[j], A.v_quaternions[j], gq[k], A.lr_quaternions,
+                            A.beta1, A.beta2, c1, c2, A.epsilon, -inf, inf, false);
+            }
+            adam_update(A.opacity_logits[i], A.m_opacity_logits[i], A.v_opacity_logits[i], glogit, A.lr_opacity,
+                        A.beta1, A.beta2, c1, c2, A.epsilon, -12.0f, 12.0f, false);
+            if constexpr (LP != PHT_LP_NONE) {
+                for (int k = 0; k < 3; ++k)
+                    if (A.log_scales_lp) A.log_scales_lp[3 * i + k] = f32_to_lp(A.log_scales[3 * i + k], LP);
+                if (A.opacity_logits_lp) A.opacity_logits_lp[i] = f32_to_lp(A.opacity_logits[i], LP);
+            }
         }
-        for (int k = 0; k < 4; ++k) {
-            const int64_t j = 4 * i + k;
-            adam_update(A.quaternions[j], A.m_quaternions[j], A.v_quaternions[j], gq[k], A.lr_quaternions,
-                        A.beta1, A.beta2, c1, c2, A.epsilon, -inf, inf, false);
-        }
-        adam_update(A.opacity_logits[i], A.m_opacity_logits[i], A.v_opacity_logits[i], glogit, A.lr_opacity,
-                    A.beta1, A.beta2, c1, c2, A.epsilon, -12.0f, 12.0f, false);
-        if (A.sh && !precomp) {
-#pragma unroll
-            for (int k = 0; k < NB * 3; ++k) {
-                const int64_t j = row + k;
-                adam_update(A.sh[j], A.m_sh[j], A.v_sh[j], gsh[k], k < 3 ? A.lr_sh0 : A.lr_sh_rest, A.beta1,
+        if (A.sh && use_sh) {
+            float* P = A.sh + row0;
+            float* M = A.m_sh + row0;
+            float* V = A.v_sh + row0;
+            uint16_t* L = (LP != PHT_LP_NONE && A.sh_lp) ? A.sh_lp + row0 : nullptr;
+            for_block_rows(rows, stride, NB * 3, [&](int e, int row, int k) {
+                float pv = P[e], mv = M[e], vv = V[e];
+                adam_update(pv, mv, vv, s_sh[row * kShPitch + k], k < 3 ? A.lr_sh0 : A.lr_sh_rest, A.beta1,
                             A.beta2, c1, c2, A.epsilon, -inf, inf, true);
-            }
-        }
-        if constexpr (LP != PHT_LP_NONE) {
-            for (int k = 0; k < 3; ++k)
-                if (A.log_scales_lp) A.log_scales_lp[3 * i + k] = f32_to_lp(A.log_scales[3 * i + k], LP);
-            if (A.opacity_logits_lp) A.opacity_logits_lp[i] = f32_to_lp(A.opacity_logits[i], LP);
-            if (A.sh_lp && A.sh && !precomp) {
-#pragma unroll
-                for (int k = 0; k < NB * 3; ++k) A.sh_lp[row + k] = f32_to_lp(A.sh[row + k], LP);
-            }
+                P[e] = pv;
+                M[e] = mv;
+                V[e] = vv;
+                if constexpr (LP != PHT_LP_NONE) {
+                    if (L) L[e] = f32_to_lp(pv, LP);
+                }
+            });
         }
     }
 }
@@ -175,20 +224,46 @@ void launch_adam_standalone(float* p, const float* g, float* m, float* v, int64_
 }
 
 // ---------------------------------------------------------------- L1 loss
-// loss_kernel L1 part (photara/src/splat/cuda_ops.cu:886-898), no mask, full image.
+// loss_kernel L1 part (photara/src/splat/cuda_ops.cu:886-898), no mask, full image. A bounded
+// grid strides over the pixels (four at a time when the planes are 16-byte aligned), so the
+// optional loss sum takes one atomic per block.
+__device__ __forceinline__ float l1_term(float c, float t, float scale, float& g) {
+    const float d = c - t;
+    g = scale * float((d > 0.0f) - (d < 0.0f));
+    return fabsf(d);
+}
+
+template <bool VEC>
 __global__ __launch_bounds__(256) void k_l1(const float* color, const float* target, float* grad, float* loss_sum,
                                             int64_t pixels, float weight) {
     __shared__ float s_w[4];
-    const int64_t p = int64_t(blockIdx.x) * 256 + threadIdx.x;
     const float inv_p = 1.0f / float(pixels);
     const float scale = (weight * inv_p) / 3.0f;
+    const int64_t stride = int64_t(gridDim.x) * 256;
     float part = 0.0f;
-    if (p < pixels) {
-        for (int ch = 0; ch < 3; ++ch) {
-            const int64_t o = ch * pixels + p;
-            const float d = color[o] - target[o];
-            part += fabsf(d);
-            grad[o] = scale * float((d > 0.0f) - (d < 0.0f));
+    if constexpr (VEC) {
+        const int64_t n4 = pixels >> 2;
+        for (int64_t q = int64_t(blockIdx.x) * 256 + threadIdx.x; q < n4; q += stride) {
+            for (int ch = 0; ch < 3; ++ch) {
+                const int64_t o = (ch * pixels >> 2) + q;
+                const float4 c = reinterpret_cast<const float4*>(color)[o];
+                const float4 t = reinterpret_cast<const float4*>(target)[o];
+                float4 g;
+                part += l1_term(c.x, t.x, scale, g.x);
+                part += l1_term(c.y, t.y, scale, g.y);
+                part += l1_term(c.z, t.z, scale, g.z);
+                part += l1_term(c.w, t.w, scale, g.w);
+                reinterpret_cast<float4*>(grad)[o] = g;
+            }
+        }
+    } else {
+        for (int64_t p = int64_t(blockIdx.x) * 256 + threadIdx.x; p < pixels; p += stride) {
+            for (int ch = 0; ch < 3; ++ch) {
+                const int64_t o = ch * pixels + p;
+                float g;
+                part += l1_term(color[o], target[o], scale, g);
+                grad[o] = g;
+            }
         }
     }
     if (loss_sum) {
@@ -199,16 +274,21 @@ __global__ __launch_bounds__(256) void k_l1(const float* color, const float* tar
     }
 }
 
-__global__ void k_zero_f32(float* p, int n) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < n) p[i] = 0.0f;
-}
-
 void launch_l1_loss(const float* color, const float* target, float* grad, float* loss_sum, int64_t pixels,
                     float weight, hipStream_t st) {
     if (pixels <= 0) return;
-    hipLaunchKernelGGL(k_l1, dim3(unsigned((pixels + 255) / 256)), dim3(256), 0, st, color, target, grad, loss_sum,
-                       pixels, weight);
+    const bool vec = (pixels & 3) == 0 && ((reinterpret_cast<uintptr_t>(color) | reinterpret_cast<uintptr_t>(target) |
+                                            reinterpret_cast<uintptr_t>(grad)) & 15u) == 0;
+    const int64_t items = vec ? pixels >> 2 : pixels;
+    int64_t blocks = (items + 255) / 256;
+    if (blocks > 512) blocks = 512;
+    if (vec) hipLaunchKernelGGL(k_l1<true>, dim3(unsigned(blocks)), dim3(256), 0, st, color, target, grad, loss_sum, pixels, weight);
+    else hipLaunchKernelGGL(k_l1<false>, dim3(unsigned(blocks)), dim3(256), 0, st, color, target, grad, loss_sum, pixels, weight);
+}
+
+__global__ void k_zero_f32(float* p, int n) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
 }
 
 void launch_zero_f32(float* p, int n, hipStream_t st) {

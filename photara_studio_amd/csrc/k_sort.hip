@@ -7,8 +7,8 @@
 // own one contiguous chunk of the list; within a chunk elements are processed in order, which
 // makes every scan exclusive-prefix exact and every sort pass stable:
 //   reduce  : per-block partial (sum, or 256-bin digit histogram)
-//   scan    : one block turns the partials into exclusive block offsets
-//   write   : per-block in-order rescan / wave-level multi-split ranking + scatter
+//   scan    : partials -> exclusive block offsets (one block; one block per digit for the sort)
+//   write   : per-block in-order rescan / LDS tile ranking + per-digit contiguous writes
 #include "launch.h"
 
 namespace pht {
@@ -49,19 +49,21 @@ struct SrcFlags { // visibility flags of the dense slot array
     const uint32_t* flags;
     __device__ uint32_t operator()(uint32_t i) const { return flags[i]; }
 };
-struct SrcTiles { // tile counts gathered in depth order
-    const uint32_t* sorted_slots;
-    const uint4* rec_c;
-    __device__ uint32_t operator()(uint32_t i) const { return rec_c[sorted_slots[i]].z; }
+struct SrcTiles { // tile counts, already permuted into depth order by the depth sort
+    const uint32_t* counts;
+    __device__ uint32_t operator()(uint32_t i) const { return counts[i]; }
 };
-struct SinkCompact { // visible slot -> (depth key, slot) pair at its compact position
+struct SinkCompact { // visible slot -> (depth key, slot, tile count) at its compact position
     const uint4* rec_c;
     uint32_t* keys;
     uint32_t* vals;
+    uint32_t* tiles;
     __device__ void operator()(uint32_t i, uint32_t excl, uint32_t v) const {
         if (v) {
-            keys[excl] = rec_c[i].y;
+            const uint4 c = rec_c[i];
+            keys[excl] = c.y;
             vals[excl] = i;
+            tiles[excl] = c.z;
         }
     }
 };
@@ -127,94 +129,154 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_write(Src src, Sink sink,
 }
 
 // ---------------------------------------------------------------- radix sort (8-bit digits)
+// One pass = upsweep (per-block digit histogram of its chunk), digit scan (one block per digit
+// scans that digit's counts over the blocks and stores the digit total) and downsweep. The
+// downsweep ranks tiles of kSortTile keys in LDS: each wave ranks its 1024 keys row by row with a
+// wave-level multi-split (8 ballots) against per-wave running digit counters, the tile is
+// reordered by digit in LDS, and each digit's run is written out contiguously.
+constexpr int kSortRows = 16;                      // keys per lane and tile
+constexpr int kSortTile = kScanThreads * kSortRows; // 4096 keys per block iteration
+
 __global__ __launch_bounds__(kScanThreads) void k_radix_upsweep(const uint32_t* keys, const uint32_t* n_ptr,
                                                                 uint32_t n_const, uint32_t cap, int shift, int nb,
                                                                 uint32_t* hist) {
-    __shared__ uint32_t s_h[256];
-    s_h[threadIdx.x] = 0;
+    __shared__ uint32_t s_h[4][256];
+    const int t = threadIdx.x, w = t >> 6;
+    for (int k = 0; k < 4; ++k) s_h[k][t] = 0;
     __syncthreads();
     const uint32_t n = load_count(n_ptr, n_const, cap);
     uint32_t lo, hi;
     chunk_of(n, nb, blockIdx.x, lo, hi);
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += kScanThreads)
-        atomicAdd(&s_h[(keys[i] >> shift) & 255u], 1u);
+    // lo is a multiple of 256 keys, so the uint4 view of [lo, hi4) is 16-byte aligned
+    const uint32_t hi4 = lo + ((hi - lo) & ~3u);
+    const uint4* k4 = reinterpret_cast<const uint4*>(keys);
+    for (uint32_t i = lo + 4u * t; i < hi4; i += 4u * kScanThreads) {
+        const uint4 q = k4[i >> 2];
+        atomicAdd(&s_h[w][(q.x >> shift) & 255u], 1u);
+        atomicAdd(&s_h[w][(q.y >> shift) & 255u], 1u);
+        atomicAdd(&s_h[w][(q.z >> shift) & 255u], 1u);
+        atomicAdd(&s_h[w][(q.w >> shift) & 255u], 1u);
+    }
+    for (uint32_t i = hi4 + t; i < hi; i += kScanThreads) atomicAdd(&s_h[w][(keys[i] >> shift) & 255u], 1u);
     __syncthreads();
-    hist[threadIdx.x * nb + blockIdx.x] = s_h[threadIdx.x];
+    hist[size_t(t) * nb + blockIdx.x] = (s_h[0][t] + s_h[1][t]) + (s_h[2][t] + s_h[3][t]);
 }
 
-// Exclusive scan of the digit-major histogram hist[256 * nb] by one block of 1024 threads.
-__global__ __launch_bounds__(1024) void k_radix_scan(uint32_t* hist, int nb) {
+// Block d (256 blocks of 1024 threads): exclusive scan of digit d's counts over the nb <= 1024
+// blocks, in place; the digit's total goes to digit_total[d].
+__global__ __launch_bounds__(1024) void k_radix_scan(uint32_t* hist, int nb, uint32_t* digit_total) {
     __shared__ uint32_t s_w[16];
-    const int total = 256 * nb;
-    const int per = (total + 1023) / 1024;
     const int t = threadIdx.x, lane = lane_id(), w = t >> 6;
-    const int b = t * per, e = min(total, b + per);
-    uint32_t sum = 0;
-    for (int i = b; i < e; ++i) sum += hist[i];
-    const uint32_t inc = wave_inclusive_scan(sum);
+    uint32_t* row = hist + size_t(blockIdx.x) * nb;
+    const uint32_t v = t < nb ? row[t] : 0u;
+    const uint32_t inc = wave_inclusive_scan(v);
     if (lane == 63) s_w[w] = inc;
     __syncthreads();
-    uint32_t before = 0;
-    for (int k = 0; k < w; ++k) before += s_w[k];
-    uint32_t run = inc - sum + before;
-    for (int i = b; i < e; ++i) {
-        const uint32_t x = hist[i];
-        hist[i] = run;
-        run += x;
+    uint32_t before = 0, total = 0;
+    for (int k = 0; k < 16; ++k) {
+        const uint32_t x = s_w[k];
+        if (k < w) before += x;
+        total += x;
     }
+    if (t < nb) row[t] = inc + before - v;
+    if (t == 0) digit_total[blockIdx.x] = total;
 }
 
-// Stable scatter of one chunk. Tiles of 256 keys are ranked with a wave-level multi-split
-// (8 ballots give each lane the mask of same-digit lanes), per-wave digit counts are combined
-// across the 4 waves in LDS, and a running per-digit base carries the order across tiles.
+template <bool AUX>
 __global__ __launch_bounds__(kScanThreads) void k_radix_downsweep(const uint32_t* keys_in, const uint32_t* vals_in,
-                                                                  uint32_t* keys_out, uint32_t* vals_out,
+                                                                  const uint32_t* aux_in, uint32_t* keys_out,
+                                                                  uint32_t* vals_out, uint32_t* aux_out,
                                                                   const uint32_t* n_ptr, uint32_t n_const,
                                                                   uint32_t cap, int shift, int nb,
-                                                                  const uint32_t* hist) {
-    __shared__ uint32_t s_base[256];
-    __shared__ uint32_t s_wh[4][256];
+                                                                  const uint32_t* hist, const uint32_t* digit_total) {
+    __shared__ uint32_t s_key[kSortTile];
+    __shared__ uint32_t s_val[kSortTile];
+    __shared__ uint32_t s_aux[AUX ? kSortTile : 1];
+    __shared__ uint32_t s_cnt[4][256];  // per-wave running digit counts, then per-wave offsets
+    __shared__ uint32_t s_start[256];   // digit start inside the reordered tile
+    __shared__ uint32_t s_gbase[256];   // global position of the tile's first key of each digit
+    __shared__ uint32_t s_w[4];
     const int t = threadIdx.x, lane = lane_id(), w = t >> 6;
-    s_base[t] = hist[t * nb + blockIdx.x];
+    // global digit base of this block = exclusive digit prefix + this block's offset in the digit
+    uint32_t run;
+    {
+        const uint32_t tot = digit_total[t];
+        uint32_t all;
+        const uint32_t inc = block_inclusive_scan(tot, s_w, all);
+        run = inc - tot + hist[size_t(t) * nb + blockIdx.x];
+    }
     const uint32_t n = load_count(n_ptr, n_const, cap);
     uint32_t lo, hi;
     chunk_of(n, nb, blockIdx.x, lo, hi);
     const uint64_t lt = __lanemask_lt();
-    for (uint32_t base = lo; base < hi; base += kScanThreads) {
-        for (int k = 0; k < 4; ++k) s_wh[k][t] = 0;
+    for (uint32_t base = lo; base < hi; base += kSortTile) {
+        for (int k = 0; k < 4; ++k) s_cnt[k][t] = 0;
         __syncthreads();
-        const uint32_t i = base + t;
-        const bool valid = i < hi;
-        uint32_t key = 0, val = 0, d = 0;
-        if (valid) {
-            key = keys_in[i];
-            val = vals_in[i];
-            d = (key >> shift) & 255u;
-        }
-        uint64_t peers = __ballot(valid);
+        const uint32_t wbase = base + uint32_t(w) * (kSortTile / 4);
+        uint32_t key[kSortRows], val[kSortRows], aux[kSortRows], rank[kSortRows];
 #pragma unroll
-        for (int bit = 0; bit < 8; ++bit) {
-            const bool set = (d >> bit) & 1u;
-            const uint64_t bal = __ballot(valid && set);
-            peers &= set ? bal : ~bal;
+        for (int r = 0; r < kSortRows; ++r) {
+            const uint32_t i = wbase + uint32_t(r) * 64u + uint32_t(lane);
+            const bool valid = i < hi;
+            key[r] = valid ? keys_in[i] : 0u;
+            val[r] = valid ? vals_in[i] : 0u;
+            if constexpr (AUX) aux[r] = valid ? aux_in[i] : 0u;
         }
-        const uint32_t rank = uint32_t(__popcll(peers & lt));
-        if (valid && (peers & lt) == 0) s_wh[w][d] = uint32_t(__popcll(peers));
-        __syncthreads();
-        {
-            uint32_t s = s_base[t];
-            for (int k = 0; k < 4; ++k) {
-                const uint32_t c = s_wh[k][t];
-                s_wh[k][t] = s;
-                s += c;
+#pragma unroll
+        for (int r = 0; r < kSortRows; ++r) {
+            const uint32_t i = wbase + uint32_t(r) * 64u + uint32_t(lane);
+            const bool valid = i < hi;
+            const uint32_t d = (key[r] >> shift) & 255u;
+            uint64_t peers = __ballot(valid);
+#pragma unroll
+            for (int bit = 0; bit < 8; ++bit) {
+                const bool set = (d >> bit) & 1u;
+                const uint64_t bal = __ballot(valid && set);
+                peers &= set ? bal : ~bal;
             }
-            s_base[t] = s;
+            const uint32_t before = valid ? s_cnt[w][d] : 0u;
+            rank[r] = before + uint32_t(__popcll(peers & lt));
+            // the highest lane of each digit group advances the wave's running count
+            if (valid && (peers >> lane) == 1ull) s_cnt[w][d] = before + uint32_t(__popcll(peers));
+            __builtin_amdgcn_wave_barrier();
         }
         __syncthreads();
-        if (valid) {
-            const uint32_t dst = s_wh[w][d] + rank;
-            keys_out[dst] = key;
-            vals_out[dst] = val;
+        // thread t = digit t: per-wave offsets inside the digit, digit starts inside the tile
+        {
+            uint32_t c[4], s = 0;
+            for (int k = 0; k < 4; ++k) {
+                c[k] = s_cnt[k][t];
+                s_cnt[k][t] = s;
+                s += c[k];
+            }
+            uint32_t all;
+            const uint32_t inc = block_inclusive_scan(s, s_w, all);
+            s_start[t] = inc - s;
+            s_gbase[t] = run;
+            run += s;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kSortRows; ++r) {
+            const uint32_t i = wbase + uint32_t(r) * 64u + uint32_t(lane);
+            if (i < hi) {
+                const uint32_t d = (key[r] >> shift) & 255u;
+                const uint32_t p = s_start[d] + s_cnt[w][d] + rank[r];
+                s_key[p] = key[r];
+                s_val[p] = val[r];
+                if constexpr (AUX) s_aux[p] = aux[r];
+            }
+        }
+        __syncthreads();
+        const uint32_t tile_n = hi - base < uint32_t(kSortTile) ? hi - base : uint32_t(kSortTile);
+        for (uint32_t j = t; j < tile_n; j += kScanThreads) {
+            const uint32_t k = s_key[j];
+            const uint32_t d = (k >> shift) & 255u;
+            const uint32_t dst = s_gbase[d] + (j - s_start[d]);
+            if (dst >= n) continue; // unreachable for consistent ranks; keeps every store in bounds
+            keys_out[dst] = k;
+            vals_out[dst] = s_val[j];
+            if constexpr (AUX) aux_out[dst] = s_aux[j];
         }
         __syncthreads();
     }
@@ -229,7 +291,7 @@ static int blocks_for(uint32_t capacity) {
 int scan_blocks_for(uint32_t capacity) { return blocks_for(capacity); }
 
 void launch_compact_visible(const uint32_t* flags, const uint4* rec_c, uint32_t n_slots, uint32_t* keys,
-                            uint32_t* vals, uint32_t* block_scratch, uint32_t* visible_out,
+                            uint32_t* vals, uint32_t* tiles, uint32_t* block_scratch, uint32_t* visible_out,
                             hipStream_t st) {
     const int nb = blocks_for(n_slots);
     SrcFlags src{flags};
@@ -237,16 +299,16 @@ void launch_compact_visible(const uint32_t* flags, const uint4* rec_c, uint32_t 
                        n_slots, nb, block_scratch);
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st, block_scratch, nb, visible_out, nullptr,
                        0xffffffffu);
-    SinkCompact sink{rec_c, keys, vals};
+    SinkCompact sink{rec_c, keys, vals, tiles};
     hipLaunchKernelGGL((k_scan_write<SrcFlags, SinkCompact>), dim3(nb), dim3(kScanThreads), 0, st, src, sink,
                        nullptr, n_slots, n_slots, nb, block_scratch);
 }
 
-void launch_tile_offsets(const uint32_t* sorted_slots, const uint4* rec_c, const uint32_t* visible_ptr,
-                         uint32_t slot_cap, uint32_t* offsets, uint32_t* block_scratch,
-                         uint32_t* instances_out, uint32_t* overflow_out, uint32_t inst_cap, hipStream_t st) {
+void launch_tile_offsets(const uint32_t* sorted_tiles, const uint32_t* visible_ptr, uint32_t slot_cap,
+                         uint32_t* offsets, uint32_t* block_scratch, uint32_t* instances_out,
+                         uint32_t* overflow_out, uint32_t inst_cap, hipStream_t st) {
     const int nb = blocks_for(slot_cap);
-    SrcTiles src{sorted_slots, rec_c};
+    SrcTiles src{sorted_tiles};
     hipLaunchKernelGGL(k_scan_reduce<SrcTiles>, dim3(nb), dim3(kScanThreads), 0, st, src, visible_ptr, 0u,
                        slot_cap, nb, block_scratch);
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st, block_scratch, nb, instances_out,
@@ -256,16 +318,22 @@ void launch_tile_offsets(const uint32_t* sorted_slots, const uint4* rec_c, const
                        visible_ptr, 0u, slot_cap, nb, block_scratch);
 }
 
-int launch_radix_sort(uint32_t* keys[2], uint32_t* vals[2], const uint32_t* n_ptr, uint32_t cap, int bits,
-                      uint32_t* hist, hipStream_t st) {
+int launch_radix_sort(uint32_t* keys[2], uint32_t* vals[2], uint32_t* aux[2], const uint32_t* n_ptr, uint32_t cap,
+                      int bits, uint32_t* hist, uint32_t* digit_total, hipStream_t st) {
     const int nb = blocks_for(cap);
     int cur = 0;
     for (int shift = 0; shift < bits; shift += 8) {
         hipLaunchKernelGGL(k_radix_upsweep, dim3(nb), dim3(kScanThreads), 0, st, keys[cur], n_ptr, 0u, cap, shift,
                            nb, hist);
-        hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, st, hist, nb);
-        hipLaunchKernelGGL(k_radix_downsweep, dim3(nb), dim3(kScanThreads), 0, st, keys[cur], vals[cur],
-                           keys[cur ^ 1], vals[cur ^ 1], n_ptr, 0u, cap, shift, nb, hist);
+        hipLaunchKernelGGL(k_radix_scan, dim3(256), dim3(1024), 0, st, hist, nb, digit_total);
+        if (aux)
+            hipLaunchKernelGGL(k_radix_downsweep<true>, dim3(nb), dim3(kScanThreads), 0, st, keys[cur], vals[cur],
+                               aux[cur], keys[cur ^ 1], vals[cur ^ 1], aux[cur ^ 1], n_ptr, 0u, cap, shift, nb,
+                               hist, digit_total);
+        else
+            hipLaunchKernelGGL(k_radix_downsweep<false>, dim3(nb), dim3(kScanThreads), 0, st, keys[cur], vals[cur],
+                               nullptr, keys[cur ^ 1], vals[cur ^ 1], nullptr, n_ptr, 0u, cap, shift, nb, hist,
+                               digit_total);
         cur ^= 1;
     }
     return cur;

@@ -54,15 +54,20 @@ struct GaussArgs {
 
 // k_sort.hip
 int scan_blocks_for(uint32_t capacity);
+// Compacts the visible slots (slot order) into (depth key, slot, tile count) lists.
 void launch_compact_visible(const uint32_t* flags, const uint4* rec_c, uint32_t n_slots, uint32_t* keys,
-                            uint32_t* vals, uint32_t* block_scratch, uint32_t* visible_out, hipStream_t st);
-void launch_tile_offsets(const uint32_t* sorted_slots, const uint4* rec_c, const uint32_t* visible_ptr,
-                         uint32_t slot_cap, uint32_t* offsets, uint32_t* block_scratch,
-                         uint32_t* instances_out, uint32_t* overflow_out, uint32_t inst_cap, hipStream_t st);
-// Sorts (keys, vals) pairs by the low `bits` bits of the keys; returns the buffer index (0/1)
-// holding the result.
-int launch_radix_sort(uint32_t* keys[2], uint32_t* vals[2], const uint32_t* n_ptr, uint32_t cap, int bits,
-                      uint32_t* hist, hipStream_t st);
+                            uint32_t* vals, uint32_t* tiles, uint32_t* block_scratch, uint32_t* visible_out,
+                            hipStream_t st);
+// Exclusive scan of the depth-ordered tile counts into instance offsets; stores the instance
+// total and the overflow flag (total > inst_cap).
+void launch_tile_offsets(const uint32_t* sorted_tiles, const uint32_t* visible_ptr, uint32_t slot_cap,
+                         uint32_t* offsets, uint32_t* block_scratch, uint32_t* instances_out,
+                         uint32_t* overflow_out, uint32_t inst_cap, hipStream_t st);
+// Stable sort of (keys, vals[, aux]) by the low `bits` bits of the keys (aux may be null);
+// returns the buffer index (0/1) holding the result. hist holds 256 * kScanBlocks counts,
+// digit_total 256.
+int launch_radix_sort(uint32_t* keys[2], uint32_t* vals[2], uint32_t* aux[2], const uint32_t* n_ptr, uint32_t cap,
+                      int bits, uint32_t* hist, uint32_t* digit_total, hipStream_t st);
 
 // k_preprocess.hip
 void launch_upload_views(DevView* dst, const DevView* host_views, int n_views, hipStream_t st);
