@@ -150,8 +150,12 @@ def main():
     p.add_argument("--plain-steps", type=int, default=0, help="plain-step count (default: 4 x steps)")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--workload", choices=("ring", "c2"), default="ring",
+                   help="ring: --views 1920x1080 views on a radius-8 ring; c2: 2x1080p + 3x720p + 3x540p")
     p.add_argument("--tile", type=int, default=16)
-    p.add_argument("--batch", type=int, default=256)
+    p.add_argument("--batch", type=int, default=64)
+    p.add_argument("--autotune", action="store_true",
+                   help="sweep tile {16,8} x batch {32,64,128,256} for the batched step and time the fastest")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="first copy 1 GiB device-to-device (a kernel of known traffic for counter scaling)")
     args = p.parse_args()
@@ -175,10 +179,12 @@ def main():
         torch.cuda.synchronize()
         print("[bench] calibration copy: 1073741824 bytes read + written", flush=True)
         del src, dst
-    cams = workload_cameras(f"bench{nv}")
+    cams = workload_cameras("c2" if args.workload == "c2" else f"bench{nv}")
+    nv = len(cams)
+    res_label = ",".join(sorted({f"{c.width}x{c.height}" for c in cams}, reverse=True))
     opts = RasterizeOptions(active_sh_degree=3, kernel_size=0.3, tile_size=args.tile, batch=args.batch)
     opts.rasterization.compensate_2d = False
-    print(f"[bench] {n} gaussians, {nv} views 1920x1080, tile {args.tile}, batch {args.batch}", flush=True)
+    print(f"[bench] {n} gaussians, {nv} views ({res_label}), tile {args.tile}, batch {args.batch}", flush=True)
     targets = make_targets(n, cams, opts, dev, seed=1)
     arrays = synthetic_torch(n, 3.0, 0, dev)
 
@@ -196,7 +202,14 @@ def main():
 
     # batched BF16 graph step: all views per step
     model = to_low_precision(model_from(arrays, dev), torch.bfloat16)
-    batched = TrainStep(model, cams, targets, opts, LearningRates())
+    bopts = opts
+    tune_rows = None
+    if args.autotune:
+        from photara_studio_amd.autotune import autotune
+        bopts, tune_rows = autotune(model, cams, targets, opts, LearningRates(), steps=3, warmup=1,
+                                    log=lambda m: print(m, flush=True))
+        print(f"[bench] autotuned: tile {bopts.tile_size}, batch {bopts.batch}", flush=True)
+    batched = TrainStep(model, cams, targets, bopts, LearningRates())
     batched.ensure_capacity()
     batched.capture()
     step_s = timed(batched.step_graph, args.steps, args.warmup)
@@ -214,12 +227,14 @@ def main():
             acc[k] = acc.get(k, 0.0) + v / prof_steps
     batched.ws.set_profiling(False)
     pixels = sum(c.width * c.height for c in cams)
-    tiles = sum(((c.width + args.tile - 1) // args.tile) * ((c.height + args.tile - 1) // args.tile) for c in cams)
+    bt = int(bopts.tile_size)
+    tiles = sum(((c.width + bt - 1) // bt) * ((c.height + bt - 1) // bt) for c in cams)
     ab = algorithmic_bytes(n, nv, int(bc.visible), int(bc.instances), pixels, tiles, 2, True)
     dominant = max(acc, key=acc.get)
     dom_gbps = ab[dominant] / (acc[dominant] * 1e-3) / 1e9
     step_bytes = sum(ab.values())
-    workload_key = f"{n // 1_000_000}M-{nv}x1080p-bf16" if n % 1_000_000 == 0 else f"{n}-{nv}x1080p-bf16"
+    wname = "c2mixed" if args.workload == "c2" else f"{nv}x1080p"
+    workload_key = f"{n // 1_000_000}M-{wname}-bf16" if n % 1_000_000 == 0 else f"{n}-{wname}-bf16"
     traffic, traffic_src = profiled_traffic(dominant, workload_key)
     print(f"[bench] stages ms: " + ", ".join(f"{k}={v:.3f}" for k, v in acc.items()), flush=True)
 
@@ -243,9 +258,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"3dgs-train-step {n} gaussians x {nv} views 1920x1080, bf16 storage, hipGraph",
-                   "gaussians": n, "views": nv, "resolution": "1920x1080", "tile": args.tile, "batch": args.batch,
-                   "storage": "bf16 sh/log_scales/opacity, fp32 masters", "parallelism": "none (1 GPU)"},
+        "config": {"workload": f"3dgs-train-step {n} gaussians x {nv} views {res_label}, bf16 storage, hipGraph",
+                   "gaussians": n, "views": nv, "resolution": res_label, "tile": int(bopts.tile_size),
+                   "batch": int(bopts.batch), "storage": "bf16 sh/log_scales/opacity, fp32 masters",
+                   "parallelism": "none (1 GPU)"},
+        "autotune": [[r.tile_size, r.batch, round(r.ms_per_step, 3)] for r in tune_rows] if tune_rows else None,
         "plain_fp32": {"ms_per_step": round(plain_s * 1e3, 4), "steps_per_s": round(1.0 / plain_s, 3),
                        "instances": int(pc.instances)},
         "speedup_vs_plain_fp32": round((nv / step_s) * plain_s, 3),

@@ -322,8 +322,8 @@ void run_blend_backward(const pht_gaussians& g, const pht_settings& s, const Sta
                     if (!pair_alpha(r.conic[0], r.conic[1], r.conic[2], r.opacity, dx, dy, clip, alpha, G))
                         continue;
                     const T one_m = T(1) - alpha;
-                    Tr = Tr / one_m;
                     const T inv_1ma = T(1) / one_m;
+                    Tr = Tr * inv_1ma;
                     const T w = alpha * Tr;
                     const T c_dot = (r.rgb[0] * gp[0] + r.rgb[1] * gp[1]) + r.rgb[2] * gp[2];
                     const T a_dot = ((Ca[0] * gp[0] + Ca[1] * gp[1]) + Ca[2] * gp[2]) * inv_1ma;
@@ -500,6 +500,9 @@ void oracle_set_inputs64(const double* means, const double* log_scales, const do
 // Deterministic math primitives, exported for the error-bound tests.
 void oracle_pexpf(const float* x, float* y, int64_t n) {
     for (int64_t i = 0; i < n; ++i) y[i] = pexpf(x[i]);
+}
+void oracle_pexpf_inrange(const float* x, float* y, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) y[i] = pexpf_inrange(x[i]);
 }
 void oracle_plogf(const float* x, float* y, int64_t n) {
     for (int64_t i = 0; i < n; ++i) y[i] = plogf(x[i]);

@@ -28,6 +28,7 @@ def load() -> C.CDLL:
         lib = C.CDLL(str(LIB))
         vp = C.c_void_p
         lib.oracle_pexpf.argtypes = [vp, vp, C.c_int64]
+        lib.oracle_pexpf_inrange.argtypes = [vp, vp, C.c_int64]
         lib.oracle_plogf.argtypes = [vp, vp, C.c_int64]
         lib.oracle_gaussian_power.argtypes = [vp, vp, vp, C.c_int64]
         lib.oracle_lp_roundtrip.argtypes = [vp, vp, vp, C.c_int64, C.c_int]
@@ -253,6 +254,13 @@ def pexpf(x: np.ndarray) -> np.ndarray:
     x = f32(x)
     y = np.empty_like(x)
     load().oracle_pexpf(_p(x), _p(y), x.size)
+    return y
+
+
+def pexpf_inrange(x: np.ndarray) -> np.ndarray:
+    x = f32(x)
+    y = np.empty_like(x)
+    load().oracle_pexpf_inrange(_p(x), _p(y), x.size)
     return y
 
 

@@ -61,9 +61,8 @@ struct pht_workspace {
     uint32_t slot_cap = 0, inst_cap = 0;
     DevView* views = nullptr;
     Counters* counters = nullptr;
-    float4* rec_a = nullptr;
-    float4* rec_b = nullptr;
-    uint4* rec_c = nullptr;
+    SplatRec* rec = nullptr;
+    uint4* meta = nullptr;
     uint32_t* vis_flag = nullptr;
     float* sgrad = nullptr;
     uint32_t* view_mask = nullptr;
@@ -109,9 +108,8 @@ static void carve_workspace(pht_workspace& w, char* base, const pht_capacity& ca
     const size_t inst = size_t(cap.instances);
     w.views = c.take<DevView>(PHT_MAX_VIEWS);
     w.counters = c.take<Counters>(1);
-    w.rec_a = c.take<float4>(slots);
-    w.rec_b = c.take<float4>(slots);
-    w.rec_c = c.take<uint4>(slots);
+    w.rec = c.take<SplatRec>(slots);
+    w.meta = c.take<uint4>(slots);
     w.vis_flag = c.take<uint32_t>(slots);
     w.sgrad = c.take<float>(slots * kGradStride);
     w.view_mask = c.take<uint32_t>(size_t(cap.gaussians));
@@ -242,6 +240,8 @@ int pht_forward(pht_workspace* ws, const pht_gaussians* g, const pht_camera* cam
         d.vi.cam = c;
         d.vi.grid_x = (c.width + ts - 1) / ts;
         d.vi.grid_y = (c.height + ts - 1) / ts;
+        if (d.vi.grid_x >= 8192 || d.vi.grid_y >= 8192)
+            return fail(PHT_EINVAL, "camera too large: at most 8191 tiles per image axis");
         d.vi.tile_base = int32_t(tiles);
         d.vi.tile_count = d.vi.grid_x * d.vi.grid_y;
         d.vi.pixel_base = pixels;
@@ -264,9 +264,8 @@ int pht_forward(pht_workspace* ws, const pht_gaussians* g, const pht_camera* cam
     pa.views = ws->views;
     pa.n_views = n_views;
     pa.vp_log2 = ws->vp_log2;
-    pa.rec_a = ws->rec_a;
-    pa.rec_b = ws->rec_b;
-    pa.rec_c = ws->rec_c;
+    pa.rec = ws->rec;
+    pa.meta = ws->meta;
     pa.vis_flag = ws->vis_flag;
     pa.sgrad = ws->sgrad;
     pa.view_mask = ws->view_mask;
@@ -275,7 +274,7 @@ int pht_forward(pht_workspace* ws, const pht_gaussians* g, const pht_camera* cam
     launch_preprocess(pa, n_slots, st);
     ws->mark(0, 1, st);
     ws->mark(1, 0, st);
-    launch_compact_visible(ws->vis_flag, ws->rec_c, uint32_t(n_slots), ws->dkeys[0], ws->dvals[0], ws->dtiles[0],
+    launch_compact_visible(ws->vis_flag, ws->meta, uint32_t(n_slots), ws->dkeys[0], ws->dvals[0], ws->dtiles[0],
                            ws->scan_scratch, &ws->counters->visible, st);
     const int dsel = launch_radix_sort(ws->dkeys, ws->dvals, ws->dtiles, &ws->counters->visible, uint32_t(n_slots),
                                        32, ws->hist, ws->digit_total, st);
@@ -283,8 +282,8 @@ int pht_forward(pht_workspace* ws, const pht_gaussians* g, const pht_camera* cam
     ws->mark(2, 0, st);
     launch_tile_offsets(ws->dtiles[dsel], &ws->counters->visible, uint32_t(n_slots), ws->offsets, ws->scan_scratch,
                         &ws->counters->instances, &ws->counters->overflow, ws->inst_cap, st);
-    launch_emit_instances(ws->dvals[dsel], ws->offsets, ws->counters, uint32_t(n_slots), ws->rec_a, ws->rec_b,
-                          ws->rec_c, ws->views, ws->vp_log2, ts, ws->inst_cap, ws->tkeys[0], ws->tvals[0], st);
+    launch_emit_instances(ws->dvals[dsel], ws->dtiles[dsel], ws->offsets, ws->counters, uint32_t(n_slots), ws->rec,
+                          ws->views, ws->vp_log2, ts, ws->inst_cap, ws->tkeys[0], ws->tvals[0], st);
     ws->mark(2, 1, st);
     ws->mark(3, 0, st);
     const int tbits = bits_for(uint64_t(tiles - 1));
@@ -295,9 +294,7 @@ int pht_forward(pht_workspace* ws, const pht_gaussians* g, const pht_camera* cam
     BlendArgs ba;
     ba.ranges = ws->ranges;
     ba.inst_vals = ws->tvals[tsel];
-    ba.rec_a = ws->rec_a;
-    ba.rec_b = ws->rec_b;
-    ba.rec_c = ws->rec_c;
+    ba.rec = ws->rec;
     ba.views = ws->views;
     ba.n_views = n_views;
     ba.vp_log2 = ws->vp_log2;
@@ -359,9 +356,7 @@ int pht_backward(pht_workspace* ws, const pht_gaussians* g, const pht_view_grads
     BlendArgs ba;
     ba.ranges = ws->ranges;
     ba.inst_vals = ws->tvals[ws->tile_sel];
-    ba.rec_a = ws->rec_a;
-    ba.rec_b = ws->rec_b;
-    ba.rec_c = ws->rec_c;
+    ba.rec = ws->rec;
     ba.views = ws->views;
     ba.n_views = ws->n_views;
     ba.vp_log2 = ws->vp_log2;
@@ -380,7 +375,7 @@ int pht_backward(pht_workspace* ws, const pht_gaussians* g, const pht_view_grads
     ga.views = ws->views;
     ga.n_views = ws->n_views;
     ga.vp_log2 = ws->vp_log2;
-    ga.rec_c = ws->rec_c;
+    ga.meta = ws->meta;
     ga.sgrad = ws->sgrad;
     ga.view_mask = ws->view_mask;
     ga.counters = ws->counters;
@@ -455,7 +450,7 @@ int pht_debug_lists(pht_workspace* ws, uint64_t* keys, uint32_t* values, uint32_
     if (!ws->fwd_valid) return fail(PHT_ESTATE, "no forward recorded");
     hipStream_t st = static_cast<hipStream_t>(stream);
     if (keys || values)
-        launch_debug_keys(ws->tkeys[ws->tile_sel], ws->tvals[ws->tile_sel], ws->rec_c, ws->counters, ws->inst_cap,
+        launch_debug_keys(ws->tkeys[ws->tile_sel], ws->tvals[ws->tile_sel], ws->meta, ws->counters, ws->inst_cap,
                           ws->vp_log2, keys, values, st);
     if (ranges)
         if (int r = check_hip(hipMemcpyAsync(ranges, ws->ranges, size_t(ws->tiles_total) * sizeof(uint2),

@@ -29,9 +29,17 @@ struct Counters {
     uint32_t pad[3];
 };
 
-// Splat record (one per (gaussian, view) slot; slot = g * vp + v):
+// Blend record, one per (gaussian, view) slot (slot = g * vp + v), 64 bytes so that every
+// gather of it by the tile kernels is one aligned 64-byte HBM sector:
 //   a = (mean.x, mean.y, conic.x, conic.y)   b = (conic.z, opacity, r, g)
-//   c = (bits(b), depth_key, tiles, flags)   flags: bits 0-2 SH clamp, bit 3 fp64 conic
+//   c = (b, ext.x, ext.y, 0)                 d = 0
+// ext = half extents of the box around the alpha >= 1/255 ellipse (grown by 1% + 1 px), used
+// to cull 8x8 pixel patches; +inf = no culling, -inf = reaches no pixel.
+struct SplatRec {
+    float4 a, b, c, d;
+};
+// Slot metadata (16 bytes, read in slot order): (0, depth_key, tiles, flags),
+// flags: bits 0-2 SH clamp, bit 3 fp64 conic.
 // Splat gradient (12 floats): dmean.xy, dconic.xyz, dopacity, dcolor.rgb, refine, 2 spare.
 constexpr int kGradStride = 12;
 
@@ -80,6 +88,15 @@ __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
 }
 
 __device__ __forceinline__ void atomic_add_f32(float* p, float v) { unsafeAtomicAdd(p, v); }
+
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2): map block b of
+// n to a work item so that every XCD gets one contiguous run of items (neighbouring tiles share
+// splats, so their records stay in one L2).
+__device__ __forceinline__ int xcd_contiguous(int b, int n) {
+    const int q = n >> 3, r = n & 7;
+    const int x = b & 7, k = b >> 3;
+    return x * q + (x < r ? x : r) + k;
+}
 
 __device__ __forceinline__ const DevView& view_of_tile(const DevView* views, int n_views, int tile,
                                                        int& v_out) {

@@ -100,7 +100,7 @@ __global__ __launch_bounds__(256) void k_gaussian_bwd(GaussArgs a) {
             } else {
                 float d[3], raw[3];
                 view_dir(mean, cam.position, d, raw);
-                sh_backward(DEG, raw, d, my_sh, a.rec_c[slot].w & 7u, dcol, gsh, sg.mean);
+                sh_backward(DEG, raw, d, my_sh, a.meta[slot].w & 7u, dcol, gsh, sg.mean);
             }
             refine += g2.y;
         }

@@ -25,9 +25,62 @@ __global__ __launch_bounds__(256) void k_clear(Counters* counters, uint2* ranges
     for (uint32_t i = i0; i < n_tiles; i += gridDim.x * 256u) ranges[i] = make_uint2(0u, 0u);
 }
 
+// Half extents of the box around the alpha >= 1/255 ellipse {q(d) <= 2 log(255 op)} of the
+// conic (a, b, c), grown by 1% + 1 px: +inf when the conic is degenerate or too anisotropic for
+// a safe box (no culling), -inf when the opacity is below 1/255 (reaches no pixel).
+__device__ __forceinline__ void patch_extent(float a, float b, float c, float op, float& ex, float& ey) {
+    const float inf = __builtin_huge_valf();
+    if (!(op >= K<float>::alpha_floor())) {
+        ex = ey = -inf;
+        return;
+    }
+    const float det = a * c - b * b;
+    const float thr = 2.0f * plogf(op * 255.0f);
+    const float tr = a + c;
+    ex = ey = inf;
+    if (!(det > 0.0f) || !(a > 0.0f) || !(c > 0.0f) || !(thr >= 0.0f) || !(tr * tr < 1.6e5f * det)) return;
+    const float x = sqrtf(thr * c / det) * 1.01f + 1.0f;
+    const float y = sqrtf(thr * a / det) * 1.01f + 1.0f;
+    if (!(x < 1.0e6f) || !(y < 1.0e6f)) return;
+    ex = x;
+    ey = y;
+}
+
 template <int LP> __device__ __forceinline__ float load_lp(const float* f, const uint16_t* lp, int64_t i) {
     if constexpr (LP == PHT_LP_NONE) return f[i];
     else return lp_to_f32(lp[i], LP);
+}
+
+// The first NC coefficients of one SH row (row_elems stored elements). When the array is 16-byte
+// aligned and a row is a whole number of 16-byte chunks, the row is read as uint4 chunks (NC
+// rounded up to a chunk never leaves the row); otherwise element by element.
+template <int LP, int NC>
+__device__ __forceinline__ void load_sh_row(const float* f, const uint16_t* lp, int64_t row, int row_elems,
+                                            float (&coef)[NC]) {
+    constexpr int ES = LP == PHT_LP_NONE ? 4 : 2;   // bytes per stored element
+    constexpr int PER = 16 / ES;                    // elements per 16-byte chunk
+    constexpr int CHUNKS = (NC + PER - 1) / PER;
+    const uintptr_t base = LP == PHT_LP_NONE ? reinterpret_cast<uintptr_t>(f) : reinterpret_cast<uintptr_t>(lp);
+    if ((row_elems * ES) % 16 == 0 && (base & 15u) == 0) {
+        const uint4* src = LP == PHT_LP_NONE ? reinterpret_cast<const uint4*>(f + row)
+                                             : reinterpret_cast<const uint4*>(lp + row);
+#pragma unroll
+        for (int c = 0; c < CHUNKS; ++c) {
+            const uint4 q = src[c];
+            const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+            for (int e = 0; e < PER; ++e) {
+                const int k = c * PER + e;
+                if (k < NC) {
+                    if constexpr (LP == PHT_LP_NONE) coef[k] = u2f(w[e]);
+                    else coef[k] = lp_to_f32(uint16_t(w[e >> 1] >> ((e & 1) * 16)), LP);
+                }
+            }
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < NC; ++k) coef[k] = load_lp<LP>(f, lp, row + k);
+    }
 }
 
 template <int DEG, int LP>
@@ -65,15 +118,19 @@ __global__ __launch_bounds__(256) void k_preprocess(PreArgs a, int64_t n_slots) 
                     constexpr int NB = (DEG + 1) * (DEG + 1);
                     float coef[NB * 3];
                     const int64_t row = gi * int64_t(a.s.sh_bases) * 3;
-#pragma unroll
-                    for (int k = 0; k < NB * 3; ++k) coef[k] = load_lp<LP>(a.g.sh, a.g.sh_lp, row + k);
+                    load_sh_row<LP, NB * 3>(a.g.sh, a.g.sh_lp, row, a.s.sh_bases * 3, coef);
                     float d[3], raw[3];
                     view_dir(mean, dv.vi.cam.position, d, raw);
                     sh_eval(DEG, d, coef, rgb, clamp);
                 }
-                a.rec_a[t] = make_float4(p.mean2d[0], p.mean2d[1], p.conic[0], p.conic[1]);
-                a.rec_b[t] = make_float4(p.conic[2], p.opacity, rgb[0], rgb[1]);
-                a.rec_c[t] = make_uint4(f2u(rgb[2]), f2u(p.depth), touched, clamp | (fp64 ? 8u : 0u));
+                float ex, ey;
+                patch_extent(p.conic[0], p.conic[1], p.conic[2], p.opacity, ex, ey);
+                SplatRec& r = a.rec[t];
+                r.a = make_float4(p.mean2d[0], p.mean2d[1], p.conic[0], p.conic[1]);
+                r.b = make_float4(p.conic[2], p.opacity, rgb[0], rgb[1]);
+                r.c = make_float4(rgb[2], ex, ey, 0.0f);
+                r.d = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                a.meta[t] = make_uint4(0u, f2u(p.depth), touched, clamp | (fp64 ? 8u : 0u));
                 float4* sg = reinterpret_cast<float4*>(a.sgrad + t * kGradStride);
                 sg[0] = make_float4(0.f, 0.f, 0.f, 0.f);
                 sg[1] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -97,32 +154,114 @@ __global__ __launch_bounds__(256) void k_preprocess(PreArgs a, int64_t n_slots) 
 
 // Emits (global tile, slot) pairs for the depth-sorted visible splats at their scanned offsets
 // (emit_instances, render_forward.cu:167-181; tile keys carry the view's tile_base).
-__global__ __launch_bounds__(256) void k_emit(const uint32_t* sorted_slots, const uint32_t* offsets,
-                                              const Counters* counters, uint32_t slot_cap, const float4* rec_a,
-                                              const float4* rec_b, const uint4* rec_c, const DevView* views,
-                                              int vp_log2, int ts, uint32_t inst_cap, uint32_t* keys_out,
-                                              uint32_t* vals_out) {
+//
+// Each wave takes 64 consecutive sorted splats, whose outputs form one contiguous run. Every lane
+// walks its splat's columns (AccuTile) and records each column's tile interval in LDS with its
+// position in the run; the wave then writes the run in order, 64 consecutive outputs at a time,
+// each lane locating its output's column by binary search. Writes are coalesced and the work per
+// lane no longer follows one splat's tile count. A group whose columns do not fit the LDS table,
+// or that would overflow the instance capacity, falls back to per-lane emission.
+constexpr int kEmitCols = 512; // column records per wave
+
+__global__ __launch_bounds__(256) void k_emit(const uint32_t* sorted_slots, const uint32_t* sorted_tiles,
+                                              const uint32_t* offsets, const Counters* counters, uint32_t slot_cap,
+                                              const SplatRec* rec, const DevView* views, int vp_log2, int ts,
+                                              uint32_t inst_cap, uint32_t* keys_out, uint32_t* vals_out) {
+    __shared__ uint2 s_col[4][kEmitCols]; // (run position of the column, lane << 26 | u << 13 | v_min)
+    __shared__ uint32_t s_slot[4][64];
+    __shared__ uint32_t s_tb[4][64];
+    __shared__ uint32_t s_gx[4][64];      // grid_x | iterate_x << 31
     const uint32_t nv = counters->visible;
     const uint32_t n = nv < slot_cap ? nv : slot_cap;
     const uint32_t vmask = (1u << vp_log2) - 1u;
-    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
-        const uint32_t slot = sorted_slots[i];
-        const uint32_t off = offsets[i];
-        const uint4 c = rec_c[slot];
-        if (uint64_t(off) + c.z > inst_cap) continue;
-        const float4 A = rec_a[slot], B = rec_b[slot];
-        const ViewInfo& vi = views[slot & vmask].vi;
-        const float conic[3] = {A.z, A.w, B.x};
-        uint32_t k = 0;
-        const int gx = vi.grid_x;
-        const uint32_t tb = uint32_t(vi.tile_base);
-        enumerate_tiles(A.x, A.y, conic, B.y, vi.grid_x, vi.grid_y, ts, [&](int tx, int ty) {
-            if (k < c.z) {
-                keys_out[off + k] = tb + uint32_t(ty * gx + tx);
-                vals_out[off + k] = slot;
+    const int lane = lane_id(), w = threadIdx.x >> 6;
+    const uint32_t groups = (n + 63u) >> 6;
+    for (uint32_t grp = blockIdx.x * 4u + uint32_t(w); grp < groups; grp += gridDim.x * 4u) {
+        const uint32_t i = grp * 64u + uint32_t(lane);
+        const bool valid = i < n;
+        uint32_t slot = 0, off = 0, count = 0;
+        if (valid) {
+            slot = sorted_slots[i];
+            off = offsets[i];
+            count = sorted_tiles[i];
+        }
+        const bool over = valid && uint64_t(off) + count > inst_cap;
+        TileWalk<float> tw;
+        tw.ok = false;
+        uint32_t tb = 0, gx = 0;
+        if (valid && !over) {
+            const float4 A = rec[slot].a, B = rec[slot].b;
+            const ViewInfo& vi = views[slot & vmask].vi;
+            const float conic[3] = {A.z, A.w, B.x};
+            tw = tile_walk_setup(A.x, A.y, conic, B.y, vi.grid_x, vi.grid_y, ts);
+            tb = uint32_t(vi.tile_base);
+            gx = uint32_t(vi.grid_x);
+        }
+        const uint32_t ncols = tw.ok ? uint32_t(tw.u_max - tw.u_min) : 0u;
+        const uint32_t col_incl = wave_inclusive_scan(ncols);
+        const uint32_t total_cols = uint32_t(__builtin_amdgcn_readlane(int(col_incl), 63));
+        const uint32_t cnt_incl = wave_inclusive_scan(valid && !over ? count : 0u);
+        const uint32_t total = uint32_t(__builtin_amdgcn_readlane(int(cnt_incl), 63));
+        const uint32_t run_base = uint32_t(__builtin_amdgcn_readfirstlane(int(off)));
+        if (__ballot(over) != 0 || total_cols > uint32_t(kEmitCols)) {
+            if (tw.ok) {
+                uint32_t k = 0;
+                for (int tu = tw.u_min; tu < tw.u_max; ++tu) {
+                    int v0, v1;
+                    tile_walk_column(tw, tu, v0, v1);
+                    for (int tv = v0; tv < v1; ++tv) {
+                        if (k < count) {
+                            const int tx = tw.iterate_x ? tu : tv, ty = tw.iterate_x ? tv : tu;
+                            keys_out[off + k] = tb + uint32_t(ty) * gx + uint32_t(tx);
+                            vals_out[off + k] = slot;
+                        }
+                        ++k;
+                    }
+                }
             }
-            ++k;
-        });
+            continue;
+        }
+        // column table: this lane's columns start at col_incl - ncols, its outputs at cnt_incl - count
+        {
+            uint32_t r = col_incl - ncols;
+            const uint32_t pos0 = cnt_incl - count;
+            uint32_t done = 0;
+            for (int tu = tw.u_min; tw.ok && tu < tw.u_max; ++tu) {
+                int v0, v1;
+                tile_walk_column(tw, tu, v0, v1);
+                uint32_t len = v1 > v0 ? uint32_t(v1 - v0) : 0u;
+                if (done + len > count) len = count - done;
+                s_col[w][r++] = make_uint2(pos0 + done,
+                                           (uint32_t(lane) << 26) | (uint32_t(tu) << 13) | uint32_t(v0 > 0 ? v0 : 0));
+                done += len;
+            }
+            s_slot[w][lane] = slot;
+            s_tb[w][lane] = tb;
+            s_gx[w][lane] = gx | (tw.iterate_x ? 0x80000000u : 0u);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (uint32_t o = uint32_t(lane); o < total; o += 64u) {
+            int lo = 0, hi = int(total_cols) - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_col[w][mid].x <= o) lo = mid;
+                else hi = mid - 1;
+            }
+            const uint2 c = s_col[w][lo];
+            const uint32_t L = c.y >> 26;
+            const uint32_t u = (c.y >> 13) & 0x1fffu;
+            const uint32_t v = (c.y & 0x1fffu) + (o - c.x);
+            const uint32_t g = s_gx[w][L];
+            const bool it = (g >> 31) != 0;
+            const uint32_t tx = it ? u : v, ty = it ? v : u;
+            keys_out[run_base + o] = s_tb[w][L] + ty * (g & 0x7fffffffu) + tx;
+            vals_out[run_base + o] = s_slot[w][L];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
@@ -141,13 +280,13 @@ __global__ __launch_bounds__(256) void k_ranges(const uint32_t* keys, const Coun
 }
 
 __global__ __launch_bounds__(256) void k_debug_keys(const uint32_t* tile_keys, const uint32_t* slots,
-                                                    const uint4* rec_c, const Counters* counters, uint32_t inst_cap,
+                                                    const uint4* meta, const Counters* counters, uint32_t inst_cap,
                                                     int vp_log2, uint64_t* keys_out, uint32_t* vals_out) {
     const uint32_t ni = counters->instances;
     const uint32_t n = ni < inst_cap ? ni : inst_cap;
     for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
         const uint32_t slot = slots[i];
-        if (keys_out) keys_out[i] = (uint64_t(tile_keys[i]) << 32) | rec_c[slot].y;
+        if (keys_out) keys_out[i] = (uint64_t(tile_keys[i]) << 32) | meta[slot].y;
         if (vals_out) vals_out[i] = slot >> vp_log2;
     }
 }
@@ -206,12 +345,12 @@ void launch_preprocess(const PreArgs& a, int64_t n_slots, hipStream_t st) {
     else pre_deg<PHT_LP_NONE>(a, n_slots, st);
 }
 
-void launch_emit_instances(const uint32_t* sorted_slots, const uint32_t* offsets, const Counters* counters,
-                           uint32_t slot_cap, const float4* rec_a, const float4* rec_b, const uint4* rec_c,
-                           const DevView* views, int vp_log2, int tile_size, uint32_t inst_cap,
-                           uint32_t* keys_out, uint32_t* vals_out, hipStream_t st) {
-    hipLaunchKernelGGL(k_emit, dim3(grid_stride_blocks(slot_cap)), dim3(256), 0, st, sorted_slots, offsets, counters,
-                       slot_cap, rec_a, rec_b, rec_c, views, vp_log2, tile_size, inst_cap, keys_out, vals_out);
+void launch_emit_instances(const uint32_t* sorted_slots, const uint32_t* sorted_tiles, const uint32_t* offsets,
+                           const Counters* counters, uint32_t slot_cap, const SplatRec* rec, const DevView* views,
+                           int vp_log2, int tile_size, uint32_t inst_cap, uint32_t* keys_out, uint32_t* vals_out,
+                           hipStream_t st) {
+    hipLaunchKernelGGL(k_emit, dim3(grid_stride_blocks(slot_cap)), dim3(256), 0, st, sorted_slots, sorted_tiles,
+                       offsets, counters, slot_cap, rec, views, vp_log2, tile_size, inst_cap, keys_out, vals_out);
 }
 
 void launch_ranges(const uint32_t* keys, const Counters* counters, uint32_t inst_cap, uint32_t n_tiles,
@@ -220,10 +359,10 @@ void launch_ranges(const uint32_t* keys, const Counters* counters, uint32_t inst
                        n_tiles, ranges);
 }
 
-void launch_debug_keys(const uint32_t* tile_keys, const uint32_t* slots, const uint4* rec_c,
+void launch_debug_keys(const uint32_t* tile_keys, const uint32_t* slots, const uint4* meta,
                        const Counters* counters, uint32_t inst_cap, int vp_log2, uint64_t* keys_out,
                        uint32_t* vals_out, hipStream_t st) {
-    hipLaunchKernelGGL(k_debug_keys, dim3(grid_stride_blocks(inst_cap)), dim3(256), 0, st, tile_keys, slots, rec_c,
+    hipLaunchKernelGGL(k_debug_keys, dim3(grid_stride_blocks(inst_cap)), dim3(256), 0, st, tile_keys, slots, meta,
                        counters, inst_cap, vp_log2, keys_out, vals_out);
 }
 

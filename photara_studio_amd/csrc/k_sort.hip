@@ -54,13 +54,13 @@ struct SrcTiles { // tile counts, already permuted into depth order by the depth
     __device__ uint32_t operator()(uint32_t i) const { return counts[i]; }
 };
 struct SinkCompact { // visible slot -> (depth key, slot, tile count) at its compact position
-    const uint4* rec_c;
+    const uint4* meta;
     uint32_t* keys;
     uint32_t* vals;
     uint32_t* tiles;
     __device__ void operator()(uint32_t i, uint32_t excl, uint32_t v) const {
         if (v) {
-            const uint4 c = rec_c[i];
+            const uint4 c = meta[i];
             keys[excl] = c.y;
             vals[excl] = i;
             tiles[excl] = c.z;
@@ -290,7 +290,7 @@ static int blocks_for(uint32_t capacity) {
 
 int scan_blocks_for(uint32_t capacity) { return blocks_for(capacity); }
 
-void launch_compact_visible(const uint32_t* flags, const uint4* rec_c, uint32_t n_slots, uint32_t* keys,
+void launch_compact_visible(const uint32_t* flags, const uint4* meta, uint32_t n_slots, uint32_t* keys,
                             uint32_t* vals, uint32_t* tiles, uint32_t* block_scratch, uint32_t* visible_out,
                             hipStream_t st) {
     const int nb = blocks_for(n_slots);
@@ -299,7 +299,7 @@ void launch_compact_visible(const uint32_t* flags, const uint4* rec_c, uint32_t 
                        n_slots, nb, block_scratch);
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st, block_scratch, nb, visible_out, nullptr,
                        0xffffffffu);
-    SinkCompact sink{rec_c, keys, vals, tiles};
+    SinkCompact sink{meta, keys, vals, tiles};
     hipLaunchKernelGGL((k_scan_write<SrcFlags, SinkCompact>), dim3(nb), dim3(kScanThreads), 0, st, src, sink,
                        nullptr, n_slots, n_slots, nb, block_scratch);
 }

@@ -12,9 +12,8 @@ struct PreArgs {
     const DevView* views;
     int n_views;
     int vp_log2;
-    float4* rec_a;
-    float4* rec_b;
-    uint4* rec_c;
+    SplatRec* rec;
+    uint4* meta;
     uint32_t* vis_flag;
     float* sgrad;
     uint32_t* view_mask;
@@ -24,9 +23,7 @@ struct PreArgs {
 struct BlendArgs {
     const uint2* ranges;
     const uint32_t* inst_vals;
-    const float4* rec_a;
-    const float4* rec_b;
-    const uint4* rec_c;
+    const SplatRec* rec;
     const DevView* views;
     int n_views;
     int vp_log2;
@@ -44,7 +41,7 @@ struct GaussArgs {
     const DevView* views;
     int n_views;
     int vp_log2;
-    const uint4* rec_c;
+    const uint4* meta;
     const float* sgrad;
     const uint32_t* view_mask;
     const Counters* counters;
@@ -55,7 +52,7 @@ struct GaussArgs {
 // k_sort.hip
 int scan_blocks_for(uint32_t capacity);
 // Compacts the visible slots (slot order) into (depth key, slot, tile count) lists.
-void launch_compact_visible(const uint32_t* flags, const uint4* rec_c, uint32_t n_slots, uint32_t* keys,
+void launch_compact_visible(const uint32_t* flags, const uint4* meta, uint32_t n_slots, uint32_t* keys,
                             uint32_t* vals, uint32_t* tiles, uint32_t* block_scratch, uint32_t* visible_out,
                             hipStream_t st);
 // Exclusive scan of the depth-ordered tile counts into instance offsets; stores the instance
@@ -73,13 +70,13 @@ int launch_radix_sort(uint32_t* keys[2], uint32_t* vals[2], uint32_t* aux[2], co
 void launch_upload_views(DevView* dst, const DevView* host_views, int n_views, hipStream_t st);
 void launch_clear(Counters* counters, uint2* ranges, uint32_t n_tiles, hipStream_t st);
 void launch_preprocess(const PreArgs& a, int64_t n_slots, hipStream_t st);
-void launch_emit_instances(const uint32_t* sorted_slots, const uint32_t* offsets, const Counters* counters,
-                           uint32_t slot_cap, const float4* rec_a, const float4* rec_b, const uint4* rec_c,
+void launch_emit_instances(const uint32_t* sorted_slots, const uint32_t* sorted_tiles, const uint32_t* offsets,
+                           const Counters* counters, uint32_t slot_cap, const SplatRec* rec,
                            const DevView* views, int vp_log2, int tile_size, uint32_t inst_cap,
                            uint32_t* keys_out, uint32_t* vals_out, hipStream_t st);
 void launch_ranges(const uint32_t* keys, const Counters* counters, uint32_t inst_cap, uint32_t n_tiles,
                    uint2* ranges, hipStream_t st);
-void launch_debug_keys(const uint32_t* tile_keys, const uint32_t* slots, const uint4* rec_c,
+void launch_debug_keys(const uint32_t* tile_keys, const uint32_t* slots, const uint4* meta,
                        const Counters* counters, uint32_t inst_cap, int vp_log2, uint64_t* keys_out,
                        uint32_t* vals_out, hipStream_t st);
 

@@ -79,6 +79,25 @@ PHT_HD float pexpf(float x) {
     return (p * u2f(uint32_t(k + 127 + 64) << 23)) * 5.42101086242752217e-20f; // 2^-64
 }
 
+// pexpf restricted to x in [-104, 88.7] with 2^k normal (the blend's powers lie in
+// [kSkipPower, 0]): the same operations as pexpf, so identical results there, without the
+// range branches. NaN propagates.
+PHT_HD float pexpf_inrange(float x) {
+    const float kf = rintf(x * 1.44269502162933350f);
+    float r = fma_(kf, -0.693147182464599609375f, x);
+    r = fma_(kf, 1.9046542121259336e-09f, r);
+    float p = 1.98412701138295233e-04f;
+    p = fma_(p, r, 1.38888892251998186e-03f);
+    p = fma_(p, r, 8.33333376795053482e-03f);
+    p = fma_(p, r, 4.16666679084300995e-02f);
+    p = fma_(p, r, 1.66666671633720398e-01f);
+    p = fma_(p, r, 0.5f);
+    p = fma_(p, r, 1.0f);
+    p = fma_(p, r, 1.0f);
+    const int k = x == x ? int(kf) : 0;
+    return p * u2f(uint32_t(k + 127) << 23);
+}
+
 // log(x): x = 2^e * m with m in [sqrt(1/2), sqrt(2)), log(m) = 2 atanh(s), s = (m-1)/(m+1),
 // atanh series to s^9 (|s| <= 0.1716, truncation < 1e-9 relative). ~1-2 ulp.
 PHT_HD float plogf(float x) {
@@ -105,6 +124,9 @@ PHT_HD float plogf(float x) {
 
 PHT_HD float exp_(float x) { return pexpf(x); }
 PHT_HD double exp_(double x) { return exp(x); }
+// exp for arguments already known to lie in [kSkipPower, 0] (or NaN)
+PHT_HD float exp_blend_(float x) { return pexpf_inrange(x); }
+PHT_HD double exp_blend_(double x) { return exp(x); }
 PHT_HD float log_(float x) { return plogf(x); }
 PHT_HD double log_(double x) { return log(x); }
 
@@ -359,55 +381,91 @@ PHT_HD bool project_pinhole(const T mean[3], const Activated<T>& act, const pht_
 // ---------------------------------------------------------------- tile enumeration
 // Opacity-aware AccuTile footprint (third_party/splat_drender/src/device/tiles.cuh:18-179,
 // non-wrapping path): the ellipse where alpha falls to 1/255, walked along the shorter screen
-// axis with an exact per-slice interval. emit(tx, ty) is called in walk order.
-template <class T, class Emit>
-PHT_HD uint32_t enumerate_tiles(T mx, T my, const T conic[3], T op, int grid_x, int grid_y, int ts,
-                                Emit&& emit) {
+// axis with an exact per-slice interval. Split into the walk's set-up (TileWalk), one column's
+// tile interval (tile_walk_column) and the full walk (enumerate_tiles, emit(tx, ty) in walk
+// order), all with the same arithmetic.
+template <class T> struct TileWalk {
+    bool ok;
+    bool iterate_x;   // columns run along x (u = tile x, v = tile y); else u = y, v = x
+    int gu, gv, u_min, u_max;
+    T cu, cv, au, b, det, threshold, eu, ev, arg_v_min, arg_v_max, tsf;
+    T av;
+};
+
+template <class T>
+PHT_HD TileWalk<T> tile_walk_setup(T mx, T my, const T conic[3], T op, int grid_x, int grid_y, int ts) {
+    TileWalk<T> w;
+    w.ok = false;
     const T a = conic[0], b = conic[1], c = conic[2];
     const T det = a * c - b * b;
-    if (!(a > T(0) && c > T(0)) || !(op >= K<T>::alpha_floor()) || !(det > T(0))) return 0;
+    if (!(a > T(0) && c > T(0)) || !(op >= K<T>::alpha_floor()) || !(det > T(0))) return w;
     const T threshold = T(2) * log_(op * T(255));
-    if (!(threshold >= T(0))) return 0;
+    if (!(threshold >= T(0))) return w;
     const T ex = sqrt_(threshold * c / det);
     const T ey = sqrt_(threshold * a / det);
     const T tsf = T(ts);
     const bool iterate_x = (T(2) * ex) / tsf <= (T(2) * ey) / tsf;
-    const T cu = iterate_x ? mx : my;
-    const T cv = iterate_x ? my : mx;
-    const T au = iterate_x ? a : c;
-    const T av = iterate_x ? c : a;
-    const int gu = iterate_x ? grid_x : grid_y;
-    const int gv = iterate_x ? grid_y : grid_x;
-    const T eu = sqrt_(threshold * av / det);
-    const T ev = sqrt_(threshold * au / det);
-    int u_min = int(floor_((cu - eu) / tsf));
-    u_min = u_min < gu ? u_min : gu;
+    w.iterate_x = iterate_x;
+    w.cu = iterate_x ? mx : my;
+    w.cv = iterate_x ? my : mx;
+    w.au = iterate_x ? a : c;
+    w.av = iterate_x ? c : a;
+    w.gu = iterate_x ? grid_x : grid_y;
+    w.gv = iterate_x ? grid_y : grid_x;
+    w.b = b;
+    w.det = det;
+    w.threshold = threshold;
+    w.tsf = tsf;
+    w.eu = sqrt_(threshold * w.av / det);
+    w.ev = sqrt_(threshold * w.au / det);
+    int u_min = int(floor_((w.cu - w.eu) / tsf));
+    u_min = u_min < w.gu ? u_min : w.gu;
     u_min = u_min > 0 ? u_min : 0;
-    int u_max = int(floor_((cu + eu) / tsf)) + 1;
-    u_max = u_max < gu ? u_max : gu;
-    if (u_min >= u_max) return 0;
-    const T arg_v_min = cu + b * ev / au;
-    const T arg_v_max = cu - b * ev / au;
+    int u_max = int(floor_((w.cu + w.eu) / tsf)) + 1;
+    u_max = u_max < w.gu ? u_max : w.gu;
+    w.u_min = u_min;
+    w.u_max = u_max;
+    if (u_min >= u_max) return w;
+    w.arg_v_min = w.cu + b * w.ev / w.au;
+    w.arg_v_max = w.cu - b * w.ev / w.au;
+    w.ok = true;
+    return w;
+}
+
+// Tile interval [v_min, v_max) of column tu (may be empty).
+template <class T> PHT_HD void tile_walk_column(const TileWalk<T>& w, int tu, int& v_min, int& v_max) {
+    const T cu = w.cu, cv = w.cv, av = w.av, b = w.b, det = w.det, tsf = w.tsf;
+    const T s0 = max_(cu - w.eu, T(tu) * tsf);
+    const T s1 = min_(cu + w.eu, T(tu + 1) * tsf);
+    const T d0 = s0 - cu, d1 = s1 - cu;
+    const T r0 = sqrt_(max_(T(0), av * w.threshold - det * d0 * d0));
+    const T r1 = sqrt_(max_(T(0), av * w.threshold - det * d1 * d1));
+    const T lo0 = cv + (-b * d0 - r0) / av;
+    const T lo1 = cv + (-b * d1 - r1) / av;
+    const T hi0 = cv + (-b * d0 + r0) / av;
+    const T hi1 = cv + (-b * d1 + r1) / av;
+    const T sv_min = (w.arg_v_min >= s0 && w.arg_v_min <= s1) ? cv - w.ev : min_(lo0, lo1);
+    const T sv_max = (w.arg_v_max >= s0 && w.arg_v_max <= s1) ? cv + w.ev : max_(hi0, hi1);
+    int lo = int(floor_(sv_min / tsf));
+    lo = lo < w.gv ? lo : w.gv;
+    lo = lo > 0 ? lo : 0;
+    int hi = int(floor_(sv_max / tsf)) + 1;
+    hi = hi < w.gv ? hi : w.gv;
+    v_min = lo;
+    v_max = hi;
+}
+
+template <class T, class Emit>
+PHT_HD uint32_t enumerate_tiles(T mx, T my, const T conic[3], T op, int grid_x, int grid_y, int ts,
+                                Emit&& emit) {
+    const TileWalk<T> w = tile_walk_setup(mx, my, conic, op, grid_x, grid_y, ts);
+    if (!w.ok) return 0;
     uint32_t count = 0;
-    for (int tu = u_min; tu < u_max; ++tu) {
-        const T s0 = max_(cu - eu, T(tu) * tsf);
-        const T s1 = min_(cu + eu, T(tu + 1) * tsf);
-        const T d0 = s0 - cu, d1 = s1 - cu;
-        const T r0 = sqrt_(max_(T(0), av * threshold - det * d0 * d0));
-        const T r1 = sqrt_(max_(T(0), av * threshold - det * d1 * d1));
-        const T lo0 = cv + (-b * d0 - r0) / av;
-        const T lo1 = cv + (-b * d1 - r1) / av;
-        const T hi0 = cv + (-b * d0 + r0) / av;
-        const T hi1 = cv + (-b * d1 + r1) / av;
-        const T sv_min = (arg_v_min >= s0 && arg_v_min <= s1) ? cv - ev : min_(lo0, lo1);
-        const T sv_max = (arg_v_max >= s0 && arg_v_max <= s1) ? cv + ev : max_(hi0, hi1);
-        int v_min = int(floor_(sv_min / tsf));
-        v_min = v_min < gv ? v_min : gv;
-        v_min = v_min > 0 ? v_min : 0;
-        int v_max = int(floor_(sv_max / tsf)) + 1;
-        v_max = v_max < gv ? v_max : gv;
+    for (int tu = w.u_min; tu < w.u_max; ++tu) {
+        int v_min, v_max;
+        tile_walk_column(w, tu, v_min, v_max);
         for (int tv = v_min; tv < v_max; ++tv) {
-            emit(iterate_x ? tu : tv, iterate_x ? tv : tu);
+            emit(w.iterate_x ? tu : tv, w.iterate_x ? tv : tu);
             ++count;
         }
     }
@@ -566,7 +624,7 @@ PHT_HD bool pair_alpha(T ca, T cb, T cc, T op, T dx, T dy, T alpha_clip, T& alph
     const T power = gaussian_power(ca, cb, cc, dx, dy);
     if (power > T(0)) return false;
     if (power < T(kSkipPower)) return false;
-    G = exp_(power);
+    G = exp_blend_(power);
     alpha = min_(alpha_clip, op * G);
     return !(alpha < K<T>::alpha_floor());
 }

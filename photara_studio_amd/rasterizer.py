@@ -85,7 +85,7 @@ class RasterizeOptions:
     colors_precomp: Optional[torch.Tensor] = None  # [N,3]
     # gfx950 launch shape: tile edge (16 = reference, or 8) and LDS batch depth.
     tile_size: int = 16
-    batch: int = 256
+    batch: int = 64
     fp64_conic: bool = True
 
 
@@ -509,7 +509,7 @@ def debug_lists(rendered: RenderResult) -> dict:
     dev = ctx.ws.buf.device
     counts = ctx.ws.counts()
     ni = int(counts.instances)
-    keys =torch.empty((max(ni, 1),), dtype=torch.int64, device=dev)
+    keys = torch.empty((max(ni, 1),), dtype=torch.int64, device=dev)
     vals = torch.empty((max(ni, 1),), dtype=torch.int32, device=dev)
     ranges = torch.empty((2 * ctx.tiles,), dtype=torch.int32, device=dev)
     n_contrib = torch.empty((ctx.pixels,), dtype=torch.int32, device=dev)

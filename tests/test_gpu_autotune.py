@@ -1,0 +1,44 @@
+"""The launch-shape autotuner: every candidate runs, the chosen shape renders the same image as
+the reference shape (16x16 tiles), and a cached result is reused without timing."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import small_scene, training_options
+
+pytestmark = pytest.mark.gpu
+
+
+def test_autotune_sweep_and_cache(cuda, tmp_path):
+    from photara_studio_amd.autotune import CANDIDATES, autotune
+    from photara_studio_amd.rasterizer import Rasterizer
+    from photara_studio_amd.scenes import model_from, ring_cameras
+    from photara_studio_amd.train import LearningRates
+
+    arrays = small_scene(20000, 1.0, 5)
+    cams = ring_cameras(2, 4.0, [(320, 240)])
+    opts = training_options()
+    model = model_from(arrays, cuda)
+    r = Rasterizer()
+    targets = [res.color.clone() for res in r.forward_views(model_from(small_scene(20000, 1.0, 6), cuda), cams, opts)]
+    before = model.means.clone()
+    print("[autotune] sweep", flush=True)
+    best, rows = autotune(model, cams, targets, opts, LearningRates(), steps=1, warmup=1, cache=tmp_path / "t.json")
+    expected = [(t, b) for t, b in CANDIDATES if not (t == 8 and b > 128)]
+    assert [(x.tile_size, x.batch) for x in rows] == expected
+    assert all(x.ms_per_step > 0 for x in rows)
+    assert (best.tile_size, best.batch) == min(((x.tile_size, x.batch, x.ms_per_step) for x in rows),
+                                               key=lambda t: t[2])[:2]
+    assert torch.equal(model.means, before), "tuning must not train the caller's model"
+    # the tuned shape renders the reference shape's image bit for bit
+    a = r.forward_views(model, cams, opts)
+    b = r.forward_views(model, cams, best)
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x.color.cpu().numpy(), y.color.cpu().numpy())
+        np.testing.assert_array_equal(x.alpha.cpu().numpy(), y.alpha.cpu().numpy())
+    # cached: same answer, no timing
+    best2, rows2 = autotune(model, cams, targets, opts, LearningRates(), steps=1, warmup=1, cache=tmp_path / "t.json")
+    assert (best2.tile_size, best2.batch) == (best.tile_size, best.batch)
+    assert [x.ms_per_step for x in rows2] == [x.ms_per_step for x in rows]

@@ -31,6 +31,17 @@ def test_pexpf_error_bound(oracle):
     assert np.abs(ys.astype(np.float64) - want).max() <= 2 * 1.4e-45
 
 
+def test_pexpf_inrange_equals_pexpf_on_blend_powers(oracle):
+    """The blend's branch-free exp is bit-identical to pexpf on [-5.6, 0] (the powers that
+    reach it): every 128th float32 bit pattern of the interval (8.5M values) plus its ends."""
+    lo_bits = np.float32(-5.6).view(np.uint32)  # negative floats: larger bits = more negative
+    neg = np.arange(0x80000000, lo_bits + 1, 1 << 7, dtype=np.uint64).astype(np.uint32).view(np.float32)
+    x = np.concatenate([neg, np.array([0.0, -0.0, np.float32(-5.6)], np.float32)])
+    a, b = oracle.pexpf(x), oracle.pexpf_inrange(x)
+    np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert np.isnan(oracle.pexpf_inrange(np.array([np.nan], np.float32))[0])
+
+
 def test_plogf_error_bound(oracle):
     rng = np.random.default_rng(1)
     x = np.concatenate([np.exp(rng.uniform(-87, 88, 400000)).astype(np.float32),
