@@ -51,6 +51,18 @@ __device__ __forceinline__ void wave_sum10_to_last(float (&r)[10]) {
     dpp_step10<0x143, 0xc>(r);
 }
 
+// pair_alpha (splat_math.h) without early returns, so a wave evaluates it with no exec-mask
+// branches: identical G, alpha and decision for every pair pair_alpha accepts (the exp argument
+// is the same power), including a NaN power; other lanes get a placeholder G.
+__device__ __forceinline__ bool pair_alpha_flat(float ca, float cb, float cc, float op, float dx, float dy,
+                                                float clip, float& alpha, float& G) {
+    const float power = gaussian_power(ca, cb, cc, dx, dy);
+    const bool ok = !(power > 0.0f) && !(power < kSkipPower);
+    G = pexpf_inrange(ok ? power : 0.0f);
+    alpha = min_(clip, op * G);
+    return ok && !(alpha < K<float>::alpha_floor());
+}
+
 struct TileCoord {
     int tile, v, tx, ty;
 };
@@ -139,7 +151,7 @@ __global__ __launch_bounds__(64) void k_blend_fwd(BlendArgs a) {
             for (int k = 0; k < Q; ++k) {
                 if (!((m >> k) & 1u) || done[k]) continue;
                 float alpha, G;
-                if (pair_alpha(A.z, A.w, B.x, B.y, A.x - pxf[k], A.y - pyf[k], clip, alpha, G)) {
+                if (pair_alpha_flat(A.z, A.w, B.x, B.y, A.x - pxf[k], A.y - pyf[k], clip, alpha, G)) {
                     const float test_t = T[k] * (1.0f - alpha);
                     if (test_t < t_floor) {
                         done[k] = true;
@@ -202,8 +214,11 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendArgs a) {
     __shared__ uint32_t sM[BATCH];
     __shared__ uint32_t sS[BATCH];
 
-    float pxf[Q], pyf[Q], Tfin[Q], gp0[Q], gp1[Q], gp2[Q], dL_dTfin[Q], refine_den[Q];
-    float T[Q], Ca0[Q], Ca1[Q], Ca2[Q];
+    // E = (colour composited behind) . dL/dC + T_final * dL/dT_final, the pixel's whole
+    // "behind" term of dL/dalpha (times 1/(1 - alpha)); it is carried as one scalar instead of
+    // the three colour channels.
+    float pxf[Q], pyf[Q], gp0[Q], gp1[Q], gp2[Q], refine_den[Q];
+    float T[Q], E[Q];
     uint32_t last[Q], qmax[Q];
     const int64_t P = int64_t(W) * H;
     uint32_t tile_max = 0;
@@ -214,22 +229,22 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendArgs a) {
         pxf[k] = float(px);
         pyf[k] = float(py);
         last[k] = 0;
-        Tfin[k] = 1.0f;
-        gp0[k] = gp1[k] = gp2[k] = dL_dTfin[k] = refine_den[k] = 0.0f;
+        float Tfin = 1.0f, dL_dTfin = 0.0f;
+        gp0[k] = gp1[k] = gp2[k] = refine_den[k] = 0.0f;
         if (px < W && py < H) {
             const int64_t pid = int64_t(py) * W + px;
             last[k] = a.n_contrib[dv.vi.pixel_base + pid];
-            Tfin[k] = a.final_T[dv.vi.pixel_base + pid];
+            Tfin = a.final_T[dv.vi.pixel_base + pid];
             gp0[k] = dv.grad_color[pid];
             gp1[k] = dv.grad_color[P + pid];
             gp2[k] = dv.grad_color[2 * P + pid];
             const float ga = dv.grad_alpha ? dv.grad_alpha[pid] : 0.0f;
             const float* bg = dv.vi.cam.background;
-            dL_dTfin[k] = ((bg[0] * gp0[k] + bg[1] * gp1[k]) + bg[2] * gp2[k]) - ga;
-            refine_den[k] = 1.0f / fmaxf(1.0f - Tfin[k], 1.0e-5f);
+            dL_dTfin = ((bg[0] * gp0[k] + bg[1] * gp1[k]) + bg[2] * gp2[k]) - ga;
+            refine_den[k] = 1.0f / fmaxf(1.0f - Tfin, 1.0e-5f);
         }
-        T[k] = Tfin[k];
-        Ca0[k] = Ca1[k] = Ca2[k] = 0.0f;
+        T[k] = Tfin;
+        E[k] = Tfin * dL_dTfin;
         qmax[k] = __builtin_amdgcn_readfirstlane(wave_max_u32(last[k]));
         tile_max = qmax[k] > tile_max ? qmax[k] : tile_max;
     }
@@ -276,17 +291,18 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendArgs a) {
                 if (!((m >> k) & 1u)) continue;
                 const float dx = A.x - pxf[k], dy = A.y - pyf[k];
                 float alpha = 0.0f, G = 0.0f;
-                const bool contrib = idx < last[k] && pair_alpha(A.z, A.w, B.x, B.y, dx, dy, clip, alpha, G);
+                const bool contrib =
+                    pair_alpha_flat(A.z, A.w, B.x, B.y, dx, dy, clip, alpha, G) && idx < last[k];
                 if (__ballot(contrib) == 0) continue;
                 any = true;
                 if (contrib) {
-                    const float one_m = 1.0f - alpha;
-                    const float inv_1ma = 1.0f / one_m;
+                    // the recovered T and the gradients are not compared bit for bit, so the
+                    // hardware reciprocal / square root (1 ulp) replace the IEEE sequences
+                    const float inv_1ma = __builtin_amdgcn_rcpf(1.0f - alpha);
                     const float Tb = T[k] * inv_1ma;
                     const float wgt = alpha * Tb;
                     const float c_dot = (B.z * gp0[k] + B.w * gp1[k]) + cb * gp2[k];
-                    const float a_dot = ((Ca0[k] * gp0[k] + Ca1[k] * gp1[k]) + Ca2[k] * gp2[k]) * inv_1ma;
-                    const float dL_dalpha = (Tb * c_dot - a_dot) - Tfin[k] * inv_1ma * dL_dTfin[k];
+                    const float dL_dalpha = Tb * c_dot - E[k] * inv_1ma;
                     const float dL_dG = B.y * dL_dalpha;
                     const float gdx = G * dx, gdy = G * dy;
                     const float dG_dx = -gdx * A.z - gdy * A.w;
@@ -302,10 +318,8 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendArgs a) {
                     r[7] += wgt * gp1[k];
                     r[8] += wgt * gp2[k];
                     const float rx = ddx * fW, ry = ddy * fH;
-                    r[9] += sqrtf(rx * rx + ry * ry) * refine_den[k];
-                    Ca0[k] = fmaf(B.z, wgt, Ca0[k]);
-                    Ca1[k] = fmaf(B.w, wgt, Ca1[k]);
-                    Ca2[k] = fmaf(cb, wgt, Ca2[k]);
+                    r[9] += __builtin_amdgcn_sqrtf(rx * rx + ry * ry) * refine_den[k];
+                    E[k] = fmaf(c_dot, wgt, E[k]);
                     T[k] = Tb;
                 }
             }

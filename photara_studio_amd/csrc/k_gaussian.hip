@@ -77,35 +77,51 @@ __global__ __launch_bounds__(256) void k_gaussian_bwd(GaussArgs a) {
     float gls[3] = {0.0f, 0.0f, 0.0f}, gq[4] = {0.0f, 0.0f, 0.0f, 0.0f}, glogit = 0.0f;
     if (live) {
         const float mean[3] = {a.g.means[3 * i], a.g.means[3 * i + 1], a.g.means[3 * i + 2]};
-        float ls[3], q[4];
-        for (int k = 0; k < 3; ++k) ls[k] = ld_lp<LP>(a.g.log_scales, a.g.log_scales_lp, 3 * i + k);
-        for (int k = 0; k < 4; ++k) q[k] = a.g.quaternions[4 * i + k];
-        const float logit = ld_lp<LP>(a.g.opacity_logits, a.g.opacity_logits_lp, i);
-        const bool hf = a.g.filter_3d != nullptr;
-        const float filt = hf ? a.g.filter_3d[i] : 0.0f;
-        const Activated<float> act = activate<float>(ls, q, logit, hf, filt);
         const uint32_t mask = a.view_mask[i];
-        for (int v = 0; v < a.n_views; ++v) {
-            if (!((mask >> v) & 1u)) continue;
-            const int64_t slot = (i << a.vp_log2) + v;
-            const float4* gp = reinterpret_cast<const float4*>(a.sgrad + slot * kGradStride);
-            const float4 g0 = gp[0], g1 = gp[1], g2 = gp[2];
-            const float dconic[4] = {g0.z, g0.w, g1.x, g1.y};
-            const float dm2[2] = {g0.x, g0.y};
-            const pht_camera& cam = a.views[v].vi.cam;
-            splat_backward_pinhole(mean, act, cam, a.s, dconic, dm2, sg);
-            const float dcol[3] = {g1.z, g1.w, g2.x};
-            if (precomp) {
-                gcol[0] += dcol[0]; gcol[1] += dcol[1]; gcol[2] += dcol[2];
-            } else {
-                float d[3], raw[3];
-                view_dir(mean, cam.position, d, raw);
-                sh_backward(DEG, raw, d, my_sh, a.meta[slot].w & 7u, dcol, gsh, sg.mean);
+        // Pass 1: geometry (EWA chain) and the SH view-direction term of the mean gradient.
+        // Pass 2 (below) accumulates the SH coefficient gradients; splitting the view loop keeps
+        // the 48 coefficient accumulators out of the geometry pass's registers.
+        {
+            float ls[3], q[4];
+            for (int k = 0; k < 3; ++k) ls[k] = ld_lp<LP>(a.g.log_scales, a.g.log_scales_lp, 3 * i + k);
+            for (int k = 0; k < 4; ++k) q[k] = a.g.quaternions[4 * i + k];
+            const float logit = ld_lp<LP>(a.g.opacity_logits, a.g.opacity_logits_lp, i);
+            const bool hf = a.g.filter_3d != nullptr;
+            const float filt = hf ? a.g.filter_3d[i] : 0.0f;
+            const Activated<float> act = activate<float>(ls, q, logit, hf, filt);
+            for (int v = 0; v < a.n_views; ++v) {
+                if (!((mask >> v) & 1u)) continue;
+                const int64_t slot = (i << a.vp_log2) + v;
+                const float4* gp = reinterpret_cast<const float4*>(a.sgrad + slot * kGradStride);
+                const float4 g0 = gp[0], g1 = gp[1], g2 = gp[2];
+                const float dconic[4] = {g0.z, g0.w, g1.x, g1.y};
+                const float dm2[2] = {g0.x, g0.y};
+                const pht_camera& cam = a.views[v].vi.cam;
+                splat_backward_pinhole(mean, act, cam, a.s, dconic, dm2, sg);
+                const float dcol[3] = {g1.z, g1.w, g2.x};
+                if (precomp) {
+                    gcol[0] += dcol[0]; gcol[1] += dcol[1]; gcol[2] += dcol[2];
+                } else {
+                    float d[3], raw[3];
+                    view_dir(mean, cam.position, d, raw);
+                    sh_backward_dir(DEG, raw, d, my_sh, a.meta[slot].w & 7u, dcol, sg.mean);
+                }
+                refine += g2.y;
             }
-            refine += g2.y;
+            chain_raw(act, hf, filt, sg, gls, gq, glogit);
         }
-        chain_raw(act, hf, filt, sg, gls, gq, glogit);
         if (a.out.refine_weight) a.out.refine_weight[i] = refine;
+        if (!precomp) {
+            for (int v = 0; v < a.n_views; ++v) {
+                if (!((mask >> v) & 1u)) continue;
+                const int64_t slot = (i << a.vp_log2) + v;
+                const float* gp = a.sgrad + slot * kGradStride;
+                const float dcol[3] = {gp[6], gp[7], gp[8]};
+                float d[3], raw[3];
+                view_dir(mean, a.views[v].vi.cam.position, d, raw);
+                sh_backward_coef(DEG, d, a.meta[slot].w & 7u, dcol, gsh);
+            }
+        }
     }
     // own row only: the coefficients just read are replaced by their gradients
     if (use_sh) {

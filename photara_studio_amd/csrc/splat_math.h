@@ -543,11 +543,11 @@ template <class T> PHT_HD void dnormalize(const T v[3], const T dv[3], T o[3]) {
     o[2] = ((-n[0] * n[2] * dv[0] - n[1] * n[2] * dv[1]) + (T(1) - n[2] * n[2]) * dv[2]) * inv;
 }
 
-// Backward of sh_eval (sh.cuh:75-171): accumulates dL/dcoef into gsh[k*3+ch] and returns the
-// view-direction contribution to dL/dmean in gmean (added).
+// Backward of sh_eval (sh.cuh:75-171), in two parts that can run in separate passes:
+// sh_backward_coef accumulates dL/dcoef into gsh[k*3+ch]; sh_backward_dir adds the view-direction
+// term to gmean (through the normalisation of mean - camera). Clamped channels carry no gradient.
 template <class T>
-PHT_HD void sh_backward(int degree, const T raw_dir[3], const T d[3], const T* coef,
-                        uint32_t clamp_bits, const T dl_rgb[3], T* gsh, T gmean[3]) {
+PHT_HD void sh_backward_coef(int degree, const T d[3], uint32_t clamp_bits, const T dl_rgb[3], T* gsh) {
     T dl[3];
     for (int ch = 0; ch < 3; ++ch) dl[ch] = (clamp_bits >> ch) & 1u ? T(0) : dl_rgb[ch];
     T Y[16];
@@ -555,7 +555,15 @@ PHT_HD void sh_backward(int degree, const T raw_dir[3], const T d[3], const T* c
     const int nb = sh_count(degree);
     for (int k = 0; k < nb; ++k)
         for (int ch = 0; ch < 3; ++ch) gsh[k * 3 + ch] += Y[k] * dl[ch];
+}
+
+template <class T>
+PHT_HD void sh_backward_dir(int degree, const T raw_dir[3], const T d[3], const T* coef, uint32_t clamp_bits,
+                            const T dl_rgb[3], T gmean[3]) {
     if (degree < 1) return;
+    T dl[3];
+    for (int ch = 0; ch < 3; ++ch) dl[ch] = (clamp_bits >> ch) & 1u ? T(0) : dl_rgb[ch];
+    const int nb = sh_count(degree);
     const T x = d[0], y = d[1], z = d[2];
     // dY_k/d(x,y,z) for k >= 1
     T dYx[16], dYy[16], dYz[16];
@@ -572,7 +580,6 @@ PHT_HD void sh_backward(int degree, const T raw_dir[3], const T d[3], const T* c
     }
     if (degree > 2) {
         const T xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
-        const T* C = nullptr; (void)C;
         const T k0 = T(kShC3[0]), k1 = T(kShC3[1]), k2 = T(kShC3[2]), k3 = T(kShC3[3]);
         const T k4 = T(kShC3[4]), k5 = T(kShC3[5]), k6 = T(kShC3[6]);
         dYx[9] = k0 * (T(6) * xy);                    dYy[9] = k0 * (T(3) * (xx - yy));
@@ -606,6 +613,13 @@ PHT_HD void sh_backward(int degree, const T raw_dir[3], const T d[3], const T* c
     T dm[3];
     dnormalize(raw_dir, dd, dm);
     gmean[0] += dm[0]; gmean[1] += dm[1]; gmean[2] += dm[2];
+}
+
+template <class T>
+PHT_HD void sh_backward(int degree, const T raw_dir[3], const T d[3], const T* coef,
+                        uint32_t clamp_bits, const T dl_rgb[3], T* gsh, T gmean[3]) {
+    sh_backward_coef(degree, d, clamp_bits, dl_rgb, gsh);
+    sh_backward_dir(degree, raw_dir, d, coef, clamp_bits, dl_rgb, gmean);
 }
 
 // ---------------------------------------------------------------- per-pixel blend term
