@@ -199,6 +199,21 @@ int pht_backward(pht_workspace* ws, const pht_gaussians* g, const pht_view_grads
 int pht_l1_loss(const float* color, const float* target, float* grad_color, float* loss_sum,
                 int64_t pixels, float weight, void* stream);
 
+/* ---- fused L1 + SSIM photometric loss of one view: replaces cuda_backend::fused_l1_ssim_loss
+ * (photara/src/splat/fused_ssim.cu:473-560; semantics :120-470). 11x11 Gaussian window
+ * (sigma 1.5, zero padding), C1 = 0.01^2, C2 = 0.03^2; over the valid window
+ * [5, W-5) x [5, H-5): loss = norm * sum (ssim_weight (1 - SSIM) + (1 - ssim_weight) |x - y|),
+ * norm = photometric_weight / (3 (W-10) (H-10)). color/target/grad_color are [3,H,W]; mask
+ * ([H,W], may be NULL) multiplies both images and the gradient. accumulate != 0 adds the
+ * gradient to grad_color (the reference's +=), else overwrites it. loss_sum (device float, may
+ * be NULL) receives += loss and then needs scratch (pht_l1_ssim_scratch_bytes bytes of device
+ * memory). W and H must exceed 10 (PHT_EINVAL otherwise, like the reference's
+ * std::invalid_argument). ---- */
+size_t pht_l1_ssim_scratch_bytes(int width, int height);
+int pht_l1_ssim_loss(const float* color, const float* target, const float* mask, float* grad_color,
+                     int accumulate, float* loss_sum, void* scratch, int width, int height,
+                     float ssim_weight, float photometric_weight, void* stream);
+
 /* ---- standalone Adam (adam_kernel / adam_active_prefix_kernel, cuda_ops.cu:1336-1358,1506-1546):
  * group_stride/active/secondary lr follow the reference signature. ---- */
 int pht_adam_step(float* param, const float* grad, float* m, float* v, int64_t rows,

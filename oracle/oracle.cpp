@@ -274,15 +274,100 @@ void run_forward(const pht_gaussians& g, const pht_camera* cams, int n_views, co
     }
 }
 
+// Accuracy oracle on the float run's discrete decisions ("matched FP64"): the tile lists, the
+// per-pair skip test (alpha < 1/255, power outside [-5.6, 0]), the early termination and the SH
+// clamp flags are those of the float run `f`; every value (projection, colour, alpha, the
+// composited sums, the gradients) is computed in double. The image is piecewise smooth in the
+// inputs, with jumps where a pair crosses the 1/255 cut-off; comparing on the same piece isolates
+// the arithmetic error from such threshold flips.
+void run_forward_matched(const pht_gaussians& g, const pht_settings& s, const State<float>& f,
+                         State<double>& st, int threads) {
+    st.views = f.views;
+    st.splat_of = f.splat_of;
+    st.keys = f.keys;
+    st.vals = f.vals;
+    st.ranges = f.ranges;
+    st.n_contrib = f.n_contrib;
+    st.fp64_count = f.fp64_count;
+    st.splats.assign(f.splats.size(), SplatRec<double>{});
+#pragma omp parallel for schedule(static) num_threads(threads)
+    for (int64_t si = 0; si < int64_t(f.splats.size()); ++si) {
+        const SplatRec<float>& r32 = f.splats[size_t(si)];
+        SplatRec<double> r{};
+        int32_t radius = 0;
+        if (!preprocess_one<double>(g, s, st.views.info[size_t(r32.v)], r32.g, r, radius)) {
+            // culled in double only (a near-plane or zero-tile flip): carry the float values
+            r.mean2d[0] = r32.mean2d[0]; r.mean2d[1] = r32.mean2d[1];
+            for (int k = 0; k < 3; ++k) { r.conic[k] = r32.conic[k]; r.rgb[k] = r32.rgb[k]; }
+            r.opacity = r32.opacity;
+        }
+        r.g = r32.g;
+        r.v = r32.v;
+        r.clamp_bits = r32.clamp_bits;
+        r.depth_key = r32.depth_key;
+        r.tiles = r32.tiles;
+        r.fp64 = r32.fp64;
+        st.splats[size_t(si)] = r;
+    }
+    const Views& vs = st.views;
+    const int n_views = int(vs.info.size());
+    st.final_T.assign(size_t(vs.pixels), 1.0);
+    st.color.assign(size_t(3 * vs.pixels), 0.0);
+    st.alpha.assign(size_t(vs.pixels), 0.0);
+    const float clip32 = s.alpha_clip;
+    const double clip = double(s.alpha_clip);
+    const int ts = s.tile_size;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(threads)
+    for (int64_t tile = 0; tile < vs.tiles; ++tile) {
+        int v = 0;
+        while (v + 1 < n_views && vs.info[v + 1].tile_base <= tile) ++v;
+        const ViewInfo& vi = vs.info[v];
+        const int local = int(tile - vi.tile_base);
+        const int tx = local % vi.grid_x, ty = local / vi.grid_x;
+        const uint32_t b = st.ranges[2 * tile], e = st.ranges[2 * tile + 1];
+        const int W = vi.cam.width, H = vi.cam.height;
+        const int64_t P = int64_t(W) * H;
+        for (int py = ty * ts; py < std::min(H, (ty + 1) * ts); ++py)
+            for (int px = tx * ts; px < std::min(W, (tx + 1) * ts); ++px) {
+                float T32 = 1.0f;
+                double Tr = 1.0, C[3] = {0.0, 0.0, 0.0};
+                for (uint32_t k = b; k < e; ++k) {
+                    const SplatRec<float>& q = f.splats[f.vals[k]];
+                    float a32, G32;
+                    if (!pair_alpha(q.conic[0], q.conic[1], q.conic[2], q.opacity, q.mean2d[0] - float(px),
+                                    q.mean2d[1] - float(py), clip32, a32, G32))
+                        continue;
+                    const float test32 = T32 * (1.0f - a32);
+                    if (test32 < K<float>::transmittance_floor()) break;
+                    T32 = test32;
+                    const SplatRec<double>& r = st.splats[st.vals[k]];
+                    const double power = gaussian_power(r.conic[0], r.conic[1], r.conic[2], r.mean2d[0] - double(px),
+                                                        r.mean2d[1] - double(py));
+                    const double alpha = std::min(clip, r.opacity * std::exp(power));
+                    const double a_t = alpha * Tr;
+                    for (int ch = 0; ch < 3; ++ch) C[ch] = fma(r.rgb[ch], a_t, C[ch]);
+                    Tr = Tr * (1.0 - alpha);
+                }
+                const int64_t pid = int64_t(py) * W + px;
+                st.final_T[size_t(vi.pixel_base + pid)] = Tr;
+                for (int ch = 0; ch < 3; ++ch)
+                    st.color[size_t(3 * vi.pixel_base + ch * P + pid)] = C[ch] + Tr * double(vi.cam.background[ch]);
+                st.alpha[size_t(vi.pixel_base + pid)] = 1.0 - Tr;
+            }
+    }
+}
+
 // Per-splat blend gradients (the GradState of render_backward.cu): d_mean2d xy, d_conic xyz,
 // d_opacity, d_color, refine.
 template <class T> struct SplatGradAcc {
     T dmean[2], dconic[3], dop, dcol[3], refine;
 };
 
+// With `dec` set (matched FP64), which pairs contribute is decided by the float run `dec`.
 template <class T>
 void run_blend_backward(const pht_gaussians& g, const pht_settings& s, const State<T>& st,
-                        const pht_view_grads* dl, std::vector<SplatGradAcc<T>>& acc, int threads) {
+                        const pht_view_grads* dl, std::vector<SplatGradAcc<T>>& acc, int threads,
+                        const State<float>* dec = nullptr) {
     (void)g;
     const Views& vs = st.views;
     const int n_views = int(vs.info.size());
@@ -319,8 +404,17 @@ void run_blend_backward(const pht_gaussians& g, const pht_settings& s, const Sta
                     const SplatRec<T>& r = st.splats[st.vals[size_t(k)]];
                     const T dx = r.mean2d[0] - T(px), dy = r.mean2d[1] - T(py);
                     T alpha, G;
-                    if (!pair_alpha(r.conic[0], r.conic[1], r.conic[2], r.opacity, dx, dy, clip, alpha, G))
+                    if (dec) {
+                        const SplatRec<float>& q = dec->splats[dec->vals[size_t(k)]];
+                        float a32, G32;
+                        if (!pair_alpha(q.conic[0], q.conic[1], q.conic[2], q.opacity, q.mean2d[0] - float(px),
+                                        q.mean2d[1] - float(py), float(s.alpha_clip), a32, G32))
+                            continue;
+                        G = exp_(gaussian_power(r.conic[0], r.conic[1], r.conic[2], dx, dy));
+                        alpha = min_(clip, r.opacity * G);
+                    } else if (!pair_alpha(r.conic[0], r.conic[1], r.conic[2], r.opacity, dx, dy, clip, alpha, G)) {
                         continue;
+                    }
                     const T one_m = T(1) - alpha;
                     const T inv_1ma = T(1) / one_m;
                     Tr = Tr * inv_1ma;
@@ -483,6 +577,113 @@ void export_splat_grads(const State<T>& st, const std::vector<SplatGradAcc<T>>& 
 
 } // namespace
 
+// Fused L1 + SSIM loss and gradient of one view (fused_ssim.cu:120-470, the fused-ssim
+// formulation): separable 11-tap Gaussian moments with zero padding, SSIM partials on the valid
+// window [5, W-5) x [5, H-5), the separable backward window, plus the L1 term. Float: the same
+// operations in the same order as the GPU kernel; double: the accuracy reference. Returns the
+// loss (summed in double); grad is overwritten.
+const float kSsimWindow[11] = {0.001028380123898387f, 0.0075987582094967365f, 0.036000773310661316f,
+                               0.10936068743467331f,  0.21300552785396576f,   0.26601171493530273f,
+                               0.21300552785396576f,  0.10936068743467331f,   0.036000773310661316f,
+                               0.0075987582094967365f, 0.001028380123898387f};
+
+template <class T>
+double l1_ssim(const float* color, const float* target, const float* mask, int W, int H, float ssim_w_f,
+               float photometric_f, T* grad) {
+    const int64_t P = int64_t(W) * H;
+    const T ssim_w = T(ssim_w_f);
+    // the normalisation is a float computed on the host (fused_ssim.cu:535-536), in both precisions
+    const T norm = T(photometric_f / float(3 * (W - 10) * (H - 10)));
+    const T chain_ssim = -ssim_w * norm;
+    const T C1 = T(0.01f * 0.01f), C2 = T(0.03f * 0.03f);
+    T w[11];
+    for (int d = 0; d < 11; ++d) w[d] = T(kSsimWindow[d]);
+    auto inside = [&](int x, int y) { return x >= 0 && y >= 0 && x < W && y < H; };
+    auto valid = [&](int x, int y) { return x >= 5 && x < W - 5 && y >= 5 && y < H - 5; };
+    const size_t np = static_cast<size_t>(P);
+    std::vector<T> X(np), Y(np), hm(5 * np), dmap(3 * np), hd(3 * np);
+    double loss = 0.0;
+    for (int ch = 0; ch < 3; ++ch) {
+        for (int64_t p = 0; p < P; ++p) {
+            const float m = mask ? mask[p] : 1.0f;
+            X[size_t(p)] = T(mask ? color[ch * P + p] * m : color[ch * P + p]);
+            Y[size_t(p)] = T(mask ? target[ch * P + p] * m : target[ch * P + p]);
+        }
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0), s4 = T(0);
+                for (int d = 0; d < 11; ++d) {
+                    const int xx = x + d - 5;
+                    const T a = inside(xx, y) ? X[size_t(y) * W + xx] : T(0);
+                    const T b = inside(xx, y) ? Y[size_t(y) * W + xx] : T(0);
+                    s0 += a * w[d];
+                    s1 += a * a * w[d];
+                    s2 += b * w[d];
+                    s3 += b * b * w[d];
+                    s4 += a * b * w[d];
+                }
+                const size_t i = size_t(y) * W + x;
+                hm[i] = s0; hm[P + i] = s1; hm[2 * P + i] = s2; hm[3 * P + i] = s3; hm[4 * P + i] = s4;
+            }
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                T m[5] = {T(0), T(0), T(0), T(0), T(0)};
+                for (int d = 0; d < 11; ++d) {
+                    const int yy = y + d - 5;
+                    for (int k = 0; k < 5; ++k)
+                        m[k] += (yy >= 0 && yy < H ? hm[size_t(k) * P + size_t(yy) * W + x] : T(0)) * w[d];
+                }
+                const size_t i = size_t(y) * W + x;
+                T d0 = T(0), d1 = T(0), d2 = T(0);
+                if (valid(x, y)) {
+                    const T mu1 = m[0], mu2 = m[2];
+                    const T mu1_sq = mu1 * mu1, mu2_sq = mu2 * mu2;
+                    const T sigma1_sq = m[1] - mu1_sq, sigma2_sq = m[3] - mu2_sq, sigma12 = m[4] - mu1 * mu2;
+                    const T A = mu1_sq + mu2_sq + C1;
+                    const T B = sigma1_sq + sigma2_sq + C2;
+                    const T Cc = T(2) * mu1 * mu2 + C1;
+                    const T D = T(2) * sigma12 + C2;
+                    const T ab = A * B;
+                    d0 = ((T(2) * mu2 * D) / ab - (T(2) * mu2 * Cc) / ab - (T(2) * mu1 * Cc * D) / (A * A * B) +
+                          (T(2) * mu1 * Cc * D) / (A * B * B)) * chain_ssim;
+                    d1 = (-Cc * D / (A * B * B)) * chain_ssim;
+                    d2 = (T(2) * Cc / ab) * chain_ssim;
+                    const T ssim = Cc * D / ab;
+                    const T l1 = std::fabs(X[i] - Y[i]);
+                    loss += double(norm * (ssim_w * (T(1) - ssim) + (T(1) - ssim_w) * l1));
+                }
+                dmap[i] = d0; dmap[P + i] = d1; dmap[2 * P + i] = d2;
+            }
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                T a[3] = {T(0), T(0), T(0)};
+                for (int d = 0; d < 11; ++d) {
+                    const int xx = x + d - 5;
+                    for (int k = 0; k < 3; ++k)
+                        a[k] += (inside(xx, y) ? dmap[size_t(k) * P + size_t(y) * W + xx] : T(0)) * w[d];
+                }
+                const size_t i = size_t(y) * W + x;
+                hd[i] = a[0]; hd[P + i] = a[1]; hd[2 * P + i] = a[2];
+            }
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                T g[3] = {T(0), T(0), T(0)};
+                for (int d = 0; d < 11; ++d) {
+                    const int yy = y + d - 5;
+                    for (int k = 0; k < 3; ++k)
+                        g[k] += (yy >= 0 && yy < H ? hd[size_t(k) * P + size_t(yy) * W + x] : T(0)) * w[d];
+                }
+                const size_t i = size_t(y) * W + x;
+                const T xv = X[i], yv = Y[i];
+                const T chain = valid(x, y) ? norm : T(0);
+                const T sign = xv == yv ? T(0) : std::copysign(T(1), xv - yv);
+                const T m = mask ? T(mask[i]) : T(1);
+                grad[size_t(ch) * P + i] = (g[0] + T(2) * xv * g[1] + yv * g[2] + (T(1) - ssim_w) * sign * chain) * m;
+            }
+    }
+    return loss;
+}
+
 extern "C" {
 
 int oracle_abi_version(void) { return PHT_ABI_VERSION; }
@@ -566,13 +767,25 @@ int oracle_enumerate(float mx, float my, const float* conic3, float op, int grid
 }
 
 // Forward. precision 32: parity oracle (float outputs); 64: accuracy oracle (double results
-// rounded to float in `outs`). Keys/values/ranges/n_contrib/final_T of the float run are copied
+// rounded to float in `outs`); 65: accuracy oracle on the float run's decisions (see
+// run_forward_matched; `outs` is not written). Keys/values/ranges/n_contrib/final_T of the float run are copied
 // when the pointers are non-null.
 int oracle_forward(int precision, const pht_gaussians* g, const pht_camera* cams, int n_views,
                    const pht_settings* s, pht_view_outputs* outs, uint64_t* keys, uint32_t* values,
                    int64_t key_cap, uint32_t* ranges, uint32_t* n_contrib, float* final_T,
                    pht_counts* counts, int threads) {
     g_alpha_clip = s->alpha_clip;
+    if (precision == 65) {
+        run_forward<float>(*g, cams, n_views, *s, g_state32, nullptr, threads);
+        run_forward_matched(*g, *s, g_state32, g_state64, threads);
+        if (counts) {
+            counts->visible = int64_t(g_state64.splats.size());
+            counts->instances = int64_t(g_state64.keys.size());
+            counts->fp64_conics = g_state64.fp64_count;
+            counts->overflow = 0;
+        }
+        return 0;
+    }
     if (precision == 64) {
         run_forward<double>(*g, cams, n_views, *s, g_state64, outs, threads);
         if (counts) {
@@ -664,7 +877,7 @@ int oracle_bruteforce(float* color_concat, float* alpha_concat, float* final_T) 
 // Images of the last forward at its own precision (double for precision 64): colour per view
 // [3,H,W] and alpha per view [H,W], each concatenated over views.
 int oracle_images(int precision, double* color, double* alpha) {
-    if (precision == 64) {
+    if (precision == 64 || precision == 65) {
         std::copy(g_state64.color.begin(), g_state64.color.end(), color);
         std::copy(g_state64.alpha.begin(), g_state64.alpha.end(), alpha);
     } else {
@@ -680,9 +893,9 @@ int oracle_backward(int precision, const pht_gaussians* g, const pht_settings* s
                     void* means, void* log_scales, void* quats, void* logits, void* sh, void* colors,
                     void* refine, int threads) {
     const int64_t N = g->count;
-    if (precision == 64) {
+    if (precision == 64 || precision == 65) {
         std::vector<SplatGradAcc<double>>& acc = g_acc64;
-        run_blend_backward<double>(*g, *s, g_state64, dl, acc, threads);
+        run_blend_backward<double>(*g, *s, g_state64, dl, acc, threads, precision == 65 ? &g_state32 : nullptr);
 #pragma omp parallel for schedule(static) num_threads(threads)
         for (int64_t i = 0; i < N; ++i) {
             RowGrad<double> r;
@@ -707,7 +920,7 @@ int oracle_backward(int precision, const pht_gaussians* g, const pht_settings* s
 // Per-(gaussian, view) blend gradients of the last backward: out[(g * n_views + v) * 10 + k]
 // (d_mean2d.xy, d_conic.xyz, d_opacity, d_color.rgb, refine), zero where not visible.
 int oracle_splat_grads(int precision, int64_t n, double* out) {
-    if (precision == 64) export_splat_grads<double>(g_state64, g_acc64, n, out);
+    if (precision == 64 || precision == 65) export_splat_grads<double>(g_state64, g_acc64, n, out);
     else export_splat_grads<float>(g_state32, g_acc32, n, out);
     return 0;
 }
@@ -723,6 +936,17 @@ double oracle_l1_loss(const float* color, const float* target, float* grad, int6
         grad[c] = ((weight * inv_p) / 3.0f) * sgn;
     }
     return loss * double(weight) / (3.0 * double(pixels));
+}
+
+// Fused L1 + SSIM of one [3,H,W] view (mask [H,W] or NULL): precision 32 writes float grad,
+// 64 writes double grad; returns the loss.
+double oracle_l1_ssim(int precision, const float* color, const float* target, const float* mask, int width, int height,
+                      float ssim_weight, float photometric_weight, void* grad) {
+    if (precision == 64)
+        return l1_ssim<double>(color, target, mask, width, height, ssim_weight, photometric_weight,
+                               static_cast<double*>(grad));
+    return l1_ssim<float>(color, target, mask, width, height, ssim_weight, photometric_weight,
+                          static_cast<float*>(grad));
 }
 
 // Standalone Adam over rows x row_stride with an active prefix (see pht_adam_step).

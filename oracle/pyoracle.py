@@ -28,6 +28,8 @@ def load() -> C.CDLL:
         lib = C.CDLL(str(LIB))
         vp = C.c_void_p
         lib.oracle_pexpf.argtypes = [vp, vp, C.c_int64]
+        lib.oracle_l1_ssim.argtypes = [C.c_int, vp, vp, vp, C.c_int, C.c_int, C.c_float, C.c_float, vp]
+        lib.oracle_l1_ssim.restype = C.c_double
         lib.oracle_pexpf_inrange.argtypes = [vp, vp, C.c_int64]
         lib.oracle_plogf.argtypes = [vp, vp, C.c_int64]
         lib.oracle_gaussian_power.argtypes = [vp, vp, vp, C.c_int64]
@@ -126,7 +128,9 @@ def settings_for(scene: Scene, options: RasterizeOptions) -> _abi.pht_settings:
 
 
 class OracleRun:
-    """One oracle forward (and optional backward) over a list of cameras."""
+    """One oracle forward (and optional backward) over a list of cameras. precision: 32 = float
+    parity oracle, 64 = double accuracy oracle, 65 = double values on the float run's discrete
+    decisions (tile lists, 1/255 skips, early termination, SH clamps)."""
 
     def __init__(self, scene: Scene, cameras: Sequence[Camera], options: RasterizeOptions, precision: int = 32,
                  threads: Optional[int] = None, key_cap: int = 1 << 26):
@@ -162,8 +166,8 @@ class OracleRun:
             self.keys, self.values = keys[:ni].copy(), values[:ni].copy()
             self.ranges, self.n_contrib, self.final_T = ranges, n_contrib, final_T
         else:
-            lib.oracle_forward(64, C.byref(self.g), self.cams, nv, C.byref(self.settings), outs, None, None, 0, None,
-                               None, None, C.byref(counts), self.threads)
+            lib.oracle_forward(precision, C.byref(self.g), self.cams, nv, C.byref(self.settings), outs, None, None, 0,
+                               None, None, None, C.byref(counts), self.threads)
         self.counts = counts
         self._outs = outs
         # images at the run's own precision (double for the accuracy oracle)
@@ -188,7 +192,7 @@ class OracleRun:
             dl[k].grad_color = gcs[k].ctypes.data
             dl[k].grad_alpha = None if gas[k] is None else gas[k].ctypes.data
         n, s = self.scene.n, self.scene
-        dt = np.float64 if self.precision == 64 else np.float32
+        dt = np.float64 if self.precision in (64, 65) else np.float32
         out = {
             "means": np.zeros((n, 3), dt), "log_scales": np.zeros((n, 3), dt), "quaternions": np.zeros((n, 4), dt),
             "opacity_logits": np.zeros((n, 1), dt), "sh": np.zeros((n, s.sh_bases, 3), dt),
@@ -262,6 +266,18 @@ def pexpf_inrange(x: np.ndarray) -> np.ndarray:
     y = np.empty_like(x)
     load().oracle_pexpf_inrange(_p(x), _p(y), x.size)
     return y
+
+
+def l1_ssim(color: np.ndarray, target: np.ndarray, mask: Optional[np.ndarray] = None, ssim_weight: float = 0.2,
+            photometric_weight: float = 1.0, precision: int = 32):
+    """Fused L1 + SSIM of one [3,H,W] view: (loss, gradient wrt color) at the given precision."""
+    color, target = f32(color), f32(target)
+    m = None if mask is None else f32(mask)
+    _, H, W = color.shape
+    grad = np.zeros(color.shape, np.float64 if precision == 64 else np.float32)
+    loss = load().oracle_l1_ssim(precision, _p(color), _p(target), None if m is None else _p(m), W, H,
+                                 ssim_weight, photometric_weight, _p(grad))
+    return float(loss), grad
 
 
 def plogf(x: np.ndarray) -> np.ndarray:

@@ -114,6 +114,9 @@ SIGNATURES = {
     "pht_backward": (C.c_int, [C.c_void_p, C.POINTER(pht_gaussians), C.POINTER(pht_view_grads),
                                C.POINTER(pht_model_grads), C.POINTER(pht_adam), C.c_void_p]),
     "pht_l1_loss": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_float, C.c_void_p]),
+    "pht_l1_ssim_scratch_bytes": (C.c_size_t, [C.c_int, C.c_int]),
+    "pht_l1_ssim_loss": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                   C.c_int, C.c_int, C.c_float, C.c_float, C.c_void_p]),
     "pht_adam_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_int64,
                                 C.c_float, C.c_float, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_float,
                                 C.c_float, C.c_float, C.c_float, C.c_int32, C.c_void_p]),

@@ -417,6 +417,22 @@ int pht_l1_loss(const float* color, const float* target, float* grad_color, floa
     return check_hip(hipGetLastError(), "l1 launch");
 }
 
+size_t pht_l1_ssim_scratch_bytes(int width, int height) {
+    if (width <= 10 || height <= 10) return 0;
+    return l1_ssim_scratch_floats(width, height) * sizeof(float);
+}
+
+int pht_l1_ssim_loss(const float* color, const float* target, const float* mask, float* grad_color, int accumulate,
+                     float* loss_sum, void* scratch, int width, int height, float ssim_weight,
+                     float photometric_weight, void* stream) {
+    if (!color || !target || !grad_color) return fail(PHT_EINVAL, "color, target and grad_color are required");
+    if (width <= 10 || height <= 10) return fail(PHT_EINVAL, "fused SSIM requires width and height > 10");
+    if (loss_sum && !scratch) return fail(PHT_EINVAL, "loss_sum needs scratch (pht_l1_ssim_scratch_bytes)");
+    launch_l1_ssim(color, target, mask, grad_color, accumulate, loss_sum, static_cast<float*>(scratch), width, height,
+                   ssim_weight, photometric_weight, static_cast<hipStream_t>(stream));
+    return check_hip(hipGetLastError(), "l1+ssim launch");
+}
+
 int pht_adam_step(float* param, const float* grad, float* m, float* v, int64_t rows, int64_t row_stride,
                   int64_t active_cols, float lr, float lr_secondary, int64_t secondary_from_col, float beta1,
                   float beta2, float correction1, float correction2, float epsilon, float clamp_min, float clamp_max,

@@ -206,9 +206,11 @@ __global__ __launch_bounds__(256) void k_emit(const uint32_t* sorted_slots, cons
         if (__ballot(over) != 0 || total_cols > uint32_t(kEmitCols)) {
             if (tw.ok) {
                 uint32_t k = 0;
+#pragma unroll 1
                 for (int tu = tw.u_min; tu < tw.u_max; ++tu) {
                     int v0, v1;
                     tile_walk_column(tw, tu, v0, v1);
+#pragma unroll 1
                     for (int tv = v0; tv < v1; ++tv) {
                         if (k < count) {
                             const int tx = tw.iterate_x ? tu : tv, ty = tw.iterate_x ? tv : tu;
@@ -226,6 +228,7 @@ __global__ __launch_bounds__(256) void k_emit(const uint32_t* sorted_slots, cons
             uint32_t r = col_incl - ncols;
             const uint32_t pos0 = cnt_incl - count;
             uint32_t done = 0;
+#pragma unroll 1
             for (int tu = tw.u_min; tw.ok && tu < tw.u_max; ++tu) {
                 int v0, v1;
                 tile_walk_column(tw, tu, v0, v1);
@@ -242,8 +245,10 @@ __global__ __launch_bounds__(256) void k_emit(const uint32_t* sorted_slots, cons
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll 1
         for (uint32_t o = uint32_t(lane); o < total; o += 64u) {
             int lo = 0, hi = int(total_cols) - 1;
+#pragma unroll 1
             while (lo < hi) {
                 const int mid = (lo + hi + 1) >> 1;
                 if (s_col[w][mid].x <= o) lo = mid;
@@ -266,16 +271,32 @@ __global__ __launch_bounds__(256) void k_emit(const uint32_t* sorted_slots, cons
 }
 
 // Per-tile [start, end) of the tile-sorted list (extract_tile_ranges, render_forward.cu:215-232).
+// Each thread takes four consecutive keys (one 16-byte load) and their two outer neighbours.
 __global__ __launch_bounds__(256) void k_ranges(const uint32_t* keys, const Counters* counters, uint32_t inst_cap,
                                                 uint32_t n_tiles, uint2* ranges) {
     if (counters->overflow) return; // the list is incomplete: leave every range empty
     const uint32_t ni = counters->instances;
     const uint32_t n = ni < inst_cap ? ni : inst_cap;
-    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
-        const uint32_t tile = keys[i];
-        if (tile >= n_tiles) continue;
-        if (i == 0 || keys[i - 1] != tile) ranges[tile].x = i;
-        if (i + 1 == n || keys[i + 1] != tile) ranges[tile].y = i + 1;
+    const uint32_t quads = (n + 3u) >> 2;
+    for (uint32_t q = blockIdx.x * 256u + threadIdx.x; q < quads; q += gridDim.x * 256u) {
+        const uint32_t i0 = q << 2;
+        uint32_t k[6]; // k[0] = keys[i0 - 1], k[1..4] = keys[i0 .. i0 + 3], k[5] = keys[i0 + 4]
+        if (i0 + 3u < n) {
+            const uint4 v = reinterpret_cast<const uint4*>(keys)[q];
+            k[1] = v.x; k[2] = v.y; k[3] = v.z; k[4] = v.w;
+        } else {
+            for (int e = 0; e < 4; ++e) k[1 + e] = i0 + uint32_t(e) < n ? keys[i0 + uint32_t(e)] : 0xffffffffu;
+        }
+        k[0] = i0 > 0u ? keys[i0 - 1u] : 0xffffffffu;
+        k[5] = i0 + 4u < n ? keys[i0 + 4u] : 0xffffffffu;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const uint32_t i = i0 + uint32_t(e);
+            const uint32_t tile = k[1 + e];
+            if (i >= n || tile >= n_tiles) continue;
+            if (i == 0 || k[e] != tile) ranges[tile].x = i;
+            if (i + 1 == n || k[2 + e] != tile) ranges[tile].y = i + 1;
+        }
     }
 }
 

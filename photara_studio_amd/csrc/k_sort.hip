@@ -209,19 +209,25 @@ __global__ __launch_bounds__(kScanThreads) void k_radix_downsweep(const uint32_t
     uint32_t lo, hi;
     chunk_of(n, nb, blockIdx.x, lo, hi);
     const uint64_t lt = __lanemask_lt();
-    for (uint32_t base = lo; base < hi; base += kSortTile) {
-        for (int k = 0; k < 4; ++k) s_cnt[k][t] = 0;
-        __syncthreads();
-        const uint32_t wbase = base + uint32_t(w) * (kSortTile / 4);
-        uint32_t key[kSortRows], val[kSortRows], aux[kSortRows], rank[kSortRows];
+    // The next tile's keys are loaded into the same registers as soon as the current tile has
+    // been reordered into LDS, so its global loads overlap the current tile's write-out.
+    uint32_t key[kSortRows], val[kSortRows], aux[kSortRows], rank[kSortRows];
+    auto load_tile = [&](uint32_t b) {
+        const uint32_t wb = b + uint32_t(w) * (kSortTile / 4);
 #pragma unroll
         for (int r = 0; r < kSortRows; ++r) {
-            const uint32_t i = wbase + uint32_t(r) * 64u + uint32_t(lane);
+            const uint32_t i = wb + uint32_t(r) * 64u + uint32_t(lane);
             const bool valid = i < hi;
             key[r] = valid ? keys_in[i] : 0u;
             val[r] = valid ? vals_in[i] : 0u;
             if constexpr (AUX) aux[r] = valid ? aux_in[i] : 0u;
         }
+    };
+    if (lo < hi) load_tile(lo);
+    for (uint32_t base = lo; base < hi; base += kSortTile) {
+        for (int k = 0; k < 4; ++k) s_cnt[k][t] = 0;
+        __syncthreads();
+        const uint32_t wbase = base + uint32_t(w) * (kSortTile / 4);
 #pragma unroll
         for (int r = 0; r < kSortRows; ++r) {
             const uint32_t i = wbase + uint32_t(r) * 64u + uint32_t(lane);
@@ -267,6 +273,7 @@ __global__ __launch_bounds__(kScanThreads) void k_radix_downsweep(const uint32_t
                 if constexpr (AUX) s_aux[p] = aux[r];
             }
         }
+        if (base + kSortTile < hi) load_tile(base + kSortTile);
         __syncthreads();
         const uint32_t tile_n = hi - base < uint32_t(kSortTile) ? hi - base : uint32_t(kSortTile);
         for (uint32_t j = t; j < tile_n; j += kScanThreads) {

@@ -92,6 +92,12 @@ void launch_gaussian_backward(const GaussArgs& a, bool adam, hipStream_t st);
 void launch_adam_standalone(float* p, const float* g, float* m, float* v, int64_t rows, int64_t stride,
                             int64_t active, float lr, float lr2, int64_t secondary_from, float b1, float b2,
                             float c1, float c2, float eps, float lo, float hi, int prefix, hipStream_t st);
+// k_ssim.hip
+size_t l1_ssim_scratch_floats(int width, int height);
+void launch_l1_ssim(const float* color, const float* target, const float* mask, float* grad, int accumulate,
+                    float* loss_sum, float* scratch, int width, int height, float ssim_weight,
+                    float photometric_weight, hipStream_t st);
+
 void launch_l1_loss(const float* color, const float* target, float* grad, float* loss_sum, int64_t pixels,
                     float weight, hipStream_t st);
 void launch_zero_f32(float* p, int n, hipStream_t st);

@@ -388,10 +388,12 @@ template <class T> struct TileWalk {
     bool ok;
     bool iterate_x;   // columns run along x (u = tile x, v = tile y); else u = y, v = x
     int gu, gv, u_min, u_max;
-    T cu, cv, au, b, det, threshold, eu, ev, arg_v_min, arg_v_max, tsf;
-    T av;
+    T cu, cv, b, det, threshold, eu, ev, arg_v_min, arg_v_max, tsf, inv_ts;
+    T av, inv_av;
 };
 
+// The tile edge is 8 or 16, so x / ts and x * (1 / ts) are the same number; the per-column
+// quotients by av use one reciprocal per splat (the oracle runs this same code).
 template <class T>
 PHT_HD TileWalk<T> tile_walk_setup(T mx, T my, const T conic[3], T op, int grid_x, int grid_y, int ts) {
     TileWalk<T> w;
@@ -403,53 +405,55 @@ PHT_HD TileWalk<T> tile_walk_setup(T mx, T my, const T conic[3], T op, int grid_
     if (!(threshold >= T(0))) return w;
     const T ex = sqrt_(threshold * c / det);
     const T ey = sqrt_(threshold * a / det);
-    const T tsf = T(ts);
-    const bool iterate_x = (T(2) * ex) / tsf <= (T(2) * ey) / tsf;
+    const T inv_ts = T(1) / T(ts);
+    const bool iterate_x = ex <= ey; // walk the shorter screen extent
     w.iterate_x = iterate_x;
     w.cu = iterate_x ? mx : my;
     w.cv = iterate_x ? my : mx;
-    w.au = iterate_x ? a : c;
+    const T au = iterate_x ? a : c;
     w.av = iterate_x ? c : a;
+    w.inv_av = T(1) / w.av;
     w.gu = iterate_x ? grid_x : grid_y;
     w.gv = iterate_x ? grid_y : grid_x;
     w.b = b;
     w.det = det;
     w.threshold = threshold;
-    w.tsf = tsf;
+    w.tsf = T(ts);
+    w.inv_ts = inv_ts;
     w.eu = sqrt_(threshold * w.av / det);
-    w.ev = sqrt_(threshold * w.au / det);
-    int u_min = int(floor_((w.cu - w.eu) / tsf));
+    w.ev = sqrt_(threshold * au / det);
+    int u_min = int(floor_((w.cu - w.eu) * inv_ts));
     u_min = u_min < w.gu ? u_min : w.gu;
     u_min = u_min > 0 ? u_min : 0;
-    int u_max = int(floor_((w.cu + w.eu) / tsf)) + 1;
+    int u_max = int(floor_((w.cu + w.eu) * inv_ts)) + 1;
     u_max = u_max < w.gu ? u_max : w.gu;
     w.u_min = u_min;
     w.u_max = u_max;
     if (u_min >= u_max) return w;
-    w.arg_v_min = w.cu + b * w.ev / w.au;
-    w.arg_v_max = w.cu - b * w.ev / w.au;
+    w.arg_v_min = w.cu + b * w.ev / au;
+    w.arg_v_max = w.cu - b * w.ev / au;
     w.ok = true;
     return w;
 }
 
 // Tile interval [v_min, v_max) of column tu (may be empty).
 template <class T> PHT_HD void tile_walk_column(const TileWalk<T>& w, int tu, int& v_min, int& v_max) {
-    const T cu = w.cu, cv = w.cv, av = w.av, b = w.b, det = w.det, tsf = w.tsf;
-    const T s0 = max_(cu - w.eu, T(tu) * tsf);
-    const T s1 = min_(cu + w.eu, T(tu + 1) * tsf);
+    const T cu = w.cu, cv = w.cv, av = w.av, b = w.b, det = w.det;
+    const T s0 = max_(cu - w.eu, T(tu) * w.tsf);
+    const T s1 = min_(cu + w.eu, T(tu + 1) * w.tsf);
     const T d0 = s0 - cu, d1 = s1 - cu;
     const T r0 = sqrt_(max_(T(0), av * w.threshold - det * d0 * d0));
     const T r1 = sqrt_(max_(T(0), av * w.threshold - det * d1 * d1));
-    const T lo0 = cv + (-b * d0 - r0) / av;
-    const T lo1 = cv + (-b * d1 - r1) / av;
-    const T hi0 = cv + (-b * d0 + r0) / av;
-    const T hi1 = cv + (-b * d1 + r1) / av;
+    const T lo0 = cv + (-b * d0 - r0) * w.inv_av;
+    const T lo1 = cv + (-b * d1 - r1) * w.inv_av;
+    const T hi0 = cv + (-b * d0 + r0) * w.inv_av;
+    const T hi1 = cv + (-b * d1 + r1) * w.inv_av;
     const T sv_min = (w.arg_v_min >= s0 && w.arg_v_min <= s1) ? cv - w.ev : min_(lo0, lo1);
     const T sv_max = (w.arg_v_max >= s0 && w.arg_v_max <= s1) ? cv + w.ev : max_(hi0, hi1);
-    int lo = int(floor_(sv_min / tsf));
+    int lo = int(floor_(sv_min * w.inv_ts));
     lo = lo < w.gv ? lo : w.gv;
     lo = lo > 0 ? lo : 0;
-    int hi = int(floor_(sv_max / tsf)) + 1;
+    int hi = int(floor_(sv_max * w.inv_ts)) + 1;
     hi = hi < w.gv ? hi : w.gv;
     v_min = lo;
     v_max = hi;
@@ -461,9 +465,11 @@ PHT_HD uint32_t enumerate_tiles(T mx, T my, const T conic[3], T op, int grid_x, 
     const TileWalk<T> w = tile_walk_setup(mx, my, conic, op, grid_x, grid_y, ts);
     if (!w.ok) return 0;
     uint32_t count = 0;
+#pragma unroll 1
     for (int tu = w.u_min; tu < w.u_max; ++tu) {
         int v_min, v_max;
         tile_walk_column(w, tu, v_min, v_max);
+#pragma unroll 1
         for (int tv = v_min; tv < v_max; ++tv) {
             emit(w.iterate_x ? tu : tv, w.iterate_x ? tv : tu);
             ++count;

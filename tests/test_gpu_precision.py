@@ -2,12 +2,17 @@
 workload: 400k Gaussians, 4 x 960x540): SH, log-scales and opacity logits stored in bf16 / fp16,
 blending and accumulation in FP32.
 
-Reported per format:
-  * image max-abs error vs the FP64 oracle render of the same stored inputs (bar: 1e-3);
-  * per-parameter gradient error (max-abs error / max-abs value) vs the FP64 oracle (bar: 1e-3);
-  * the count of splats routed through the FP64 2x2 conic path, equal to the float oracle's;
-  * for information, the image error vs the FP64 render of the FP32 master values (storage
-    quantisation, no bar).
+The rendered image is piecewise smooth in the inputs: a pair whose alpha sits within rounding of
+the 1/255 cut-off is kept in one precision and dropped in the other, which moves that pixel by
+alpha x colour (up to ~4e-3). Two FP64 references are therefore reported per format:
+  * the decision-matched FP64 oracle (the float run's tile lists, skips, termination and SH
+    clamps, every value in double): the arithmetic error; bar 1e-3 max-abs for images and
+    1e-3 for gradients (max-abs error over max-abs value);
+  * the plain FP64 oracle of the same stored inputs: the same plus threshold flips; only
+    isolated pixels (<= 1e-4 of them) may exceed 1e-3.
+Also printed: the count of splats routed through the FP64 2x2 conic path (equal to the float
+oracle's), and the image error against the FP64 render of the FP32 master values (the storage
+quantisation itself, no bar).
 """
 from __future__ import annotations
 
@@ -18,6 +23,24 @@ import torch
 from conftest import small_scene, training_options
 
 pytestmark = pytest.mark.gpu
+
+PARAMS = ("means", "log_scales", "quaternions", "opacity_logits", "sh")
+
+
+def _img_err(res, ref):
+    diffs = [np.abs(x.color.cpu().numpy().astype(np.float64) - ref.color_hp[k]) for k, x in enumerate(res)]
+    adiffs = [np.abs(x.alpha.cpu().numpy().astype(np.float64) - ref.alpha_hp[k]) for k, x in enumerate(res)]
+    over = sum(int((d.max(axis=0) > 1e-3).sum()) for d in diffs)
+    return max(float(d.max()) for d in diffs), max(float(d.max()) for d in adiffs), over
+
+
+def _grad_err(grads, ref):
+    out = {}
+    for key in PARAMS:
+        got = getattr(grads, key).cpu().numpy().astype(np.float64)
+        want = np.asarray(ref[key], np.float64)
+        out[key] = float(np.abs(got - want).max() / max(np.abs(want).max(), 1e-30))
+    return out
 
 
 @pytest.mark.parametrize("lp", ["bf16", "fp16"])
@@ -37,35 +60,25 @@ def test_reduced_precision_error_vs_fp64(cuda, oracle, lp):
     rng = np.random.default_rng(3)
     gimg = [rng.normal(0, 1e-3, size=tuple(x.color.shape)).astype(np.float32) for x in res]
     grads = r.backward_views(model, res, [torch.from_numpy(g).to(cuda) for g in gimg])
+    n_pix = sum(c.width * c.height for c in cams)
 
-    print(f"[precision] {lp}: FP64 oracle", flush=True)
+    print(f"[precision] {lp}: FP64 oracles", flush=True)
     scene = oracle.Scene(arrays, lp=lp)
-    o64 = oracle.OracleRun(scene, cams, opts, 64)
-    og = o64.backward(gimg)
+    matched = oracle.OracleRun(scene, cams, opts, 65)
+    m_img, m_alpha, _ = _img_err(res, matched)
+    m_grad = _grad_err(grads, matched.backward(gimg))
+    plain = oracle.OracleRun(scene, cams, opts, 64)
+    p_img, p_alpha, over = _img_err(res, plain)
+    p_grad = _grad_err(grads, plain.backward(gimg))
     o32 = oracle.OracleRun(scene, cams, opts, 32)
-    diffs = [np.abs(x.color.cpu().numpy().astype(np.float64) - o64.color_hp[k]) for k, x in enumerate(res)]
-    img_err = max(float(d.max()) for d in diffs)
-    over = sum(int((d.max(axis=0) > 1e-3).sum()) for d in diffs)
-    n_pix = sum(d.shape[1] * d.shape[2] for d in diffs)
-    alpha_err = max(float(np.abs(x.alpha.cpu().numpy().astype(np.float64) - o64.alpha_hp[k]).max())
-                    for k, x in enumerate(res))
-    gerr = {}
-    for key in ("means", "log_scales", "quaternions", "opacity_logits", "sh"):
-        got = getattr(grads, key).cpu().numpy().astype(np.float64)
-        want = np.asarray(og[key], np.float64)
-        gerr[key] = float(np.abs(got - want).max() / max(np.abs(want).max(), 1e-30))
-    q64 = oracle.OracleRun(oracle.Scene(arrays), cams, opts, 64)
-    quant = max(float(np.abs(x.color.cpu().numpy().astype(np.float64) - q64.color_hp[k]).max())
-                for k, x in enumerate(res))
-    print(f"  {lp}: image max-abs vs FP64 (same inputs) {img_err:.2e} ({over} of {n_pix} pixels above 1e-3), "
-          f"alpha {alpha_err:.2e}; "
-          f"fp64-routed conics {int(counts.fp64_conics)} (oracle {int(o32.counts.fp64_conics)})")
-    print("  gradient error vs FP64: " + ", ".join(f"{k} {v:.2e}" for k, v in gerr.items()))
-    print(f"  storage quantisation: image max-abs vs FP64 render of the FP32 masters {quant:.2e}")
-    # A pair whose alpha lands within rounding of the 1/255 cut-off is kept in one precision and
-    # dropped in the other; such a flip moves that pixel by up to alpha * colour ~ 4e-3. Only
-    # isolated pixels may exceed 1e-3.
+    quant, _, _ = _img_err(res, oracle.OracleRun(oracle.Scene(arrays), cams, opts, 64))
+    print(f"  {lp} vs decision-matched FP64: image {m_img:.2e}, alpha {m_alpha:.2e}; gradients "
+          + ", ".join(f"{k} {v:.2e}" for k, v in m_grad.items()))
+    print(f"  {lp} vs plain FP64: image {p_img:.2e} ({over} of {n_pix} pixels above 1e-3), alpha {p_alpha:.2e}; "
+          "gradients " + ", ".join(f"{k} {v:.2e}" for k, v in p_grad.items()))
+    print(f"  fp64-routed conics {int(counts.fp64_conics)} (float oracle {int(o32.counts.fp64_conics)}); "
+          f"storage quantisation vs FP64 of the FP32 masters: image {quant:.2e}")
+    assert m_img <= 1e-3 and m_alpha <= 1e-3, (m_img, m_alpha)
+    assert all(v <= 1e-3 for v in m_grad.values()), m_grad
     assert over <= n_pix * 1e-4, (over, n_pix)
-    assert img_err <= 1e-2 and alpha_err <= 1e-2
-    assert all(v <= 1e-2 for v in gerr.values()), gerr
     assert int(counts.fp64_conics) == int(o32.counts.fp64_conics)
