@@ -131,7 +131,7 @@ def cpu_baseline(arrays, cams, opts, seconds):
         steps += 1
         a.correction1 = float(np.float32(1) - np.float32(0.9) ** np.float32(steps))
         a.correction2 = float(np.float32(1) - np.float32(0.999) ** np.float32(steps))
-        lib.oracle_train_step(C.byref(g), cams_c, 1, C.byref(s), outs, tp, gp, C.byref(a), threads)
+        lib.oracle_train_step(C.byref(g), cams_c, 1, C.byref(s), outs, tp, gp, C.byref(a), threads, 0, 0.2, 1.0)
         el = time.perf_counter() - t0
         if el >= seconds or steps >= 50:
             break

@@ -244,13 +244,29 @@ int pht_debug_splat_grads(pht_workspace* ws, float* out, void* stream);
 int pht_set_profiling(pht_workspace* ws, int enable);
 int pht_stage_times(pht_workspace* ws, float* ms_out, int max_stages);
 
-/* ---- whole training step as one hipGraph (forward + L1 + backward + fused Adam) ---- */
+/* Photometric loss of a captured step (the trainer's compute_step_loss, trainer.cpp:2024-2379):
+ * kind PHT_LOSS_L1 = pht_l1_loss with weight photometric_weight; PHT_LOSS_L1_SSIM =
+ * pht_l1_ssim_loss with ssim_weight and photometric_weight, no mask. ssim_scratch: per view,
+ * pht_l1_ssim_scratch_bytes of device memory (needed when the step sums the loss). */
+#define PHT_LOSS_L1 0
+#define PHT_LOSS_L1_SSIM 1
+typedef struct pht_loss_config {
+    int32_t kind;
+    float ssim_weight;
+    float photometric_weight;
+    int32_t reserved0;
+    void* const* ssim_scratch;
+    int64_t reserved[2];
+} pht_loss_config;
+
+/* ---- whole training step as one hipGraph (forward + loss + backward + fused Adam); loss may
+ * be NULL (L1 with weight 1) ---- */
 typedef struct pht_step_graph pht_step_graph;
 int pht_step_graph_create(pht_workspace* ws, const pht_gaussians* g, const pht_camera* cams,
                           int n_views, const pht_settings* s, pht_view_outputs* out,
                           const float* const* targets, float* const* grad_color_bufs,
-                          float* loss_sum, const pht_adam* adam, void* stream,
-                          pht_step_graph** out_graph);
+                          float* loss_sum, const pht_loss_config* loss, const pht_adam* adam,
+                          void* stream, pht_step_graph** out_graph);
 int pht_step_graph_launch(pht_step_graph* graph, void* stream);
 void pht_step_graph_destroy(pht_step_graph* graph);
 

@@ -963,16 +963,23 @@ void oracle_adam_step(float* p, const float* gr, float* m, float* v, int64_t row
 
 // One full training step on the CPU (forward + L1 + backward + fused Adam) in float: the CPU
 // baseline. Returns the loss. targets/grad buffers are per view [3,H,W].
+// One training step (forward, photometric loss, backward, fused Adam) in float: loss_kind 0 =
+// L1 with weight photometric_weight, 1 = fused L1 + SSIM. Returns the loss.
 double oracle_train_step(const pht_gaussians* g, const pht_camera* cams, int n_views, const pht_settings* s,
                          pht_view_outputs* outs, const float* const* targets, float* const* grad_bufs,
-                         const pht_adam* adam, int threads) {
+                         const pht_adam* adam, int threads, int loss_kind, float ssim_weight,
+                         float photometric_weight) {
     State<float>& st = g_state32;
     run_forward<float>(*g, cams, n_views, *s, st, outs, threads);
     double loss = 0.0;
     std::vector<pht_view_grads> dl(static_cast<size_t>(n_views));
     for (int v = 0; v < n_views; ++v) {
         const int64_t P = int64_t(cams[v].width) * cams[v].height;
-        loss += oracle_l1_loss(outs[v].color, targets[v], grad_bufs[v], P, 1.0f);
+        if (loss_kind == 1)
+            loss += l1_ssim<float>(outs[v].color, targets[v], nullptr, cams[v].width, cams[v].height, ssim_weight,
+                                   photometric_weight, grad_bufs[v]);
+        else
+            loss += oracle_l1_loss(outs[v].color, targets[v], grad_bufs[v], P, photometric_weight);
         dl[size_t(v)].grad_color = grad_bufs[v];
         dl[size_t(v)].grad_alpha = nullptr;
     }

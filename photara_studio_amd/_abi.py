@@ -101,6 +101,17 @@ class pht_counts(C.Structure):
     ]
 
 
+PHT_LOSS_L1 = 0
+PHT_LOSS_L1_SSIM = 1
+
+
+class pht_loss_config(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32), ("ssim_weight", C.c_float), ("photometric_weight", C.c_float),
+        ("reserved0", C.c_int32), ("ssim_scratch", C.POINTER(C.c_void_p)), ("reserved", C.c_int64 * 2),
+    ]
+
+
 # Every symbol the header declares, with its ctypes signature.
 SIGNATURES = {
     "pht_abi_version": (C.c_int, []),
@@ -129,7 +140,8 @@ SIGNATURES = {
     "pht_step_graph_create": (C.c_int, [C.c_void_p, C.POINTER(pht_gaussians), C.POINTER(pht_camera), C.c_int,
                                         C.POINTER(pht_settings), C.POINTER(pht_view_outputs),
                                         C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_void_p,
-                                        C.POINTER(pht_adam), C.c_void_p, C.POINTER(C.c_void_p)]),
+                                        C.POINTER(pht_loss_config), C.POINTER(pht_adam), C.c_void_p,
+                                        C.POINTER(C.c_void_p)]),
     "pht_step_graph_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
     "pht_step_graph_destroy": (None, [C.c_void_p]),
 }

@@ -499,9 +499,16 @@ struct pht_step_graph {
 
 int pht_step_graph_create(pht_workspace* ws, const pht_gaussians* g, const pht_camera* cams, int n_views,
                           const pht_settings* s, pht_view_outputs* out, const float* const* targets,
-                          float* const* grad_color_bufs, float* loss_sum, const pht_adam* adam, void* stream,
-                          pht_step_graph** out_graph) {
+                          float* const* grad_color_bufs, float* loss_sum, const pht_loss_config* loss,
+                          const pht_adam* adam, void* stream, pht_step_graph** out_graph) {
     if (!out_graph || !targets || !grad_color_bufs || !adam) return fail(PHT_EINVAL, "null argument");
+    pht_loss_config lc{};
+    lc.kind = PHT_LOSS_L1;
+    lc.photometric_weight = 1.0f;
+    if (loss) lc = *loss;
+    if (lc.kind != PHT_LOSS_L1 && lc.kind != PHT_LOSS_L1_SSIM) return fail(PHT_EINVAL, "unknown loss kind");
+    if (lc.kind == PHT_LOSS_L1_SSIM && loss_sum && !lc.ssim_scratch)
+        return fail(PHT_EINVAL, "the L1+SSIM loss sum needs per-view scratch");
     *out_graph = nullptr;
     hipStream_t cap_stream = nullptr;
     if (int r = check_hip(hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking), "capture stream")) return r;
@@ -516,7 +523,13 @@ int pht_step_graph_create(pht_workspace* ws, const pht_gaussians* g, const pht_c
         std::vector<pht_view_grads> dl(static_cast<size_t>(n_views));
         for (int v = 0; rc == PHT_OK && v < n_views; ++v) {
             const int64_t P = int64_t(cams[v].width) * cams[v].height;
-            rc = pht_l1_loss(out[v].color, targets[v], grad_color_bufs[v], loss_sum, P, 1.0f, cap_stream);
+            if (lc.kind == PHT_LOSS_L1_SSIM)
+                rc = pht_l1_ssim_loss(out[v].color, targets[v], nullptr, grad_color_bufs[v], 0, loss_sum,
+                                      loss_sum ? lc.ssim_scratch[v] : nullptr, cams[v].width, cams[v].height,
+                                      lc.ssim_weight, lc.photometric_weight, cap_stream);
+            else
+                rc = pht_l1_loss(out[v].color, targets[v], grad_color_bufs[v], loss_sum, P, lc.photometric_weight,
+                                 cap_stream);
             dl[size_t(v)].grad_color = grad_color_bufs[v];
             dl[size_t(v)].grad_alpha = nullptr;
         }

@@ -1,4 +1,4 @@
-"""Training-step driver: forward + L1 loss + backward + fused Adam over a set of views.
+"""Training-step driver: forward + photometric loss + backward + fused Adam over a set of views.
 
 Reproduces the per-iteration call sequence of the reference trainer (Trainer::train_impl,
 photara/src/splat/trainer.cpp:1922-2491: rasterizer.forward, compute_step_loss, rasterizer.backward
@@ -47,7 +47,12 @@ def to_low_precision(model: GaussianModel, dtype: torch.dtype) -> GaussianModel:
 class TrainStep:
     def __init__(self, model: GaussianModel, cameras: Sequence[Camera], targets: Sequence[torch.Tensor],
                  options: RasterizeOptions, lrs: Optional[LearningRates] = None,
-                 instance_capacity: Optional[int] = None):
+                 instance_capacity: Optional[int] = None, loss: str = "l1", ssim_weight: float = 0.2,
+                 photometric_weight: float = 1.0):
+        """loss: "l1" (loss_kernel's L1, cuda_ops.cu:886-898) or "l1_ssim" (the trainer's default
+        (1 - w) L1 + w (1 - SSIM), fused_ssim.cu:473-560, w = ssim_weight)."""
+        if loss not in ("l1", "l1_ssim"):
+            raise ValueError(f"unknown loss {loss!r}")
         validate_model(model, options)
         self.model, self.cameras, self.options = model, list(cameras), options
         self.targets = [t.contiguous() for t in targets]
@@ -78,6 +83,22 @@ class TrainStep:
         for k in range(nv):
             self.dl[k].grad_color = self.grad_colors[k].data_ptr()
         self.loss_sum = torch.zeros((1,), device=dev)
+        self.loss_kind = loss
+        self.ssim_weight, self.photometric_weight = float(ssim_weight), float(photometric_weight)
+        self.loss_cfg = _abi.pht_loss_config()
+        self.loss_cfg.kind = _abi.PHT_LOSS_L1_SSIM if loss == "l1_ssim" else _abi.PHT_LOSS_L1
+        self.loss_cfg.ssim_weight = self.ssim_weight
+        self.loss_cfg.photometric_weight = self.photometric_weight
+        self.ssim_scratch = []
+        if loss == "l1_ssim":
+            lib = _abi.load()
+            for c in self.cameras:
+                nbytes = int(lib.pht_l1_ssim_scratch_bytes(int(c.width), int(c.height)))
+                if nbytes == 0:
+                    raise ValueError("the L1+SSIM loss needs views wider and taller than 10 pixels")
+                self.ssim_scratch.append(torch.empty((nbytes // 4,), dtype=torch.float32, device=dev))
+            self._scratch_ptrs = (C.c_void_p * nv)(*[t.data_ptr() for t in self.ssim_scratch])
+            self.loss_cfg.ssim_scratch = C.cast(self._scratch_ptrs, C.POINTER(C.c_void_p))
         self.hparams = torch.zeros((3,), device=dev)
         self.step_index = 0
         self.graph = None
@@ -100,9 +121,16 @@ class TrainStep:
                                    self.outs, 0, stream))
         self.loss_sum.zero_()
         for k in range(nv):
-            check(self.lib.pht_l1_loss(C.c_void_p(self.colors[k].data_ptr()), C.c_void_p(self.targets[k].data_ptr()),
-                                       C.c_void_p(self.grad_colors[k].data_ptr()),
-                                       C.c_void_p(self.loss_sum.data_ptr()), self.colors[k][0].numel(), 1.0, stream))
+            color, target, grad = (C.c_void_p(self.colors[k].data_ptr()), C.c_void_p(self.targets[k].data_ptr()),
+                                   C.c_void_p(self.grad_colors[k].data_ptr()))
+            if self.loss_kind == "l1_ssim":
+                cam = self.cameras[k]
+                check(self.lib.pht_l1_ssim_loss(color, target, None, grad, 0, C.c_void_p(self.loss_sum.data_ptr()),
+                                                C.c_void_p(self.ssim_scratch[k].data_ptr()), int(cam.width),
+                                                int(cam.height), self.ssim_weight, self.photometric_weight, stream))
+            else:
+                check(self.lib.pht_l1_loss(color, target, grad, C.c_void_p(self.loss_sum.data_ptr()),
+                                           self.colors[k][0].numel(), self.photometric_weight, stream))
         adam = make_adam(self.model, self.adam)
         check(self.lib.pht_backward(self.ws.handle, C.byref(self.gauss), self.dl, None, C.byref(adam), stream))
 
@@ -117,8 +145,8 @@ class TrainStep:
         stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
         check(self.lib.pht_step_graph_create(self.ws.handle, C.byref(self.gauss), self.cams, nv,
                                              C.byref(self.settings), self.outs, tgt, grd,
-                                             C.c_void_p(self.loss_sum.data_ptr()), C.byref(self._adam_graph), stream,
-                                             C.byref(handle)))
+                                             C.c_void_p(self.loss_sum.data_ptr()), C.byref(self.loss_cfg),
+                                             C.byref(self._adam_graph), stream, C.byref(handle)))
         self.graph = handle
 
     def step_graph(self) -> None:

@@ -1,0 +1,99 @@
+"""The whole training step: the fused forward + loss + backward + Adam of TrainStep against the
+oracle's training step, eager against captured-graph replays, and the L1+SSIM objective.
+
+Adam's first updates are nearly lr * sign(g), so parameters are compared where the gradient is
+well above float noise (|m| > 1e-3 max|m|, m = the first moment); the moments themselves are
+compared as gradients (1e-4 relative to the tensor's max)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import small_scene, training_options
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("means", "log_scales", "quaternions", "opacity_logits", "sh")
+MOMENTS = {"means": "means", "log_scales": "scales", "quaternions": "rotations", "opacity_logits": "opacity",
+           "sh": "sh"}
+
+
+def _setup(cuda, oracle, n=3000, seed=11, loss="l1"):
+    from photara_studio_amd.scenes import model_from, ring_cameras
+    from photara_studio_amd.train import LearningRates, TrainStep
+    arrays = small_scene(n, 1.0, seed)
+    cams = ring_cameras(2, 3.5, [(160, 120)])
+    opts = training_options()
+    gt = oracle.OracleRun(oracle.Scene(small_scene(n, 1.0, seed + 1)), cams, opts, 32)
+    targets = [c.copy() for c in gt.color]
+    lrs = LearningRates.uniform(1e-3)
+    step = TrainStep(model_from(arrays, cuda), cams, [torch.from_numpy(t).to(cuda) for t in targets], opts, lrs,
+                     loss=loss)
+    return arrays, cams, opts, targets, lrs, step
+
+
+def _compare_to_oracle(step, trainer, label):
+    for k in KEYS:
+        m_gpu = getattr(step.adam, MOMENTS[k] + "_first").cpu().numpy()
+        m_orc = trainer.m[k]
+        scale = max(float(np.abs(m_orc).max()), 1e-30)
+        rel = float(np.abs(m_gpu - m_orc).max()) / scale
+        assert rel <= 1e-4, (label, k, "first moment", rel)
+        p_gpu = getattr(step.model, k).cpu().numpy()
+        p_orc = trainer.scene.a[k]
+        strong = np.abs(m_orc) > 1e-3 * scale
+        err = float(np.abs(p_gpu - p_orc)[strong].max()) if strong.any() else 0.0
+        assert err <= 1e-6, (label, k, "parameters", err)
+
+
+@pytest.mark.parametrize("loss", ["l1", "l1_ssim"])
+def test_fused_step_matches_oracle_train_step(cuda, oracle, loss):
+    arrays, cams, opts, targets, lrs, step = _setup(cuda, oracle, loss=loss)
+    trainer = oracle.OracleTrainer(oracle.Scene(arrays), cams, opts, targets, lrs, loss=loss)
+    for k in range(2):
+        print(f"[train] {loss} step {k + 1}: GPU eager vs oracle", flush=True)
+        step.step_eager()
+        want_loss = trainer.step()
+        torch.cuda.synchronize()
+        assert abs(step.loss() - want_loss) <= 1e-5 * abs(want_loss), (step.loss(), want_loss)
+        _compare_to_oracle(step, trainer, f"{loss} step {k + 1}")
+
+
+def test_graph_replays_match_eager_steps(cuda, oracle):
+    _, _, _, _, _, eager = _setup(cuda, oracle)
+    _, _, _, _, _, graph = _setup(cuda, oracle)
+    graph.capture()
+    for k in range(4):
+        print(f"[train] replay {k + 1}", flush=True)
+        eager.step_eager()
+        graph.step_graph()
+        # unrelated eager work and a synchronisation between replays
+        scratch = torch.randn(1 << 16, device=cuda)
+        scratch.mul_(2.0).sum().item()
+        torch.cuda.synchronize()
+        if k == 0:  # identical parameters: the first forward must agree bit for bit
+            for a, b in zip(eager.colors, graph.colors):
+                assert torch.equal(a, b)
+        assert abs(eager.loss() - graph.loss()) <= 1e-5 * abs(eager.loss())
+        for key in KEYS:
+            me = getattr(eager.adam, MOMENTS[key] + "_first")
+            mg = getattr(graph.adam, MOMENTS[key] + "_first")
+            rel = float((me - mg).abs().max() / me.abs().max().clamp_min(1e-30))
+            assert rel <= 1e-3, (k, key, rel)
+            strong = me.abs() > 1e-2 * me.abs().max()
+            pe, pg = getattr(eager.model, key), getattr(graph.model, key)
+            assert float((pe - pg).abs()[strong].max()) <= 1e-5, (k, key)
+
+
+def test_ssim_step_gradient_is_the_loss_gradient(cuda, oracle):
+    from photara_studio_amd.loss import l1_ssim_loss
+    _, _, _, _, _, step = _setup(cuda, oracle, loss="l1_ssim")
+    step.step_eager()
+    torch.cuda.synchronize()
+    total = 0.0
+    for c, t, g in zip(step.colors, step.targets, step.grad_colors):
+        loss, grad = l1_ssim_loss(c, t, None, 0.2, 1.0)
+        assert torch.equal(grad, g)
+        total += float(loss)
+    assert abs(total - step.loss()) <= 1e-6 * abs(total)
