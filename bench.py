@@ -29,9 +29,12 @@ METRIC = "train steps/s at 3M Gaussians 1080p, batched BF16 vs plain FP32; HBM G
 HBM_PEAK_GBPS = 8000.0
 
 
-def algorithmic_bytes(n, views, visible, instances, pixels, tiles, sh_bytes, lp):
+def algorithmic_bytes(n, views, visible, instances, pixels, tiles, sh_bytes, lp, sh_quant=False):
     """Per-stage algorithmic HBM bytes of one step (each logical array counted once per kernel
-    that must touch it; element sizes of the stored types)."""
+    that must touch it; element sizes of the stored types). The per-Gaussian backward reads and
+    writes the 59 parameters and the Adam state: FP32 moments (8 B per value) for all of them, or,
+    with the quantised SH state, for the 11 structure values plus 48 x (2 B fp16 update + 1 B code)
+    + 16 B bounds for the SH row."""
     s = sh_bytes
     sort_passes_depth = 4
     tile_bits = max(1, int(tiles - 1).bit_length())
@@ -45,7 +48,8 @@ def algorithmic_bytes(n, views, visible, instances, pixels, tiles, sh_bytes, lp)
         "blend_forward": 40 * instances + 20 * pixels,
         "loss": 36 * pixels,
         "blend_backward": 40 * instances + 24 * pixels + 44 * visible,
-        "gaussian_backward+adam": n * (59 * 4 * 2 + 59 * 4 * 4 + (52 * 2 if lp else 0)) + 44 * visible,
+        "gaussian_backward+adam": n * (59 * 4 * 2 + (11 * 8 + 48 * 3 + 16 if sh_quant else 59 * 8) * 2
+                                       + (52 * 2 if lp else 0)) + 44 * visible,
     }
     return out
 
@@ -189,7 +193,7 @@ def main():
     arrays = synthetic_torch(n, 3.0, 0, dev)
 
     # plain FP32 step: one view per step, eager launches
-    plain = TrainStep(model_from(arrays, dev), [cams[0]], [targets[0]], opts, LearningRates())
+    plain = TrainStep(model_from(arrays, dev), [cams[0]], [targets[0]], opts, LearningRates(), sh_optimizer="fp32")
     plain.ensure_capacity()
     plain_steps = args.plain_steps or 4 * args.steps
     plain_s = timed(plain.step_eager, plain_steps, args.warmup)
@@ -229,7 +233,7 @@ def main():
     pixels = sum(c.width * c.height for c in cams)
     bt = int(bopts.tile_size)
     tiles = sum(((c.width + bt - 1) // bt) * ((c.height + bt - 1) // bt) for c in cams)
-    ab = algorithmic_bytes(n, nv, int(bc.visible), int(bc.instances), pixels, tiles, 2, True)
+    ab = algorithmic_bytes(n, nv, int(bc.visible), int(bc.instances), pixels, tiles, 2, True, sh_quant=True)
     dominant = max(acc, key=acc.get)
     dom_gbps = ab[dominant] / (acc[dominant] * 1e-3) / 1e9
     step_bytes = sum(ab.values())
@@ -261,6 +265,7 @@ def main():
         "config": {"workload": f"3dgs-train-step {n} gaussians x {nv} views {res_label}, bf16 storage, hipGraph",
                    "gaussians": n, "views": nv, "resolution": res_label, "tile": int(bopts.tile_size),
                    "batch": int(bopts.batch), "storage": "bf16 sh/log_scales/opacity, fp32 masters",
+                   "sh_optimizer": "quantised (fp16 update + 8-bit log-second), fp32 structure moments",
                    "parallelism": "none (1 GPU)"},
         "autotune": [[r.tile_size, r.batch, round(r.ms_per_step, 3)] for r in tune_rows] if tune_rows else None,
         "plain_fp32": {"ms_per_step": round(plain_s * 1e3, 4), "steps_per_s": round(1.0 / plain_s, 3),

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define PHT_ABI_VERSION 1
+#define PHT_ABI_VERSION 2
 #define PHT_MAX_VIEWS 32
 
 /* Status codes. The reference throws std::invalid_argument for bad inputs
@@ -143,6 +143,16 @@ typedef struct pht_adam {
     uint16_t* sh_lp; uint16_t* log_scales_lp; uint16_t* opacity_logits_lp;
     int32_t lp_format;
     int32_t reserved;
+    /* Quantised SH Adam state, the reference trainer's SH optimizer (ShAdamQuant,
+     * photara/src/splat/sh_adam_quant.hpp:12-24; trainer.cpp:1569-1577). When sh_first is
+     * non-NULL it replaces m_sh / v_sh: sh_first fp16 [N, stride] normalised update, sh_packed
+     * uint8 [N, stride] log-second codes, sh_bounds float [N, 4] (DC min, DC max, rest min, rest
+     * max), stride = sh_bases * 3, all zero for a new model. sh_regularization is the factor of
+     * the L2 term added to the active non-DC gradients (2 * weight / (N * (stride - 3)) in
+     * rasterizer.cu:598-602; 0 for none). */
+    uint16_t* sh_first; uint8_t* sh_packed; float* sh_bounds;
+    float sh_regularization;
+    int32_t reserved1;
 } pht_adam;
 
 /* Workspace capacities. instances = capacity of the (tile, splat) instance list summed over views. */
@@ -221,6 +231,17 @@ int pht_adam_step(float* param, const float* grad, float* m, float* v, int64_t r
                   int64_t secondary_from_col, float beta1, float beta2, float correction1,
                   float correction2, float epsilon, float clamp_min, float clamp_max,
                   int32_t prefix_semantics, void* stream);
+
+/* ---- standalone quantised SH Adam (sh_adam_quant_step, photara/src/splat/sh_adam_quant.hpp:34-49,
+ * kernel sh_adam_quant_warp_row, sh_adam_quant.cuh:146-234): rows x stride coefficients, columns
+ * [0, active) updated (lr for columns 0-2, rest_lr after), columns [active, stride) re-encoded
+ * under the new bounds with their parameters untouched; regularization is the factor added times
+ * the parameter to active non-DC gradients (0 when the gradient already holds it). first is fp16
+ * bits, packed uint8, bounds float [rows, 4]; stride in [1, 48]. ---- */
+int pht_sh_adam_quant_step(float* parameter, const float* gradient, uint16_t* first, uint8_t* packed,
+                           float* bounds, int64_t rows, int32_t stride, int32_t active, float lr, float rest_lr,
+                           float beta1, float beta2, float correction1, float correction2, float epsilon,
+                           float regularization, void* stream);
 
 /* ---- counters and debug views of the last forward ---- */
 int pht_get_counts(pht_workspace* ws, pht_counts* out, void* stream); /* synchronises */

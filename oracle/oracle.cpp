@@ -515,6 +515,38 @@ void write_grads(const pht_gaussians& g, const pht_settings& s, int64_t i, const
     }
 }
 
+// One row of the quantised SH Adam (sh_adam_quant_warp_row,
+// third_party/splat_drender/include/splat_drender/sh_adam_quant.cuh:146-234), walked column by
+// column as the reference's own CPU check does (photara/tests/splat_test.cpp:1334-1434).
+inline void shq_row(const ShqStep& q, int stride, float* p, const float* grad, uint16_t* first, uint8_t* packed,
+                    float* bounds) {
+    float ls[64];
+    const ShqConst kc = shq_const(q);
+    for (int k = 0; k < stride; ++k) {
+        float lo, hi;
+        shq_bounds(bounds, k, lo, hi);
+        float u = fp16_to_f32(first[k]);
+        float pv = p[k];
+        const bool act = k < q.active;
+        ls[k] = shq_column(kc, k, pv, act ? grad[k] : 0.0f, u, shq_decode(packed[k], lo, hi));
+        if (act) p[k] = pv;
+        first[k] = f32_to_fp16(u);
+    }
+    float b[4] = {1e30f, -1e30f, 1e30f, -1e30f};
+    for (int k = 0; k < stride; ++k) {
+        const int o = k < 3 ? 0 : 2;
+        b[o] = fminf(b[o], ls[k]);
+        b[o + 1] = fmaxf(b[o + 1], ls[k]);
+    }
+    if (stride <= 3) b[2] = b[3] = 0.0f;
+    for (int o = 0; o < 4; ++o) bounds[o] = b[o];
+    const float sc[2] = {shq_scale(b[0], b[1]), shq_scale(b[2], b[3])};
+    for (int k = 0; k < stride; ++k) {
+        const int g = k < 3 ? 0 : 1;
+        packed[k] = uint8_t(shq_encode(ls[k], b[2 * g], sc[g]));
+    }
+}
+
 // One fused Adam row over all parameters (structure + active-prefix SH).
 inline void adam_row(const pht_adam& a, const pht_settings& s, int64_t i, const float gm[3],
                      const float gls[3], const float gq[4], float glog, const float* gsh) {
@@ -533,7 +565,14 @@ inline void adam_row(const pht_adam& a, const pht_settings& s, int64_t i, const 
     }
     adam_update(a.opacity_logits[i], a.m_opacity_logits[i], a.v_opacity_logits[i], glog, a.lr_opacity,
                 a.beta1, a.beta2, a.correction1, a.correction2, a.epsilon, -12.0f, 12.0f, false);
-    if (a.sh) {
+    if (a.sh && a.sh_first) {
+        const int active = sh_count(s.active_sh_degree) * 3;
+        const int stride = s.sh_bases * 3;
+        const int64_t o = i * int64_t(stride);
+        shq_row(ShqStep{a.lr_sh0, a.lr_sh_rest, a.beta1, a.beta2, a.correction1, a.correction2, a.epsilon,
+                        a.sh_regularization, active},
+                stride, a.sh + o, gsh, a.sh_first + o, a.sh_packed + o, a.sh_bounds + 4 * i);
+    } else if (a.sh) {
         const int active = sh_count(s.active_sh_degree) * 3;
         for (int c = 0; c < active; ++c) {
             const int64_t j = i * int64_t(s.sh_bases) * 3 + c;
@@ -707,6 +746,12 @@ void oracle_pexpf_inrange(const float* x, float* y, int64_t n) {
 }
 void oracle_plogf(const float* x, float* y, int64_t n) {
     for (int64_t i = 0; i < n; ++i) y[i] = plogf(x[i]);
+}
+void oracle_plog1pf(const float* x, float* y, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) y[i] = plog1pf(x[i]);
+}
+void oracle_pexpm1f(const float* x, float* y, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) y[i] = pexpm1f(x[i]);
 }
 void oracle_gaussian_power(const float* conic3, const float* d2, float* out, int64_t n) {
     for (int64_t i = 0; i < n; ++i)
@@ -959,6 +1004,16 @@ void oracle_adam_step(float* p, const float* gr, float* m, float* v, int64_t row
             const float lrx = (secondary_from > 0 && c >= secondary_from) ? lr2 : lr;
             adam_update(p[j], m[j], v[j], gr[j], lrx, b1, b2, c1, c2, eps, lo, hi, prefix != 0);
         }
+}
+
+// Standalone quantised SH Adam over rows x stride (see pht_sh_adam_quant_step).
+void oracle_sh_adam_quant_step(float* p, const float* gr, uint16_t* first, uint8_t* packed, float* bounds,
+                               int64_t rows, int stride, int active, float lr, float rest_lr, float b1, float b2,
+                               float c1, float c2, float eps, float regularization) {
+    const ShqStep q{lr, rest_lr, b1, b2, c1, c2, eps, regularization, active};
+    for (int64_t r = 0; r < rows; ++r)
+        shq_row(q, stride, p + r * stride, gr ? gr + r * stride : nullptr, first + r * stride, packed + r * stride,
+                bounds + 4 * r);
 }
 
 // One full training step on the CPU (forward + L1 + backward + fused Adam) in float: the CPU

@@ -31,6 +31,11 @@ def load() -> C.CDLL:
         lib.oracle_l1_ssim.argtypes = [C.c_int, vp, vp, vp, C.c_int, C.c_int, C.c_float, C.c_float, vp]
         lib.oracle_l1_ssim.restype = C.c_double
         lib.oracle_pexpf_inrange.argtypes = [vp, vp, C.c_int64]
+        lib.oracle_plog1pf.argtypes = [vp, vp, C.c_int64]
+        lib.oracle_pexpm1f.argtypes = [vp, vp, C.c_int64]
+        lib.oracle_sh_adam_quant_step.argtypes = [vp, vp, vp, vp, vp, C.c_int64, C.c_int, C.c_int, C.c_float,
+                                                  C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
+                                                  C.c_float]
         lib.oracle_plogf.argtypes = [vp, vp, C.c_int64]
         lib.oracle_gaussian_power.argtypes = [vp, vp, vp, C.c_int64]
         lib.oracle_lp_roundtrip.argtypes = [vp, vp, vp, C.c_int64, C.c_int]
@@ -287,6 +292,46 @@ def plogf(x: np.ndarray) -> np.ndarray:
     return y
 
 
+def plog1pf(x: np.ndarray) -> np.ndarray:
+    x = f32(x)
+    y = np.empty_like(x)
+    load().oracle_plog1pf(_p(x), _p(y), x.size)
+    return y
+
+
+def pexpm1f(x: np.ndarray) -> np.ndarray:
+    x = f32(x)
+    y = np.empty_like(x)
+    load().oracle_pexpm1f(_p(x), _p(y), x.size)
+    return y
+
+
+class ShQuantState:
+    """Quantised SH Adam state on the host: first (fp16 bits as uint16) and packed codes
+    [rows, stride], bounds float32 [rows, 4]."""
+
+    def __init__(self, rows: int, stride: int):
+        self.stride = stride
+        self.first = np.zeros((rows, stride), np.uint16)
+        self.packed = np.zeros((rows, stride), np.uint8)
+        self.bounds = np.zeros((rows, 4), np.float32)
+
+    def first_f32(self) -> np.ndarray:
+        return self.first.view(np.float16).astype(np.float32)
+
+
+def sh_adam_quant_step(p: np.ndarray, g: Optional[np.ndarray], state: ShQuantState, active: int, lr: float,
+                       rest_lr: float, b1: float, b2: float, c1: float, c2: float, eps: float,
+                       regularization: float = 0.0) -> None:
+    """The oracle's quantised SH Adam step, in place (the same contract as pht_sh_adam_quant_step)."""
+    assert p.dtype == np.float32 and p.flags.c_contiguous and p.size == state.first.size
+    if g is not None:
+        assert g.dtype == np.float32 and g.flags.c_contiguous and g.size == p.size
+    rows = state.first.shape[0]
+    load().oracle_sh_adam_quant_step(_p(p), _p(g), _p(state.first), _p(state.packed), _p(state.bounds), rows,
+                                     state.stride, active, lr, rest_lr, b1, b2, c1, c2, eps, regularization)
+
+
 def gaussian_power(conic3: np.ndarray, d2: np.ndarray) -> np.ndarray:
     conic3, d2 = f32(conic3), f32(d2)
     out = np.empty((conic3.shape[0],), np.float32)
@@ -341,7 +386,7 @@ class OracleTrainer:
 
     def __init__(self, scene: Scene, cameras: Sequence[Camera], options: RasterizeOptions, targets, lrs,
                  loss: str = "l1", ssim_weight: float = 0.2, photometric_weight: float = 1.0,
-                 threads: Optional[int] = None):
+                 threads: Optional[int] = None, sh_optimizer: str = "fp32", sh_regularization: float = 0.0):
         self.scene, self.cameras, self.options = scene, list(cameras), options
         self.lrs, self.threads = lrs, threads or threads_default()
         self.loss_kind = 1 if loss == "l1_ssim" else 0
@@ -361,6 +406,11 @@ class OracleTrainer:
         keys = ("means", "log_scales", "quaternions", "opacity_logits", "sh")
         self.m = {k: np.zeros_like(scene.a[k]) for k in keys}
         self.v = {k: np.zeros_like(scene.a[k]) for k in keys}
+        self.sh_state = None
+        if sh_optimizer == "quant":
+            n, bases = scene.a["sh"].shape[0], scene.a["sh"].shape[1]
+            self.sh_state = ShQuantState(n, bases * 3)
+        self.sh_regularization = float(sh_regularization)
         self.step_index = 0
 
     def step(self) -> float:
@@ -377,6 +427,9 @@ class OracleTrainer:
         a.m_opacity_logits = self.m["opacity_logits"].ctypes.data
         a.v_opacity_logits = self.v["opacity_logits"].ctypes.data
         a.m_sh, a.v_sh = self.m["sh"].ctypes.data, self.v["sh"].ctypes.data
+        if self.sh_state is not None:
+            a.sh_first, a.sh_packed = self.sh_state.first.ctypes.data, self.sh_state.packed.ctypes.data
+            a.sh_bounds, a.sh_regularization = self.sh_state.bounds.ctypes.data, self.sh_regularization
         l = self.lrs
         a.lr_means, a.lr_scales, a.lr_quaternions = l.means, l.scales, l.quaternions
         a.lr_opacity, a.lr_sh0, a.lr_sh_rest = l.opacities, l.sh0, l.sh_rest

@@ -84,6 +84,8 @@ class pht_adam(C.Structure):
         ("device_hparams", C.c_void_p),
         ("sh_lp", C.c_void_p), ("log_scales_lp", C.c_void_p), ("opacity_logits_lp", C.c_void_p),
         ("lp_format", C.c_int32), ("reserved", C.c_int32),
+        ("sh_first", C.c_void_p), ("sh_packed", C.c_void_p), ("sh_bounds", C.c_void_p),
+        ("sh_regularization", C.c_float), ("reserved1", C.c_int32),
     ]
 
 
@@ -131,6 +133,9 @@ SIGNATURES = {
     "pht_adam_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_int64,
                                 C.c_float, C.c_float, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_float,
                                 C.c_float, C.c_float, C.c_float, C.c_int32, C.c_void_p]),
+    "pht_sh_adam_quant_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                         C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_float, C.c_float,
+                                         C.c_float, C.c_float, C.c_float, C.c_float, C.c_void_p]),
     "pht_get_counts": (C.c_int, [C.c_void_p, C.POINTER(pht_counts), C.c_void_p]),
     "pht_debug_lists": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                   C.c_void_p]),

@@ -304,6 +304,7 @@ int pht_forward(pht_workspace* ws, const pht_gaussians* g, const pht_camera* cam
     ba.sgrad = ws->sgrad;
     ba.counters = ws->counters;
     ba.slot_cap = ws->slot_cap;
+    ba.n_gauss = g->count;
     ws->mark(4, 0, st);
     launch_blend_forward(ba, int(tiles), ts, s->batch, st);
     ws->mark(4, 1, st);
@@ -342,7 +343,10 @@ int pht_backward(pht_workspace* ws, const pht_gaussians* g, const pht_view_grads
             !adam->v_means || !adam->m_log_scales || !adam->v_log_scales || !adam->m_quaternions ||
             !adam->v_quaternions || !adam->m_opacity_logits || !adam->v_opacity_logits)
             return fail(PHT_EINVAL, "fused Adam requires every structure parameter and moment");
-        if (adam->sh && (!adam->m_sh || !adam->v_sh)) return fail(PHT_EINVAL, "SH Adam requires its moments");
+        if (adam->sh && !adam->sh_first && (!adam->m_sh || !adam->v_sh))
+            return fail(PHT_EINVAL, "SH Adam requires its moments");
+        if (adam->sh && adam->sh_first && (!adam->sh_packed || !adam->sh_bounds))
+            return fail(PHT_EINVAL, "quantised SH Adam requires sh_first, sh_packed and sh_bounds");
         if (adam->lp_format != g->lp_format) return fail(PHT_EINVAL, "Adam lp_format differs from the model's");
     }
     hipStream_t st = static_cast<hipStream_t>(stream);
@@ -366,6 +370,7 @@ int pht_backward(pht_workspace* ws, const pht_gaussians* g, const pht_view_grads
     ba.sgrad = ws->sgrad;
     ba.counters = ws->counters;
     ba.slot_cap = ws->slot_cap;
+    ba.n_gauss = ws->gaussians;
     ws->mark(5, 0, st);
     launch_blend_backward(ba, int(ws->tiles_total), ws->settings.tile_size, ws->settings.batch, st);
     ws->mark(5, 1, st);
@@ -443,6 +448,20 @@ int pht_adam_step(float* param, const float* grad, float* m, float* v, int64_t r
                            beta1, beta2, correction1, correction2, epsilon, clamp_min, clamp_max, prefix_semantics,
                            static_cast<hipStream_t>(stream));
     return check_hip(hipGetLastError(), "adam launch");
+}
+
+int pht_sh_adam_quant_step(float* parameter, const float* gradient, uint16_t* first, uint8_t* packed, float* bounds,
+                           int64_t rows, int32_t stride, int32_t active, float lr, float rest_lr, float beta1,
+                           float beta2, float correction1, float correction2, float epsilon, float regularization,
+                           void* stream) {
+    if (rows < 0 || stride < 1 || stride > kShqMaxStride || active < 0 || active > stride)
+        return fail(PHT_EINVAL, "quantised SH Adam needs rows >= 0, stride in [1, 48] and active in [0, stride]");
+    if (rows > 0 && (!parameter || !first || !packed || !bounds || (active > 0 && !gradient)))
+        return fail(PHT_EINVAL, "null quantised SH Adam array");
+    launch_sh_adam_quant(parameter, gradient, first, packed, bounds, rows, stride,
+                         ShqStep{lr, rest_lr, beta1, beta2, correction1, correction2, epsilon, regularization, active},
+                         static_cast<hipStream_t>(stream));
+    return check_hip(hipGetLastError(), "quantised SH Adam launch");
 }
 
 int pht_get_counts(pht_workspace* ws, pht_counts* out, void* stream) {

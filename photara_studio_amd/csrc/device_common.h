@@ -40,8 +40,13 @@ struct SplatRec {
 };
 // Slot metadata (16 bytes, read in slot order): (0, depth_key, tiles, flags),
 // flags: bits 0-2 SH clamp, bit 3 fp64 conic.
-// Splat gradient (12 floats): dmean.xy, dconic.xyz, dopacity, dcolor.rgb, refine, 2 spare.
+// Splat gradient (12 floats): dmean.xy, dconic.xyz, dopacity, dcolor.rgb, refine, 2 spare. The
+// records are stored view-major: slot g * vp + v owns record v * n + g (n Gaussians), so a wave
+// of consecutive Gaussians reads one view's records contiguously.
 constexpr int kGradStride = 12;
+__device__ __forceinline__ int64_t grad_offset(int64_t slot, int vp_log2, int64_t n) {
+    return ((slot & ((int64_t(1) << vp_log2) - 1)) * n + (slot >> vp_log2)) * kGradStride;
+}
 
 // Fixed block count of the chunked scans / radix-sort passes (each block owns one contiguous
 // chunk of the list, so the passes are stable and need no device-wide synchronisation).

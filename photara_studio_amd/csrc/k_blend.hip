@@ -146,30 +146,25 @@ __global__ __launch_bounds__(64) void k_blend_fwd(BlendArgs a) {
             if (m == 0) continue;
             const float4 A = sA[j], B = sB[j];
             const float cb = sC[j];
-            // all quadrants' pair tests first: four independent exp chains in flight; the
-            // per-pixel updates below are selects, so the arithmetic of every blended pair is
-            // exactly the sequential one
-            float alpha[Q], G[Q];
-            bool ok[Q];
-#pragma unroll
-            for (int k = 0; k < Q; ++k)
-                ok[k] = pair_alpha_flat(A.z, A.w, B.x, B.y, A.x - pxf[k], A.y - pyf[k], clip, alpha[k], G[k]) &&
-                        ((m >> k) & 1u) && !done[k];
             bool blended = false;
-            const uint32_t n_after = base - range.x + j + 1;
 #pragma unroll
             for (int k = 0; k < Q; ++k) {
-                const float test_t = T[k] * (1.0f - alpha[k]);
-                const bool stop = ok[k] && test_t < t_floor;
-                const bool blend = ok[k] && !stop;
-                const float a_t = alpha[k] * T[k];
-                C0[k] = blend ? fmaf(B.z, a_t, C0[k]) : C0[k];
-                C1[k] = blend ? fmaf(B.w, a_t, C1[k]) : C1[k];
-                C2[k] = blend ? fmaf(cb, a_t, C2[k]) : C2[k];
-                T[k] = blend ? test_t : T[k];
-                last[k] = blend ? n_after : last[k];
-                done[k] = done[k] || stop;
-                blended = blended || blend;
+                if (!((m >> k) & 1u) || done[k]) continue;
+                float alpha, G;
+                if (pair_alpha_flat(A.z, A.w, B.x, B.y, A.x - pxf[k], A.y - pyf[k], clip, alpha, G)) {
+                    const float test_t = T[k] * (1.0f - alpha);
+                    if (test_t < t_floor) {
+                        done[k] = true;
+                    } else {
+                        const float a_t = alpha * T[k];
+                        C0[k] = fmaf(B.z, a_t, C0[k]);
+                        C1[k] = fmaf(B.w, a_t, C1[k]);
+                        C2[k] = fmaf(cb, a_t, C2[k]);
+                        T[k] = test_t;
+                        last[k] = base - range.x + j + 1;
+                        blended = true;
+                    }
+                }
             }
             if (__ballot(blended) != 0 && lane == 0) sV[j] = 1u;
         }
@@ -290,24 +285,17 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendArgs a) {
             float r[NG];
 #pragma unroll
             for (int g = 0; g < NG; ++g) r[g] = 0.0f;
-            // all quadrants' pair tests first (independent exp chains), then the gradient terms
-            // of the quadrants with a contributing pixel
-            float dxq[Q], dyq[Q], alq[Q], Gq[Q];
-            bool cq[Q];
-#pragma unroll
-            for (int k = 0; k < Q; ++k) {
-                dxq[k] = A.x - pxf[k];
-                dyq[k] = A.y - pyf[k];
-                cq[k] = pair_alpha_flat(A.z, A.w, B.x, B.y, dxq[k], dyq[k], clip, alq[k], Gq[k]) &&
-                        idx < last[k] && ((m >> k) & 1u);
-            }
             bool any = false;
 #pragma unroll
             for (int k = 0; k < Q; ++k) {
-                if (__ballot(cq[k]) == 0) continue;
+                if (!((m >> k) & 1u)) continue;
+                const float dx = A.x - pxf[k], dy = A.y - pyf[k];
+                float alpha = 0.0f, G = 0.0f;
+                const bool contrib =
+                    pair_alpha_flat(A.z, A.w, B.x, B.y, dx, dy, clip, alpha, G) && idx < last[k];
+                if (__ballot(contrib) == 0) continue;
                 any = true;
-                const float dx = dxq[k], dy = dyq[k], alpha = alq[k], G = Gq[k];
-                if (cq[k]) {
+                if (contrib) {
                     // the recovered T and the gradients are not compared bit for bit, so the
                     // hardware reciprocal / square root (1 ulp) replace the IEEE sequences
                     const float inv_1ma = __builtin_amdgcn_rcpf(1.0f - alpha);
@@ -338,7 +326,7 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendArgs a) {
             if (!any) continue;
             wave_sum10_to_last(r);
             if (lane == 63) {
-                float* dst = a.sgrad + size_t(sS[j]) * kGradStride;
+                float* dst = a.sgrad + grad_offset(sS[j], a.vp_log2, a.n_gauss);
 #pragma unroll
                 for (int g = 0; g < NG; ++g)
                     if (r[g] != 0.0f) atomic_add_f32(dst + g, r[g]);

@@ -42,6 +42,76 @@ __device__ __forceinline__ void for_block_rows(int rows, int stride, int active,
     }
 }
 
+// Quantised SH Adam over the block's `rows` consecutive rows starting at row_begin
+// (sh_adam_quant_warp_row, third_party/splat_drender/include/splat_drender/sh_adam_quant.cuh:146-234,
+// with the per-coefficient step of shq_column). On entry s_row holds each row's gradient in
+// columns [0, q.step.active) (pitch kShPitch); its contents are consumed. Two passes:
+// (1) cooperative and coalesced over the block's contiguous coefficient range: decode under the
+//     row's old bounds, update, write parameter, fp16 update (and the reduced-precision mirror),
+//     leave log_s in LDS;
+// (2) one thread per row: the DC and non-DC (min, max) of its log_s, the new bounds, and the
+//     row's codes, packed four to a word when the row length allows.
+struct ShqArgs {
+    float* param;
+    uint16_t* first;
+    uint8_t* packed;
+    float* bounds;
+    uint16_t* lp;
+    ShqStep step;
+};
+
+template <int LP>
+__device__ void shq_block(const ShqArgs& q, int64_t row_begin, int rows, int stride, float* s_row) {
+    const int64_t e0 = row_begin * stride;
+    const ShqConst kc = shq_const(q.step);
+    for_block_rows(rows, stride, stride, [&](int e, int row, int k) {
+        float lo, hi;
+        shq_bounds(q.bounds + (row_begin + row) * 4, k, lo, hi);
+        float p = q.param[e0 + e];
+        float u = fp16_to_f32(q.first[e0 + e]);
+        const bool act = k < q.step.active;
+        const float ls = shq_column(kc, k, p, act ? s_row[row * kShPitch + k] : 0.0f, u,
+                                    shq_decode(q.packed[e0 + e], lo, hi));
+        if (act) {
+            q.param[e0 + e] = p;
+            if constexpr (LP != PHT_LP_NONE) {
+                if (q.lp) q.lp[e0 + e] = f32_to_lp(p, LP);
+            }
+        }
+        q.first[e0 + e] = f32_to_fp16(u);
+        s_row[row * kShPitch + k] = ls;
+    });
+    __syncthreads();
+    if (int(threadIdx.x) >= rows) return;
+    const float* r = s_row + threadIdx.x * kShPitch;
+    float b[4] = {1e30f, -1e30f, 1e30f, -1e30f};
+    for (int k = 0; k < stride; ++k) {
+        const int o = k < 3 ? 0 : 2;
+        b[o] = fminf(b[o], r[k]);
+        b[o + 1] = fmaxf(b[o + 1], r[k]);
+    }
+    if (stride <= 3) b[2] = b[3] = 0.0f;
+    const int64_t row = row_begin + threadIdx.x;
+    for (int o = 0; o < 4; ++o) q.bounds[row * 4 + o] = b[o];
+    const float sc[2] = {shq_scale(b[0], b[1]), shq_scale(b[2], b[3])};
+    uint8_t* dst = q.packed + row * stride;
+    if ((stride & 3) == 0 && (reinterpret_cast<uintptr_t>(q.packed) & 3u) == 0) {
+        for (int k = 0; k < stride; k += 4) {
+            uint32_t w = 0;
+            for (int j = 0; j < 4; ++j) {
+                const int g = k + j < 3 ? 0 : 1;
+                w |= shq_encode(r[k + j], b[2 * g], sc[g]) << (8 * j);
+            }
+            *reinterpret_cast<uint32_t*>(dst + k) = w;
+        }
+    } else {
+        for (int k = 0; k < stride; ++k) {
+            const int g = k < 3 ? 0 : 1;
+            dst[k] = uint8_t(shq_encode(r[k], b[2 * g], sc[g]));
+        }
+    }
+}
+
 template <int DEG, int LP, bool ADAM>
 __global__ __launch_bounds__(256) void k_gaussian_bwd(GaussArgs a) {
     constexpr int NB = (DEG + 1) * (DEG + 1);
@@ -52,6 +122,7 @@ __global__ __launch_bounds__(256) void k_gaussian_bwd(GaussArgs a) {
     const int64_t i0 = int64_t(blockIdx.x) * kScanThreads;
     const int64_t i = i0 + threadIdx.x;
     const bool live = i < a.g.count;
+    const int64_t n = a.g.count;
     const int rows = int(a.g.count - i0 < kScanThreads ? a.g.count - i0 : kScanThreads);
     const int stride = a.s.sh_bases * 3;
     const int64_t row0 = i0 * int64_t(stride);
@@ -91,8 +162,7 @@ __global__ __launch_bounds__(256) void k_gaussian_bwd(GaussArgs a) {
             const Activated<float> act = activate<float>(ls, q, logit, hf, filt);
             for (int v = 0; v < a.n_views; ++v) {
                 if (!((mask >> v) & 1u)) continue;
-                const int64_t slot = (i << a.vp_log2) + v;
-                const float4* gp = reinterpret_cast<const float4*>(a.sgrad + slot * kGradStride);
+                const float4* gp = reinterpret_cast<const float4*>(a.sgrad + (v * n + i) * kGradStride);
                 const float4 g0 = gp[0], g1 = gp[1], g2 = gp[2];
                 const float dconic[4] = {g0.z, g0.w, g1.x, g1.y};
                 const float dm2[2] = {g0.x, g0.y};
@@ -104,7 +174,7 @@ __global__ __launch_bounds__(256) void k_gaussian_bwd(GaussArgs a) {
                 } else {
                     float d[3], raw[3];
                     view_dir(mean, cam.position, d, raw);
-                    sh_backward_dir(DEG, raw, d, my_sh, a.meta[slot].w & 7u, dcol, sg.mean);
+                    sh_backward_dir(DEG, raw, d, my_sh, a.meta[(i << a.vp_log2) + v].w & 7u, dcol, sg.mean);
                 }
                 refine += g2.y;
             }
@@ -114,12 +184,12 @@ __global__ __launch_bounds__(256) void k_gaussian_bwd(GaussArgs a) {
         if (!precomp) {
             for (int v = 0; v < a.n_views; ++v) {
                 if (!((mask >> v) & 1u)) continue;
-                const int64_t slot = (i << a.vp_log2) + v;
-                const float* gp = a.sgrad + slot * kGradStride;
-                const float dcol[3] = {gp[6], gp[7], gp[8]};
+                const float4* gp = reinterpret_cast<const float4*>(a.sgrad + (v * n + i) * kGradStride);
+                const float4 g1 = gp[1], g2 = gp[2];
+                const float dcol[3] = {g1.z, g1.w, g2.x};
                 float d[3], raw[3];
                 view_dir(mean, a.views[v].vi.cam.position, d, raw);
-                sh_backward_coef(DEG, d, a.meta[slot].w & 7u, dcol, gsh);
+                sh_backward_coef(DEG, d, a.meta[(i << a.vp_log2) + v].w & 7u, dcol, gsh);
             }
         }
     }
@@ -177,7 +247,16 @@ __global__ __launch_bounds__(256) void k_gaussian_bwd(GaussArgs a) {
                 if (A.opacity_logits_lp) A.opacity_logits_lp[i] = f32_to_lp(A.opacity_logits[i], LP);
             }
         }
-        if (A.sh && use_sh) {
+        if (A.sh && use_sh && A.sh_first) {
+            ShqArgs qa;
+            qa.param = A.sh;
+            qa.first = A.sh_first;
+            qa.packed = A.sh_packed;
+            qa.bounds = A.sh_bounds;
+            qa.lp = LP != PHT_LP_NONE ? A.sh_lp : nullptr;
+            qa.step = ShqStep{A.lr_sh0, A.lr_sh_rest, A.beta1, A.beta2, c1, c2, A.epsilon, A.sh_regularization, NB * 3};
+            shq_block<LP>(qa, i0, rows, stride, s_sh);
+        } else if (A.sh && use_sh) {
             float* P = A.sh + row0;
             float* M = A.m_sh + row0;
             float* V = A.v_sh + row0;
@@ -215,6 +294,35 @@ void launch_gaussian_backward(const GaussArgs& a, bool adam, hipStream_t st) {
     if (a.g.lp_format == PHT_LP_BF16) gb_deg<PHT_LP_BF16>(a, adam, st);
     else if (a.g.lp_format == PHT_LP_FP16) gb_deg<PHT_LP_FP16>(a, adam, st);
     else gb_deg<PHT_LP_NONE>(a, adam, st);
+}
+
+// ---------------------------------------------------------------- standalone quantised SH Adam
+// sh_adam_quant_step's kernel (photara/src/splat/sh_adam_quant.cu): 256 rows per block, the
+// active gradients staged into LDS with one coalesced pass, then shq_block.
+__global__ __launch_bounds__(256) void k_sh_adam_quant(float* param, const float* grad, uint16_t* first,
+                                                       uint8_t* packed, float* bounds, int64_t n_rows, int stride,
+                                                       ShqStep step) {
+    __shared__ float s_row[kScanThreads * kShPitch];
+    const int64_t r0 = int64_t(blockIdx.x) * kScanThreads;
+    const int rows = int(n_rows - r0 < kScanThreads ? n_rows - r0 : kScanThreads);
+    const float* g = grad ? grad + r0 * stride : nullptr;
+    if (g) for_block_rows(rows, stride, step.active, [&](int e, int row, int k) { s_row[row * kShPitch + k] = g[e]; });
+    __syncthreads();
+    ShqArgs q;
+    q.param = param;
+    q.first = first;
+    q.packed = packed;
+    q.bounds = bounds;
+    q.lp = nullptr;
+    q.step = step;
+    shq_block<PHT_LP_NONE>(q, r0, rows, stride, s_row);
+}
+
+void launch_sh_adam_quant(float* param, const float* grad, uint16_t* first, uint8_t* packed, float* bounds,
+                          int64_t rows, int stride, const ShqStep& step, hipStream_t st) {
+    if (rows <= 0) return;
+    hipLaunchKernelGGL(k_sh_adam_quant, dim3(unsigned((rows + kScanThreads - 1) / kScanThreads)), dim3(kScanThreads),
+                       0, st, param, grad, first, packed, bounds, rows, stride, step);
 }
 
 // ---------------------------------------------------------------- standalone Adam

@@ -131,7 +131,7 @@ __global__ __launch_bounds__(256) void k_preprocess(PreArgs a, int64_t n_slots) 
                 r.c = make_float4(rgb[2], ex, ey, 0.0f);
                 r.d = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 a.meta[t] = make_uint4(0u, f2u(p.depth), touched, clamp | (fp64 ? 8u : 0u));
-                float4* sg = reinterpret_cast<float4*>(a.sgrad + t * kGradStride);
+                float4* sg = reinterpret_cast<float4*>(a.sgrad + grad_offset(t, a.vp_log2, a.g.count));
                 sg[0] = make_float4(0.f, 0.f, 0.f, 0.f);
                 sg[1] = make_float4(0.f, 0.f, 0.f, 0.f);
                 sg[2] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -319,7 +319,7 @@ __global__ __launch_bounds__(256) void k_debug_splat_grads(const float* sgrad, c
     const int64_t g = t / n_views;
     const int v = int(t % n_views);
     const bool vis = (view_mask[g] >> v) & 1u;
-    const float* src = sgrad + ((g << vp_log2) + v) * kGradStride;
+    const float* src = sgrad + (int64_t(v) * n + g) * kGradStride;
     for (int k = 0; k < 10; ++k) out[t * 10 + k] = vis ? src[k] : 0.0f;
 }
 

@@ -33,6 +33,7 @@ struct BlendArgs {
     float* sgrad;
     const Counters* counters;
     uint32_t slot_cap;
+    int64_t n_gauss;
 };
 
 struct GaussArgs {
@@ -48,6 +49,11 @@ struct GaussArgs {
     pht_model_grads out;
     pht_adam adam;
 };
+
+// k_gaussian.hip: standalone quantised SH Adam over rows x stride (stride <= kShqMaxStride).
+constexpr int kShqMaxStride = 48;
+void launch_sh_adam_quant(float* param, const float* grad, uint16_t* first, uint8_t* packed, float* bounds,
+                          int64_t rows, int stride, const ShqStep& step, hipStream_t st);
 
 // k_sort.hip
 int scan_blocks_for(uint32_t capacity);

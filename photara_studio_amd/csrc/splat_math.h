@@ -857,6 +857,134 @@ PHT_HD void adam_update(float& p, float& m, float& v, float g, float lr, float b
     p = prefix_semantics ? upd : fminf(fmaxf(upd, lo), hi);
 }
 
+// log(1 + x): log(1 + x) directly for x >= 1; below, with u = fl(1 + x),
+// log1p(x) = log(u) * x / (u - 1), exact for u == 1 (Goldberg, "What every computer scientist
+// should know about floating-point arithmetic", 1991, thm 4).
+PHT_HD float plog1pf(float x) {
+    if (x != x || x == u2f(0x7f800000u)) return x;
+    if (x >= 1.0f) return plogf(1.0f + x); // the rounding of 1 + x moves log by < 2^-24 / ln 2 ulp
+    const float u = 1.0f + x;
+    if (u == 1.0f) return x;
+    return plogf(u) * (x / (u - 1.0f));
+}
+
+// exp(x) - 1: e^x - 1 for |x| >= 1/2; below, with u = fl(exp(x)), expm1(x) = (u - 1) * x / log(u)
+// (Kahan's correction; the rounding of u cancels between numerator and denominator), x itself
+// when u == 1.
+PHT_HD float pexpm1f(float x) {
+    if (x != x) return x;
+    const float u = pexpf(x);
+    if (!(fabsf(x) < 0.5f)) return u - 1.0f; // |e^x - 1| >= 0.39 |e^x| there: no cancellation
+    if (u == 1.0f) return x;
+    const float um1 = u - 1.0f;
+    if (um1 == -1.0f) return -1.0f;
+    if (!finite_(u)) return u;
+    return um1 * (x / plogf(u));
+}
+
+// ---------------------------------------------------------------- quantised SH Adam state
+// The reference trainer's SH optimizer state (sh_adam_quant_column / sh_adam_quant_warp_row,
+// third_party/splat_drender/include/splat_drender/sh_adam_quant.cuh:25-234). Per coefficient it
+// keeps the normalised Adam update u = (m / c1) / (sqrt(v / c2) + eps) in fp16 and an 8-bit code
+// of log_s = log1p(sqrt(v) / 1e-15), linear between the row's bounds; DC columns [0, 3) and the
+// rest [3, stride) each own a (min, max) pair. The raw moments are recovered from u with the
+// previous step's corrections (c_{t-1} = (c_t - (1 - beta)) / beta). The codec constants and
+// steps are a storage format, kept as the reference has them.
+constexpr float kShqEps = 1e-15f;
+constexpr float kShqInvEps = 1.0f / kShqEps;
+constexpr float kShqInvQ = 1.0f / 255.0f;
+
+PHT_HD float shq_decode(uint32_t code, float lo, float hi) { return lo + (hi - lo) * (float(code) * kShqInvQ); }
+// Codes of a group are (log_s - lo) * scale rounded half away from zero and clamped to [0, 255],
+// with scale = 255 / max(hi - lo, 1e-15) taken once per group.
+PHT_HD float shq_scale(float lo, float hi) { return 255.0f / fmaxf(hi - lo, kShqEps); }
+PHT_HD uint32_t shq_encode(float log_s, float lo, float scale) {
+    return uint32_t(fminf(fmaxf(roundf((log_s - lo) * scale), 0.0f), 255.0f));
+}
+PHT_HD float shq_second(float log_s) {
+    const float s = kShqEps * pexpm1f(log_s);
+    return s * s;
+}
+PHT_HD float shq_log(float v) { return plog1pf(sqrtf(fmaxf(v, 0.0f)) * kShqInvEps); }
+
+struct ShqStep {
+    float lr, rest_lr, beta1, beta2, c1, c2, eps, regularization;
+    int active;
+};
+
+// The step's per-coefficient constants, taken once: the previous step's corrections
+// c_{t-1} = max((c_t - (1 - beta)) / beta, 0) and the reciprocals of the corrections.
+struct ShqConst {
+    ShqStep q;
+    float pc1, inv_pc2, inv_c1, inv_c2;
+    bool has_prev2, has_c;
+};
+PHT_HD ShqConst shq_const(const ShqStep& q) {
+    ShqConst k;
+    k.q = q;
+    const float pc1 = q.beta1 > 0.0f ? fmaxf((q.c1 - (1.0f - q.beta1)) / q.beta1, 0.0f) : 0.0f;
+    const float pc2 = q.beta2 > 0.0f ? fmaxf((q.c2 - (1.0f - q.beta2)) / q.beta2, 0.0f) : 0.0f;
+    k.pc1 = pc1;
+    k.has_prev2 = pc2 > 0.0f;
+    k.inv_pc2 = k.has_prev2 ? 1.0f / pc2 : 0.0f;
+    k.has_c = q.c1 > 0.0f && q.c2 > 0.0f;
+    k.inv_c1 = k.has_c ? 1.0f / q.c1 : 0.0f;
+    k.inv_c2 = k.has_c ? 1.0f / q.c2 : 0.0f;
+    return k;
+}
+// (m / c1) / (sqrt(v / c2) + eps), 0 without valid corrections
+PHT_HD float shq_normalised(const ShqConst& k, float m, float v) {
+    return k.has_c ? (m * k.inv_c1) / (sqrtf(fmaxf(v, 0.0f) * k.inv_c2) + k.q.eps) : 0.0f;
+}
+
+// One coefficient: p (parameter) and u (stored first, decoded) are updated in place; returns the
+// new log_s. log_s_old is the coefficient's decoded code under the row's old bounds.
+PHT_HD float shq_column(const ShqConst& k, int col, float& p, float grad, float& u, float log_s_old) {
+    const ShqStep& q = k.q;
+    float log_s = log_s_old;
+    float g2 = shq_second(log_s);
+    float g1 = 0.0f;
+    if (k.pc1 > 0.0f) g1 = (u * ((k.has_prev2 ? sqrtf(fmaxf(g2, 0.0f) * k.inv_pc2) : 0.0f) + q.eps)) * k.pc1;
+    if (col >= q.active) { // inactive: keep the moments, renormalise u with this step's corrections
+        if (!finite_(g1)) g1 = 0.0f;
+        if (!finite_(log_s)) {
+            log_s = 0.0f;
+            g1 = 0.0f;
+            g2 = 0.0f;
+        }
+        const float e = shq_normalised(k, g1, g2);
+        u = finite_(e) ? e : 0.0f;
+        return log_s;
+    }
+    const float prev = p;
+    if (q.regularization > 0.0f && col >= 3 && finite_(prev)) grad += q.regularization * prev;
+    if (!finite_(prev) || !finite_(grad)) {
+        u = 0.0f;
+        p = finite_(prev) ? prev : 0.0f;
+        return 0.0f;
+    }
+    const float m = q.beta1 * g1 + (1.0f - q.beta1) * grad;
+    const float v = q.beta2 * g2 + (1.0f - q.beta2) * grad * grad;
+    if (!finite_(m) || !finite_(v)) {
+        u = 0.0f;
+        return 0.0f;
+    }
+    const float nrm = shq_normalised(k, m, v);
+    const float cand = prev - (col >= 3 ? q.rest_lr : q.lr) * nrm;
+    p = finite_(cand) ? cand : prev;
+    u = nrm;
+    const float ls = shq_log(v);
+    return finite_(ls) ? ls : 0.0f;
+}
+
+// A row's stored bounds pair, with a non-finite pair read as (0, 0).
+PHT_HD void shq_bounds(const float* b, int col, float& lo, float& hi) {
+    const int o = col < 3 ? 0 : 2;
+    lo = b[o];
+    hi = b[o + 1];
+    if (!finite_(lo) || !finite_(hi)) lo = hi = 0.0f;
+}
+
 // ---------------------------------------------------------------- reduced precision
 PHT_HD float bf16_to_f32(uint16_t h) { return u2f(uint32_t(h) << 16); }
 PHT_HD uint16_t f32_to_bf16(float f) { // round to nearest even, NaN kept quiet

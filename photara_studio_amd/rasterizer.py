@@ -156,20 +156,120 @@ class StructureAdamUpdate:
     maximum_log_scale: float = math.inf
 
     @staticmethod
-    def zeros_like(model: GaussianModel, **kw) -> "StructureAdamUpdate":
+    def zeros_like(model: GaussianModel, sh_moments: bool = True, **kw) -> "StructureAdamUpdate":
+        """Zero moments for every parameter; without ``sh_moments`` the SH coefficients are left
+        to a quantised SH Adam (SHAdamUpdate)."""
         z = torch.zeros_like
         return StructureAdamUpdate(
             means_first=z(model.means), means_second=z(model.means),
             scales_first=z(model.log_scales), scales_second=z(model.log_scales),
             rotations_first=z(model.quaternions), rotations_second=z(model.quaternions),
             opacity_first=z(model.opacity_logits), opacity_second=z(model.opacity_logits),
-            sh_first=z(model.sh), sh_second=z(model.sh), **kw)
+            sh_first=z(model.sh) if sh_moments else None, sh_second=z(model.sh) if sh_moments else None, **kw)
 
     def set_step(self, step: int) -> None:
         """Bias corrections 1 - beta^t in FP32 on the host (cuda_ops.cu:2943-2944)."""
         import numpy as np
         self.correction1 = float(np.float32(1.0) - np.power(np.float32(self.beta1), np.float32(step)))
         self.correction2 = float(np.float32(1.0) - np.power(np.float32(self.beta2), np.float32(step)))
+
+
+@dataclass
+class ShAdamQuant:
+    """The reference trainer's SH optimizer state (detail::ShAdamQuant,
+    photara/src/splat/sh_adam_quant.hpp:12-24): ``first`` float16 [N, stride] normalised Adam
+    update, ``packed`` uint8 [N, stride] log-second codes, ``bounds`` float32 [N, 4] (DC min, DC
+    max, rest min, rest max), stride = SH bases * 3."""
+    first: torch.Tensor = None
+    packed: torch.Tensor = None
+    bounds: torch.Tensor = None
+    stride: int = 0
+
+    @staticmethod
+    def zeros(rows: int, stride: int, device) -> "ShAdamQuant":
+        """make_sh_adam_quant (sh_adam_quant.cpp:9-24). This build takes strides up to 48 (SH degree 3)."""
+        if stride <= 0 or stride > 48:
+            raise ValueError("quantized SH Adam stride must be in [1, 48]")
+        return ShAdamQuant(torch.zeros((rows, stride), dtype=torch.float16, device=device),
+                           torch.zeros((rows, stride), dtype=torch.uint8, device=device),
+                           torch.zeros((rows, 4), dtype=torch.float32, device=device), stride)
+
+    def active(self) -> bool:
+        return self.first is not None
+
+    def select_rows(self, indices: torch.Tensor) -> None:
+        """sh_adam_quant_select_rows: keep (and reorder / repeat) the given rows, bit for bit."""
+        idx = indices.to(device=self.first.device, dtype=torch.long)
+        self.first, self.packed, self.bounds = self.first[idx].contiguous(), self.packed[idx].contiguous(), \
+            self.bounds[idx].contiguous()
+
+    def zero_rows(self, indices: torch.Tensor) -> None:
+        """sh_adam_quant_zero_rows: reset the given rows' state."""
+        if indices is None or indices.numel() == 0:
+            return
+        idx = indices.to(device=self.first.device, dtype=torch.long)
+        self.first[idx] = 0
+        self.packed[idx] = 0
+        self.bounds[idx] = 0
+
+    def append_zeros(self, count: int) -> None:
+        """sh_adam_quant_append_zeros: new rows start from a zero state."""
+        if count <= 0:
+            return
+        z = ShAdamQuant.zeros(count, self.stride, self.first.device)
+        self.first = torch.cat([self.first, z.first])
+        self.packed = torch.cat([self.packed, z.packed])
+        self.bounds = torch.cat([self.bounds, z.bounds])
+
+
+@dataclass
+class SHAdamUpdate:
+    """photara::splat::SHAdamUpdate (rasterizer.hpp:73-82): the quantised SH Adam applied by the
+    backward. ``first``, ``packed`` and ``bounds`` are an ShAdamQuant state's tensors."""
+    first: torch.Tensor = None
+    lr: float = 0.0
+    rest_lr: float = 0.0
+    beta1: float = 0.9
+    beta2: float = 0.999
+    correction1: float = 1.0
+    correction2: float = 1.0
+    epsilon: float = 1e-8
+    regularization_weight: float = 0.0
+    packed: torch.Tensor = None
+    bounds: torch.Tensor = None
+
+    @staticmethod
+    def of(state: ShAdamQuant, **kw) -> "SHAdamUpdate":
+        return SHAdamUpdate(first=state.first, packed=state.packed, bounds=state.bounds, **kw)
+
+    def regularization_factor(self, count: int, stride: int) -> float:
+        """2 w / (N (stride - 3)) in float32 (rasterizer.cu:598-602), 0 without non-DC columns."""
+        import numpy as np
+        if stride <= 3 or self.regularization_weight <= 0.0:
+            return 0.0
+        return float(np.float32(np.float32(2.0) * np.float32(self.regularization_weight)) / np.float32(count * (stride - 3)))
+
+
+def sh_adam_quant_step(parameter: torch.Tensor, gradient: torch.Tensor, state: ShAdamQuant, active_stride: int,
+                       learning_rate: float, rest_learning_rate: float, beta1: float, beta2: float,
+                       correction1: float, correction2: float, adam_epsilon: float,
+                       regularization_factor: float) -> None:
+    """detail::sh_adam_quant_step (photara/src/splat/sh_adam_quant.hpp:34-49) on the gfx950 kernel:
+    columns [0, active_stride) updated, the rest re-encoded under the new bounds."""
+    rows = parameter.numel() // max(state.stride, 1)
+    if parameter.numel() != rows * state.stride or gradient.numel() != parameter.numel():
+        raise ValueError("parameter and gradient must hold rows x stride values")
+    for t, name, dt in ((parameter, "parameter", torch.float32), (gradient, "gradient", torch.float32),
+                        (state.first, "first", torch.float16), (state.packed, "packed", torch.uint8),
+                        (state.bounds, "bounds", torch.float32)):
+        _require(t, name, dt)
+    if not 0 <= active_stride <= state.stride:
+        raise ValueError("active_stride must lie in [0, stride]")
+    check(_abi.load().pht_sh_adam_quant_step(
+        C.c_void_p(parameter.data_ptr()), C.c_void_p(gradient.data_ptr()), C.c_void_p(state.first.data_ptr()),
+        C.c_void_p(state.packed.data_ptr()), C.c_void_p(state.bounds.data_ptr()), rows, state.stride,
+        int(active_stride), learning_rate, rest_learning_rate, beta1, beta2, correction1, correction2,
+        adam_epsilon, regularization_factor, C.c_void_p(_stream())))
 
 
 # ---------------------------------------------------------------- helpers
@@ -433,14 +533,15 @@ class Rasterizer:
                  sh_adam=None, structure_adam: Optional[StructureAdamUpdate] = None) -> ModelGradients:
         if densify_map is not None and densify_map.numel() != 0:
             raise NotImplementedError("error-map densification statistics are not built")
-        if sh_adam is not None:
-            raise NotImplementedError(
-                "the quantised SH Adam state is not built; pass FP32 SH moments in structure_adam")
-        return self.backward_views(model, [rendered], [grad_color], [grad_alpha], structure_adam)
+        return self.backward_views(model, [rendered], [grad_color], [grad_alpha], structure_adam, sh_adam)
 
     def backward_views(self, model: GaussianModel, rendered: Sequence[RenderResult],
                        grad_colors: Sequence[torch.Tensor], grad_alphas: Optional[Sequence] = None,
-                       structure_adam: Optional[StructureAdamUpdate] = None) -> ModelGradients:
+                       structure_adam: Optional[StructureAdamUpdate] = None,
+                       sh_adam: Optional[SHAdamUpdate] = None) -> ModelGradients:
+        """With ``sh_adam`` the SH coefficients take the quantised SH Adam step: fused into the
+        per-Gaussian backward when ``structure_adam`` is given too (the reference's fused path,
+        render_backward.cu:833-853), else as a step on the returned SH gradient."""
         if not rendered or rendered[0].context is None or rendered[0].context.impl is None:
             raise ValueError("Splat backward requires a live forward context")
         ctx: _Context = rendered[0].context.impl
@@ -487,12 +588,27 @@ class Rasterizer:
                 out.sh = torch.empty_like(model.sh)
                 mg.sh = out.sh.data_ptr()
         else:
-            adam_ptr = C.byref(make_adam(model, structure_adam))
+            if sh_adam is not None and (sh_adam.beta1, sh_adam.beta2, sh_adam.correction1, sh_adam.correction2,
+                                        sh_adam.epsilon) != (structure_adam.beta1, structure_adam.beta2,
+                                                             structure_adam.correction1, structure_adam.correction2,
+                                                             structure_adam.epsilon):
+                raise ValueError("the fused step needs one beta1/beta2/correction/epsilon for SH and structure")
+            adam_ptr = C.byref(make_adam(model, structure_adam, sh_adam))
+        if sh_adam is not None and ctx.options.colors_precomp is not None:
+            raise ValueError("sh_adam needs SH colours (no colors_precomp)")
         lib = _abi.load()
         check(lib.pht_backward(ctx.ws.handle, C.byref(ctx.gaussians), dl, C.byref(mg), adam_ptr,
                                C.c_void_p(_stream())))
         ctx.backward_consumed = True
-        if structure_adam is not None:
+        if sh_adam is not None and structure_adam is None:
+            stride = int(model.sh.shape[1]) * 3
+            active = (int(ctx.settings.active_sh_degree) + 1) ** 2 * 3
+            state = ShAdamQuant(sh_adam.first, sh_adam.packed, sh_adam.bounds, stride)
+            sh_adam_quant_step(model.sh, out.sh, state, active, sh_adam.lr, sh_adam.rest_lr, sh_adam.beta1,
+                               sh_adam.beta2, sh_adam.correction1, sh_adam.correction2, sh_adam.epsilon,
+                               sh_adam.regularization_factor(n, stride))
+            out.sh = None  # consumed by the optimizer, as in the reference's fused SH update
+        if structure_adam is not None or sh_adam is not None:
             # parameters changed in place: the recorded forward no longer matches the model
             ctx.release()
             ctx.generation = -1
@@ -539,7 +655,7 @@ def debug_splat_grads(rendered: RenderResult):
     return out.cpu().numpy()
 
 
-def make_adam(model: GaussianModel, a: StructureAdamUpdate) -> _abi.pht_adam:
+def make_adam(model: GaussianModel, a: StructureAdamUpdate, sh: Optional[SHAdamUpdate] = None) -> _abi.pht_adam:
     s = _abi.pht_adam()
     s.means, s.log_scales = model.means.data_ptr(), model.log_scales.data_ptr()
     s.quaternions, s.opacity_logits = model.quaternions.data_ptr(), model.opacity_logits.data_ptr()
@@ -557,4 +673,17 @@ def make_adam(model: GaussianModel, a: StructureAdamUpdate) -> _abi.pht_adam:
     s.sh_lp = _ptr(model.sh_lp)
     s.log_scales_lp = _ptr(model.log_scales_lp)
     s.opacity_logits_lp = _ptr(model.opacity_logits_lp)
+    if sh is not None:
+        stride = int(model.sh.shape[1]) * 3
+        for t, name, dt in ((sh.first, "sh_adam.first", torch.float16), (sh.packed, "sh_adam.packed", torch.uint8),
+                            (sh.bounds, "sh_adam.bounds", torch.float32)):
+            _require(t, name, dt)
+        if tuple(sh.first.shape) != (model.size(), stride) or tuple(sh.packed.shape) != (model.size(), stride) \
+                or tuple(sh.bounds.shape) != (model.size(), 4):
+            raise ValueError("sh_adam state must be [N, 3K] (first, packed) and [N, 4] (bounds)")
+        s.sh = model.sh.data_ptr()
+        s.m_sh = s.v_sh = None
+        s.sh_first, s.sh_packed, s.sh_bounds = sh.first.data_ptr(), sh.packed.data_ptr(), sh.bounds.data_ptr()
+        s.lr_sh0, s.lr_sh_rest = sh.lr, sh.rest_lr
+        s.sh_regularization = sh.regularization_factor(model.size(), stride)
     return s

@@ -16,9 +16,9 @@ import torch
 
 from . import _abi
 from ._abi import check
-from .rasterizer import (Camera, GaussianModel, RasterizeOptions, StructureAdamUpdate, Workspace,
-                         default_instance_capacity, make_adam, make_camera, make_gaussians, make_settings,
-                         tiles_of, validate_model)
+from .rasterizer import (Camera, GaussianModel, RasterizeOptions, SHAdamUpdate, ShAdamQuant, StructureAdamUpdate,
+                         Workspace, default_instance_capacity, make_adam, make_camera, make_gaussians,
+                         make_settings, tiles_of, validate_model)
 
 
 @dataclass
@@ -48,20 +48,36 @@ class TrainStep:
     def __init__(self, model: GaussianModel, cameras: Sequence[Camera], targets: Sequence[torch.Tensor],
                  options: RasterizeOptions, lrs: Optional[LearningRates] = None,
                  instance_capacity: Optional[int] = None, loss: str = "l1", ssim_weight: float = 0.2,
-                 photometric_weight: float = 1.0):
+                 photometric_weight: float = 1.0, sh_optimizer: str = "quant",
+                 sh_regularization_weight: float = 0.0):
         """loss: "l1" (loss_kernel's L1, cuda_ops.cu:886-898) or "l1_ssim" (the trainer's default
-        (1 - w) L1 + w (1 - SSIM), fused_ssim.cu:473-560, w = ssim_weight)."""
+        (1 - w) L1 + w (1 - SSIM), fused_ssim.cu:473-560, w = ssim_weight).
+        sh_optimizer: "quant", the trainer's SH Adam with fp16 update + 8-bit log-second state
+        (trainer.cpp:1569-1577, sh_adam_quant.cuh), or "fp32" (FP32 moments, the active-prefix Adam
+        of cuda_ops.cu:1506-1546). sh_regularization_weight: the L2 weight on the non-DC SH
+        coefficients applied inside the quantised step (TrainingOptions::sh_regularization_weight,
+        0.001 in the reference's defaults)."""
         if loss not in ("l1", "l1_ssim"):
             raise ValueError(f"unknown loss {loss!r}")
+        if sh_optimizer not in ("quant", "fp32"):
+            raise ValueError(f"unknown SH optimizer {sh_optimizer!r}")
         validate_model(model, options)
         self.model, self.cameras, self.options = model, list(cameras), options
         self.targets = [t.contiguous() for t in targets]
         self.lrs = lrs or LearningRates()
         dev = model.means.device
         n, nv = model.size(), len(self.cameras)
+        self.sh_optimizer = sh_optimizer
         self.adam = StructureAdamUpdate.zeros_like(
-            model, means_lr=self.lrs.means, scales_lr=self.lrs.scales, quaternions_lr=self.lrs.quaternions,
-            opacities_lr=self.lrs.opacities, sh_lr=self.lrs.sh0, sh_rest_lr=self.lrs.sh_rest)
+            model, sh_moments=sh_optimizer == "fp32", means_lr=self.lrs.means, scales_lr=self.lrs.scales,
+            quaternions_lr=self.lrs.quaternions, opacities_lr=self.lrs.opacities, sh_lr=self.lrs.sh0,
+            sh_rest_lr=self.lrs.sh_rest)
+        self.sh_state = self.sh_adam = None
+        if sh_optimizer == "quant":
+            self.sh_state = ShAdamQuant.zeros(n, int(model.sh.shape[1]) * 3, dev)
+            self.sh_adam = SHAdamUpdate.of(self.sh_state, lr=self.lrs.sh0, rest_lr=self.lrs.sh_rest,
+                                           beta1=self.adam.beta1, beta2=self.adam.beta2, epsilon=self.adam.epsilon,
+                                           regularization_weight=float(sh_regularization_weight))
         self.cams = (_abi.pht_camera * nv)(*[make_camera(c, options) for c in self.cameras])
         self.settings = make_settings(model, options)
         self.gauss = make_gaussians(model, options)
@@ -115,6 +131,8 @@ class TrainStep:
         self.step_index += 1
         c1, c2 = self._corrections(self.step_index)
         self.adam.correction1, self.adam.correction2 = c1, c2
+        if self.sh_adam is not None:
+            self.sh_adam.correction1, self.sh_adam.correction2 = c1, c2
         stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
         nv = len(self.cameras)
         check(self.lib.pht_forward(self.ws.handle, C.byref(self.gauss), self.cams, nv, C.byref(self.settings),
@@ -131,13 +149,13 @@ class TrainStep:
             else:
                 check(self.lib.pht_l1_loss(color, target, grad, C.c_void_p(self.loss_sum.data_ptr()),
                                            self.colors[k][0].numel(), self.photometric_weight, stream))
-        adam = make_adam(self.model, self.adam)
+        adam = make_adam(self.model, self.adam, self.sh_adam)
         check(self.lib.pht_backward(self.ws.handle, C.byref(self.gauss), self.dl, None, C.byref(adam), stream))
 
     def capture(self) -> None:
         """Capture forward + L1 + backward + Adam as one hipGraph (hyper-parameters on device)."""
         nv = len(self.cameras)
-        self._adam_graph = make_adam(self.model, self.adam)
+        self._adam_graph = make_adam(self.model, self.adam, self.sh_adam)
         self._adam_graph.device_hparams = self.hparams.data_ptr()
         tgt = (C.c_void_p * nv)(*[t.data_ptr() for t in self.targets])
         grd = (C.c_void_p * nv)(*[g.data_ptr() for g in self.grad_colors])
