@@ -12,8 +12,8 @@
 // Backward : the closed-form per-pair terms of blend_bucket_backward (render_backward.cu:383-489),
 //            walked back to front per pixel from the stored final transmittance and contributor
 //            count (T_before = T_after / (1 - alpha)), with no per-bucket snapshots. Each lane
-//            sums its pixels' terms, the wave sums the ten values with interleaved DPP steps,
-//            and lane 63 commits them with one global atomic per value per (instance, tile).
+//            sums its pixels' terms, the wave reduce-scatters the ten values (wave_scatter10),
+//            and ten lanes commit them with one global atomic each per (instance, tile).
 #include "launch.h"
 
 namespace pht {
@@ -32,23 +32,44 @@ __device__ __forceinline__ uint32_t quadrant_mask(const SplatRec& r, int x0, int
     return m;
 }
 
-// Sum of ten values over the wave, one DPP step for all ten before the next, so consecutive
-// DPP operations are independent. Totals land in lane 63; every lane must be active.
-template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ void dpp_step10(float (&r)[10]) {
-    float t[10];
-#pragma unroll
-    for (int k = 0; k < 10; ++k) t[k] = dpp_move<CTRL, ROW_MASK>(r[k]);
-#pragma unroll
-    for (int k = 0; k < 10; ++k) r[k] += t[k];
+// Reduce-scatter of ten per-lane values over the wave (a transposed butterfly). Each of the
+// stages on lane bits 0-3 keeps half of the lane's values and adds the partner lane's copy of
+// them (the partner keeps the other half), so the ten values shrink to one per lane in
+// 5 + 3 + 2 + 1 adds; two cross-row exchanges then complete the sums. Every lane ends with the wave
+// total of value kScatterSlot[lane & 15] (-1: a padding slot). Every lane must be active.
+__constant__ int8_t kScatterSlot[16] = {0, 5, 3, 8, 2, 7, -1, -1, 1, 6, 4, 9, -1, -1, -1, -1};
+
+__device__ __forceinline__ float lane_xor4(float v) {
+    // lane ^ 4 within each row: banks 0 and 2 take lane + 4 (row_shl:4), banks 1 and 3 lane - 4
+    const int x = __builtin_bit_cast(int, v);
+    const int t = __builtin_amdgcn_update_dpp(0, x, 0x104, 0xf, 0x5, false);
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(t, x, 0x114, 0xf, 0xa, false));
 }
-__device__ __forceinline__ void wave_sum10_to_last(float (&r)[10]) {
-    dpp_step10<0xb1, 0xf>(r);
-    dpp_step10<0x4e, 0xf>(r);
-    dpp_step10<0x141, 0xf>(r);
-    dpp_step10<0x140, 0xf>(r);
-    dpp_step10<0x142, 0xa>(r);
-    dpp_step10<0x143, 0xc>(r);
+
+__device__ __forceinline__ float wave_scatter10(const float (&r)[10], int lane) {
+    const bool b0 = lane & 1, b1 = lane & 2, b2 = lane & 4, b3 = lane & 8;
+    float a[6], c[4], d[2];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) { // lane ^ 1: quad_perm [1, 0, 3, 2]
+        const float keep = b0 ? r[5 + i] : r[i], send = b0 ? r[i] : r[5 + i];
+        a[i] = keep + dpp_move<0xb1, 0xf>(send);
+    }
+    a[5] = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) { // lane ^ 2: quad_perm [2, 3, 0, 1]
+        const float keep = b1 ? a[3 + i] : a[i], send = b1 ? a[i] : a[3 + i];
+        c[i] = keep + dpp_move<0x4e, 0xf>(send);
+    }
+    c[3] = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const float keep = b2 ? c[2 + i] : c[i], send = b2 ? c[i] : c[2 + i];
+        d[i] = keep + lane_xor4(send);
+    }
+    const float keep = b3 ? d[1] : d[0], send = b3 ? d[0] : d[1];
+    float e = keep + dpp_move<0x128, 0xf>(send); // lane ^ 8: row_ror:8
+    e += __shfl_xor(e, 16, 64);
+    return e + __shfl_xor(e, 32, 64);
 }
 
 // pair_alpha (splat_math.h) without early returns, so a wave evaluates it with no exec-mask
@@ -204,6 +225,7 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendArgs a) {
     const TileCoord tc = tile_of_block(a, dvp);
     const DevView& dv = *dvp;
     const int lane = threadIdx.x;
+    const int slot_g = kScatterSlot[lane & 15];
     const int W = dv.vi.cam.width, H = dv.vi.cam.height;
     if (a.counters->overflow) return;
     const uint2 range = a.ranges[tc.tile];
@@ -324,13 +346,9 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendArgs a) {
                 }
             }
             if (!any) continue;
-            wave_sum10_to_last(r);
-            if (lane == 63) {
-                float* dst = a.sgrad + grad_offset(sS[j], a.vp_log2, a.n_gauss);
-#pragma unroll
-                for (int g = 0; g < NG; ++g)
-                    if (r[g] != 0.0f) atomic_add_f32(dst + g, r[g]);
-            }
+            const float tot = wave_scatter10(r, lane);
+            if (lane < 16 && slot_g >= 0 && tot != 0.0f)
+                atomic_add_f32(a.sgrad + grad_offset(sS[j], a.vp_log2, a.n_gauss) + slot_g, tot);
         }
     }
 }
