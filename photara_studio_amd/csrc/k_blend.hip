@@ -18,16 +18,35 @@
 
 namespace pht {
 
-// Quadrant k of a TS x TS tile: bit k set when the instance's culling box can reach that 8x8
-// patch (conservative; the exact per-pixel test still decides).
+// Minimum over the rectangle [u0, u1] x [v0, v1] of the positive-definite form
+// q(u, v) = a u^2 + 2 b u v + c v^2: 0 when the rectangle holds the origin, else the least of
+// its four edges' minima (each edge's unconstrained minimiser clamped to the edge).
+__device__ __forceinline__ float form_min_box(float a, float b, float c, float ia, float ic, float u0, float u1,
+                                              float v0, float v1) {
+    if (u0 <= 0.0f && u1 >= 0.0f && v0 <= 0.0f && v1 >= 0.0f) return 0.0f;
+    auto q = [&](float u, float v) { return (a * u + 2.0f * b * v) * u + c * v * v; };
+    auto on_u = [&](float u) { return q(u, fminf(fmaxf(-b * u * ic, v0), v1)); };
+    auto on_v = [&](float v) { return q(fminf(fmaxf(-b * v * ia, u0), u1), v); };
+    return fminf(fminf(on_u(u0), on_u(u1)), fminf(on_v(v0), on_v(v1)));
+}
+
+// Quadrant k of a TS x TS tile: bit k set when the instance can reach that 8x8 patch: its culling
+// box overlaps the patch and its alpha >= 1/255 ellipse (with the record's grown bound on q)
+// meets the patch's pixel-centre rectangle. Conservative; the exact per-pixel test still decides.
 template <int Q>
 __device__ __forceinline__ uint32_t quadrant_mask(const SplatRec& r, int x0, int y0) {
-    const float mx = r.a.x, my = r.a.y, ex = r.c.y, ey = r.c.z;
+    const float mx = r.a.x, my = r.a.y, ex = r.c.y, ey = r.c.z, qmax = r.c.w;
+    const float a = r.a.z, b = r.a.w, c = r.b.x;
+    const bool exact = qmax < __builtin_huge_valf();
+    const float ia = exact ? 1.0f / a : 0.0f, ic = exact ? 1.0f / c : 0.0f;
     uint32_t m = 0;
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
         const float px0 = float(x0 + (k & 1) * 8), py0 = float(y0 + (k >> 1) * 8);
-        if (mx + ex >= px0 && mx - ex <= px0 + 7.0f && my + ey >= py0 && my - ey <= py0 + 7.0f) m |= 1u << k;
+        if (!(mx + ex >= px0 && mx - ex <= px0 + 7.0f && my + ey >= py0 && my - ey <= py0 + 7.0f)) continue;
+        if (exact && form_min_box(a, b, c, ia, ic, mx - (px0 + 7.0f), mx - px0, my - (py0 + 7.0f), my - py0) > qmax)
+            continue;
+        m |= 1u << k;
     }
     return m;
 }

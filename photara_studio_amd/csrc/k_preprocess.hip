@@ -26,24 +26,26 @@ __global__ __launch_bounds__(256) void k_clear(Counters* counters, uint2* ranges
 }
 
 // Half extents of the box around the alpha >= 1/255 ellipse {q(d) <= 2 log(255 op)} of the
-// conic (a, b, c), grown by 1% + 1 px: +inf when the conic is degenerate or too anisotropic for
-// a safe box (no culling), -inf when the opacity is below 1/255 (reaches no pixel).
-__device__ __forceinline__ void patch_extent(float a, float b, float c, float op, float& ex, float& ey) {
+// conic (a, b, c), grown by 1% + 1 px, and the ellipse's bound on q grown by 2% + 0.05 (qmax):
+// +inf when the conic is degenerate or too anisotropic for a safe box (no culling), -inf when the
+// opacity is below 1/255 (reaches no pixel).
+__device__ __forceinline__ void patch_extent(float a, float b, float c, float op, float& ex, float& ey, float& qmax) {
     const float inf = __builtin_huge_valf();
     if (!(op >= K<float>::alpha_floor())) {
-        ex = ey = -inf;
+        ex = ey = qmax = -inf;
         return;
     }
     const float det = a * c - b * b;
     const float thr = 2.0f * plogf(op * 255.0f);
     const float tr = a + c;
-    ex = ey = inf;
+    ex = ey = qmax = inf;
     if (!(det > 0.0f) || !(a > 0.0f) || !(c > 0.0f) || !(thr >= 0.0f) || !(tr * tr < 1.6e5f * det)) return;
     const float x = sqrtf(thr * c / det) * 1.01f + 1.0f;
     const float y = sqrtf(thr * a / det) * 1.01f + 1.0f;
     if (!(x < 1.0e6f) || !(y < 1.0e6f)) return;
     ex = x;
     ey = y;
+    qmax = thr * 1.02f + 0.05f;
 }
 
 template <int LP> __device__ __forceinline__ float load_lp(const float* f, const uint16_t* lp, int64_t i) {
@@ -123,12 +125,12 @@ __global__ __launch_bounds__(256) void k_preprocess(PreArgs a, int64_t n_slots) 
                     view_dir(mean, dv.vi.cam.position, d, raw);
                     sh_eval(DEG, d, coef, rgb, clamp);
                 }
-                float ex, ey;
-                patch_extent(p.conic[0], p.conic[1], p.conic[2], p.opacity, ex, ey);
+                float ex, ey, qmax;
+                patch_extent(p.conic[0], p.conic[1], p.conic[2], p.opacity, ex, ey, qmax);
                 SplatRec& r = a.rec[t];
                 r.a = make_float4(p.mean2d[0], p.mean2d[1], p.conic[0], p.conic[1]);
                 r.b = make_float4(p.conic[2], p.opacity, rgb[0], rgb[1]);
-                r.c = make_float4(rgb[2], ex, ey, 0.0f);
+                r.c = make_float4(rgb[2], ex, ey, qmax);
                 r.d = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 a.meta[t] = make_uint4(0u, f2u(p.depth), touched, clamp | (fp64 ? 8u : 0u));
                 float4* sg = reinterpret_cast<float4*>(a.sgrad + grad_offset(t, a.vp_log2, a.g.count));
