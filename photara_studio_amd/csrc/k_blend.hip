@@ -339,27 +339,31 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendArgs a) {
                 if (contrib) {
                     // the recovered T and the gradients are not compared bit for bit, so the
                     // hardware reciprocal / square root (1 ulp) replace the IEEE sequences
+                    // (fused multiply-adds throughout: these sums are compared to a tolerance)
                     const float inv_1ma = __builtin_amdgcn_rcpf(1.0f - alpha);
                     const float Tb = T[k] * inv_1ma;
                     const float wgt = alpha * Tb;
-                    const float c_dot = (B.z * gp0[k] + B.w * gp1[k]) + cb * gp2[k];
-                    const float dL_dalpha = Tb * c_dot - E[k] * inv_1ma;
+                    const float c_dot = fmaf(cb, gp2[k], fmaf(B.w, gp1[k], B.z * gp0[k]));
+                    const float dL_dalpha = fmaf(Tb, c_dot, -E[k] * inv_1ma);
                     const float dL_dG = B.y * dL_dalpha;
                     const float gdx = G * dx, gdy = G * dy;
-                    const float dG_dx = -gdx * A.z - gdy * A.w;
-                    const float dG_dy = -gdy * B.x - gdx * A.w;
-                    const float ddx = dL_dG * dG_dx, ddy = dL_dG * dG_dy;
+                    // dL/dmean2d = dL/dG * dG/d(dx, dy), dG/ddx = -G (a dx + b dy), dG/ddy = -G (b dx + c dy)
+                    const float ddx = -dL_dG * fmaf(gdx, A.z, gdy * A.w);
+                    const float ddy = -dL_dG * fmaf(gdy, B.x, gdx * A.w);
                     r[0] += ddx;
                     r[1] += ddy;
-                    r[2] += -0.5f * gdx * dx * dL_dG;
-                    r[3] += -0.5f * gdx * dy * dL_dG;
-                    r[4] += -0.5f * gdy * dy * dL_dG;
-                    r[5] += G * dL_dalpha;
-                    r[6] += wgt * gp0[k];
-                    r[7] += wgt * gp1[k];
-                    r[8] += wgt * gp2[k];
+                    // dL/dconic = -dL/dG * G * (dx^2, dx dy, dy^2) / 2
+                    const float h = -0.5f * dL_dG;
+                    const float hx = h * gdx;
+                    r[2] = fmaf(hx, dx, r[2]);
+                    r[3] = fmaf(hx, dy, r[3]);
+                    r[4] = fmaf(h * gdy, dy, r[4]);
+                    r[5] = fmaf(G, dL_dalpha, r[5]);
+                    r[6] = fmaf(wgt, gp0[k], r[6]);
+                    r[7] = fmaf(wgt, gp1[k], r[7]);
+                    r[8] = fmaf(wgt, gp2[k], r[8]);
                     const float rx = ddx * fW, ry = ddy * fH;
-                    r[9] += __builtin_amdgcn_sqrtf(rx * rx + ry * ry) * refine_den[k];
+                    r[9] = fmaf(__builtin_amdgcn_sqrtf(fmaf(rx, rx, ry * ry)), refine_den[k], r[9]);
                     E[k] = fmaf(c_dot, wgt, E[k]);
                     T[k] = Tb;
                 }
