@@ -42,6 +42,44 @@ __device__ __forceinline__ void for_block_rows(int rows, int stride, int active,
     }
 }
 
+// Walks the block's contiguous range [0, rows * stride) in chunks of four consecutive
+// coefficients (stride % 4 == 0, so a chunk never straddles two rows): chunk c = t + 256 u. For
+// each group of U chunks it calls load(u, c, row, k0) for all of them before step(u, c, row, k0)
+// for any, so a thread keeps U chunks of every array in flight: the SH passes are bound by how
+// many bytes are outstanding, and one scalar per array per iteration leaves HBM mostly idle.
+template <int U, class Load, class Step>
+__device__ __forceinline__ void for_block_chunks(int rows, int stride, Load&& load, Step&& step) {
+    const int n = (rows * stride) >> 2;
+    for (int c0 = int(threadIdx.x); c0 < n; c0 += U * kScanThreads) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int c = c0 + u * kScanThreads;
+            if (c < n) load(u, c, (4 * c) / stride, (4 * c) % stride);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int c = c0 + u * kScanThreads;
+            if (c < n) step(u, c, (4 * c) / stride, (4 * c) % stride);
+        }
+    }
+}
+
+// Four stored SH values as floats (fp32, or four bf16/fp16 in one 8-byte load).
+template <int LP> __device__ __forceinline__ float4 ld4_lp(const float* f, const uint16_t* lp, int64_t e) {
+    if constexpr (LP == PHT_LP_NONE) {
+        return *reinterpret_cast<const float4*>(f + e);
+    } else {
+        const uint2 h = *reinterpret_cast<const uint2*>(lp + e);
+        return make_float4(lp_to_f32(uint16_t(h.x & 0xffffu), LP), lp_to_f32(uint16_t(h.x >> 16), LP),
+                           lp_to_f32(uint16_t(h.y & 0xffffu), LP), lp_to_f32(uint16_t(h.y >> 16), LP));
+    }
+}
+__device__ __forceinline__ uint2 pack4_lp(float a, float b, float c, float d, int fmt) {
+    return make_uint2(uint32_t(f32_to_lp(a, fmt)) | (uint32_t(f32_to_lp(b, fmt)) << 16),
+                      uint32_t(f32_to_lp(c, fmt)) | (uint32_t(f32_to_lp(d, fmt)) << 16));
+}
+__device__ __forceinline__ bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
 // Quantised SH Adam over the block's `rows` consecutive rows starting at row_begin
 // (sh_adam_quant_warp_row, third_party/splat_drender/include/splat_drender/sh_adam_quant.cuh:146-234,
 // with the per-coefficient step of shq_column). On entry s_row holds each row's gradient in
@@ -64,7 +102,54 @@ template <int LP>
 __device__ void shq_block(const ShqArgs& q, int64_t row_begin, int rows, int stride, float* s_row) {
     const int64_t e0 = row_begin * stride;
     const ShqConst kc = shq_const(q.step);
-    for_block_rows(rows, stride, stride, [&](int e, int row, int k) {
+    const bool chunked = (stride & 3) == 0 && aligned16(q.param + e0) && aligned16(q.bounds) &&
+                         (reinterpret_cast<uintptr_t>(q.first + e0) & 7u) == 0 &&
+                         (reinterpret_cast<uintptr_t>(q.packed + e0) & 3u) == 0 &&
+                         (!q.lp || (reinterpret_cast<uintptr_t>(q.lp + e0) & 7u) == 0);
+    if (chunked) {
+        constexpr int U = 2;
+        float4 pv[U], bv[U];
+        uint2 fv[U];
+        uint32_t cv[U];
+        float* P = q.param + e0;
+        uint16_t* F = q.first + e0;
+        uint8_t* Cd = q.packed + e0;
+        for_block_chunks<U>(
+            rows, stride,
+            [&](int u, int c, int row, int) {
+                pv[u] = reinterpret_cast<const float4*>(P)[c];
+                fv[u] = reinterpret_cast<const uint2*>(F)[c];
+                cv[u] = reinterpret_cast<const uint32_t*>(Cd)[c];
+                bv[u] = reinterpret_cast<const float4*>(q.bounds)[row_begin + row];
+            },
+            [&](int u, int c, int row, int k0) {
+                // the row's (min, max) pairs, a non-finite pair read as (0, 0)
+                const bool dc_ok = finite_(bv[u].x) && finite_(bv[u].y), rest_ok = finite_(bv[u].z) && finite_(bv[u].w);
+                float p[4] = {pv[u].x, pv[u].y, pv[u].z, pv[u].w};
+                const uint32_t h[4] = {fv[u].x & 0xffffu, fv[u].x >> 16, fv[u].y & 0xffffu, fv[u].y >> 16};
+                uint32_t nh[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int k = k0 + j;
+                    const bool dc = k < 3;
+                    const float lo = dc ? (dc_ok ? bv[u].x : 0.0f) : (rest_ok ? bv[u].z : 0.0f);
+                    const float hi = dc ? (dc_ok ? bv[u].y : 0.0f) : (rest_ok ? bv[u].w : 0.0f);
+                    float uu = fp16_to_f32(uint16_t(h[j]));
+                    const bool act = k < q.step.active;
+                    const float ls = shq_column(kc, k, p[j], act ? s_row[row * kShPitch + k] : 0.0f, uu,
+                                                shq_decode((cv[u] >> (8 * j)) & 255u, lo, hi));
+                    nh[j] = f32_to_fp16(uu);
+                    s_row[row * kShPitch + k] = ls;
+                }
+                reinterpret_cast<uint2*>(F)[c] = make_uint2(nh[0] | (nh[1] << 16), nh[2] | (nh[3] << 16));
+                if (k0 < q.step.active) {
+                    reinterpret_cast<float4*>(P)[c] = make_float4(p[0], p[1], p[2], p[3]);
+                    if constexpr (LP != PHT_LP_NONE) {
+                        if (q.lp) reinterpret_cast<uint2*>(q.lp + e0)[c] = pack4_lp(p[0], p[1], p[2], p[3], LP);
+                    }
+                }
+            });
+    } else for_block_rows(rows, stride, stride, [&](int e, int row, int k) {
         float lo, hi;
         shq_bounds(q.bounds + (row_begin + row) * 4, k, lo, hi);
         float p = q.param[e0 + e];
@@ -129,9 +214,28 @@ __global__ __launch_bounds__(256) void k_gaussian_bwd(GaussArgs a) {
     const bool precomp = a.g.colors_precomp != nullptr;
     const bool use_sh = !precomp;
     if (use_sh) {
-        for_block_rows(rows, stride, NB * 3, [&](int e, int row, int k) {
-            s_sh[row * kShPitch + k] = ld_lp<LP>(a.g.sh, a.g.sh_lp, row0 + e);
-        });
+        const bool chunked = (stride & 3) == 0 && (LP == PHT_LP_NONE ? aligned16(a.g.sh + row0)
+                                                                     : (reinterpret_cast<uintptr_t>(a.g.sh_lp + row0) & 7u) == 0);
+        if (chunked) {
+            float4 v[4];
+            for_block_chunks<4>(
+                rows, stride,
+                [&](int u, int c, int, int k0) {
+                    if (k0 < NB * 3) v[u] = ld4_lp<LP>(a.g.sh, a.g.sh_lp, row0 + 4 * int64_t(c));
+                },
+                [&](int u, int, int row, int k0) {
+                    if (k0 >= NB * 3) return;
+                    float* d = s_sh + row * kShPitch + k0;
+                    d[0] = v[u].x;
+                    d[1] = v[u].y;
+                    d[2] = v[u].z;
+                    d[3] = v[u].w;
+                });
+        } else {
+            for_block_rows(rows, stride, NB * 3, [&](int e, int row, int k) {
+                s_sh[row * kShPitch + k] = ld_lp<LP>(a.g.sh, a.g.sh_lp, row0 + e);
+            });
+        }
     }
     __syncthreads();
     float* my_sh = s_sh + threadIdx.x * kShPitch;
@@ -261,7 +365,39 @@ __global__ __launch_bounds__(256) void k_gaussian_bwd(GaussArgs a) {
             float* M = A.m_sh + row0;
             float* V = A.v_sh + row0;
             uint16_t* L = (LP != PHT_LP_NONE && A.sh_lp) ? A.sh_lp + row0 : nullptr;
-            for_block_rows(rows, stride, NB * 3, [&](int e, int row, int k) {
+            const bool chunked = (stride & 3) == 0 && aligned16(P) && aligned16(M) && aligned16(V) &&
+                                 (!L || (reinterpret_cast<uintptr_t>(L) & 7u) == 0);
+            if (chunked) {
+                constexpr int U = 4;
+                float4 pv[U], mv[U], vv[U];
+                for_block_chunks<U>(
+                    rows, stride,
+                    [&](int u, int c, int, int k0) {
+                        if (k0 >= NB * 3) return;
+                        pv[u] = reinterpret_cast<const float4*>(P)[c];
+                        mv[u] = reinterpret_cast<const float4*>(M)[c];
+                        vv[u] = reinterpret_cast<const float4*>(V)[c];
+                    },
+                    [&](int u, int c, int row, int k0) {
+                        if (k0 >= NB * 3) return;
+                        float p[4] = {pv[u].x, pv[u].y, pv[u].z, pv[u].w};
+                        float m[4] = {mv[u].x, mv[u].y, mv[u].z, mv[u].w};
+                        float v[4] = {vv[u].x, vv[u].y, vv[u].z, vv[u].w};
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            const int k = k0 + j;
+                            if (k < NB * 3)
+                                adam_update(p[j], m[j], v[j], s_sh[row * kShPitch + k], k < 3 ? A.lr_sh0 : A.lr_sh_rest,
+                                            A.beta1, A.beta2, c1, c2, A.epsilon, -inf, inf, true);
+                        }
+                        reinterpret_cast<float4*>(P)[c] = make_float4(p[0], p[1], p[2], p[3]);
+                        reinterpret_cast<float4*>(M)[c] = make_float4(m[0], m[1], m[2], m[3]);
+                        reinterpret_cast<float4*>(V)[c] = make_float4(v[0], v[1], v[2], v[3]);
+                        if constexpr (LP != PHT_LP_NONE) {
+                            if (L) reinterpret_cast<uint2*>(L)[c] = pack4_lp(p[0], p[1], p[2], p[3], LP);
+                        }
+                    });
+            } else for_block_rows(rows, stride, NB * 3, [&](int e, int row, int k) {
                 float pv = P[e], mv = M[e], vv = V[e];
                 adam_update(pv, mv, vv, s_sh[row * kShPitch + k], k < 3 ? A.lr_sh0 : A.lr_sh_rest, A.beta1,
                             A.beta2, c1, c2, A.epsilon, -inf, inf, true);
