@@ -27,7 +27,8 @@ __device__ __forceinline__ void chunk_of(uint32_t n, int nb, int b, uint32_t& lo
     hi = h < n ? uint32_t(h) : n;
 }
 
-// Block-wide inclusive scan of one value per thread (256 threads = 4 waves).
+// Block-wide inclusive scan of one value per thread (NW waves; 256 threads = 4 waves).
+template <int NW = 4>
 __device__ __forceinline__ uint32_t block_inclusive_scan(uint32_t v, uint32_t* s_w, uint32_t& total) {
     const int lane = lane_id(), w = threadIdx.x >> 6;
     const uint32_t inc = wave_inclusive_scan(v);
@@ -35,7 +36,7 @@ __device__ __forceinline__ uint32_t block_inclusive_scan(uint32_t v, uint32_t* s
     __syncthreads();
     uint32_t before = 0;
     total = 0;
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < NW; ++k) {
         const uint32_t t = s_w[k];
         if (k < w) before += t;
         total += t;
@@ -133,9 +134,11 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_write(Src src, Sink sink,
 // scans that digit's counts over the blocks and stores the digit total) and downsweep. The
 // downsweep ranks tiles of kSortTile keys in LDS: each wave ranks its 1024 keys row by row with a
 // wave-level multi-split (8 ballots) against per-wave running digit counters, the tile is
-// reordered by digit in LDS, and each digit's run is written out contiguously.
-constexpr int kSortRows = 16;                      // keys per lane and tile
-constexpr int kSortTile = kScanThreads * kSortRows; // 4096 keys per block iteration
+// reordered by digit in LDS, and each digit's run is written out contiguously. A block of BT
+// threads ranks BT * 16 keys per iteration: 256 threads for the depth sort (three payload arrays
+// in LDS), 512 for the tile sort, whose 8192-key tiles give each digit runs of 32 keys on average,
+// i.e. whole 128-byte lines per digit and array in the write-out.
+constexpr int kSortRows = 16; // keys per lane and tile
 
 __global__ __launch_bounds__(kScanThreads) void k_radix_upsweep(const uint32_t* keys, const uint32_t* n_ptr,
                                                                 uint32_t n_const, uint32_t cap, int shift, int nb,
@@ -182,28 +185,31 @@ __global__ __launch_bounds__(1024) void k_radix_scan(uint32_t* hist, int nb, uin
     if (t == 0) digit_total[blockIdx.x] = total;
 }
 
-template <bool AUX>
-__global__ __launch_bounds__(kScanThreads) void k_radix_downsweep(const uint32_t* keys_in, const uint32_t* vals_in,
+template <bool AUX, int BT>
+__global__ __launch_bounds__(BT) void k_radix_downsweep(const uint32_t* keys_in, const uint32_t* vals_in,
                                                                   const uint32_t* aux_in, uint32_t* keys_out,
                                                                   uint32_t* vals_out, uint32_t* aux_out,
                                                                   const uint32_t* n_ptr, uint32_t n_const,
                                                                   uint32_t cap, int shift, int nb,
                                                                   const uint32_t* hist, const uint32_t* digit_total) {
+    constexpr int NW = BT / 64;
+    constexpr int kSortTile = BT * kSortRows;
     __shared__ uint32_t s_key[kSortTile];
     __shared__ uint32_t s_val[kSortTile];
     __shared__ uint32_t s_aux[AUX ? kSortTile : 1];
-    __shared__ uint32_t s_cnt[4][256];  // per-wave running digit counts, then per-wave offsets
+    __shared__ uint32_t s_cnt[NW][256]; // per-wave running digit counts, then per-wave offsets
     __shared__ uint32_t s_start[256];   // digit start inside the reordered tile
     __shared__ uint32_t s_gbase[256];   // global position of the tile's first key of each digit
-    __shared__ uint32_t s_w[4];
+    __shared__ uint32_t s_w[NW];
     const int t = threadIdx.x, lane = lane_id(), w = t >> 6;
+    const bool digit_thread = t < 256; // thread t < 256 owns digit t in the per-tile bookkeeping
     // global digit base of this block = exclusive digit prefix + this block's offset in the digit
-    uint32_t run;
+    uint32_t run = 0;
     {
-        const uint32_t tot = digit_total[t];
+        const uint32_t tot = digit_thread ? digit_total[t] : 0u;
         uint32_t all;
-        const uint32_t inc = block_inclusive_scan(tot, s_w, all);
-        run = inc - tot + hist[size_t(t) * nb + blockIdx.x];
+        const uint32_t inc = block_inclusive_scan<NW>(tot, s_w, all);
+        if (digit_thread) run = inc - tot + hist[size_t(t) * nb + blockIdx.x];
     }
     const uint32_t n = load_count(n_ptr, n_const, cap);
     uint32_t lo, hi;
@@ -213,7 +219,7 @@ __global__ __launch_bounds__(kScanThreads) void k_radix_downsweep(const uint32_t
     // been reordered into LDS, so its global loads overlap the current tile's write-out.
     uint32_t key[kSortRows], val[kSortRows], aux[kSortRows], rank[kSortRows];
     auto load_tile = [&](uint32_t b) {
-        const uint32_t wb = b + uint32_t(w) * (kSortTile / 4);
+        const uint32_t wb = b + uint32_t(w) * (kSortTile / NW);
 #pragma unroll
         for (int r = 0; r < kSortRows; ++r) {
             const uint32_t i = wb + uint32_t(r) * 64u + uint32_t(lane);
@@ -225,9 +231,9 @@ __global__ __launch_bounds__(kScanThreads) void k_radix_downsweep(const uint32_t
     };
     if (lo < hi) load_tile(lo);
     for (uint32_t base = lo; base < hi; base += kSortTile) {
-        for (int k = 0; k < 4; ++k) s_cnt[k][t] = 0;
+        for (int k = t; k < NW * 256; k += BT) (&s_cnt[0][0])[k] = 0;
         __syncthreads();
-        const uint32_t wbase = base + uint32_t(w) * (kSortTile / 4);
+        const uint32_t wbase = base + uint32_t(w) * (kSortTile / NW);
 #pragma unroll
         for (int r = 0; r < kSortRows; ++r) {
             const uint32_t i = wbase + uint32_t(r) * 64u + uint32_t(lane);
@@ -249,17 +255,21 @@ __global__ __launch_bounds__(kScanThreads) void k_radix_downsweep(const uint32_t
         __syncthreads();
         // thread t = digit t: per-wave offsets inside the digit, digit starts inside the tile
         {
-            uint32_t c[4], s = 0;
-            for (int k = 0; k < 4; ++k) {
-                c[k] = s_cnt[k][t];
-                s_cnt[k][t] = s;
-                s += c[k];
+            uint32_t s = 0;
+            if (digit_thread) {
+                for (int k = 0; k < NW; ++k) {
+                    const uint32_t c = s_cnt[k][t];
+                    s_cnt[k][t] = s;
+                    s += c;
+                }
             }
             uint32_t all;
-            const uint32_t inc = block_inclusive_scan(s, s_w, all);
-            s_start[t] = inc - s;
-            s_gbase[t] = run;
-            run += s;
+            const uint32_t inc = block_inclusive_scan<NW>(s, s_w, all);
+            if (digit_thread) {
+                s_start[t] = inc - s;
+                s_gbase[t] = run;
+                run += s;
+            }
         }
         __syncthreads();
 #pragma unroll
@@ -276,7 +286,7 @@ __global__ __launch_bounds__(kScanThreads) void k_radix_downsweep(const uint32_t
         if (base + kSortTile < hi) load_tile(base + kSortTile);
         __syncthreads();
         const uint32_t tile_n = hi - base < uint32_t(kSortTile) ? hi - base : uint32_t(kSortTile);
-        for (uint32_t j = t; j < tile_n; j += kScanThreads) {
+        for (uint32_t j = t; j < tile_n; j += BT) {
             const uint32_t k = s_key[j];
             const uint32_t d = (k >> shift) & 255u;
             const uint32_t dst = s_gbase[d] + (j - s_start[d]);
@@ -327,18 +337,19 @@ void launch_tile_offsets(const uint32_t* sorted_tiles, const uint32_t* visible_p
 
 int launch_radix_sort(uint32_t* keys[2], uint32_t* vals[2], uint32_t* aux[2], const uint32_t* n_ptr, uint32_t cap,
                       int bits, uint32_t* hist, uint32_t* digit_total, hipStream_t st) {
-    const int nb = blocks_for(cap);
+    // the key-value sort takes 512-thread blocks: two resident per CU, so at most 512 chunks
+    const int nb = aux ? blocks_for(cap) : (blocks_for(cap) < 512 ? blocks_for(cap) : 512);
     int cur = 0;
     for (int shift = 0; shift < bits; shift += 8) {
         hipLaunchKernelGGL(k_radix_upsweep, dim3(nb), dim3(kScanThreads), 0, st, keys[cur], n_ptr, 0u, cap, shift,
                            nb, hist);
         hipLaunchKernelGGL(k_radix_scan, dim3(256), dim3(1024), 0, st, hist, nb, digit_total);
         if (aux)
-            hipLaunchKernelGGL(k_radix_downsweep<true>, dim3(nb), dim3(kScanThreads), 0, st, keys[cur], vals[cur],
+            hipLaunchKernelGGL((k_radix_downsweep<true, 256>), dim3(nb), dim3(256), 0, st, keys[cur], vals[cur],
                                aux[cur], keys[cur ^ 1], vals[cur ^ 1], aux[cur ^ 1], n_ptr, 0u, cap, shift, nb,
                                hist, digit_total);
         else
-            hipLaunchKernelGGL(k_radix_downsweep<false>, dim3(nb), dim3(kScanThreads), 0, st, keys[cur], vals[cur],
+            hipLaunchKernelGGL((k_radix_downsweep<false, 512>), dim3(nb), dim3(512), 0, st, keys[cur], vals[cur],
                                nullptr, keys[cur ^ 1], vals[cur ^ 1], nullptr, n_ptr, 0u, cap, shift, nb, hist,
                                digit_total);
         cur ^= 1;
