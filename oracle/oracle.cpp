@@ -341,9 +341,10 @@ void run_forward_matched(const pht_gaussians& g, const pht_settings& s, const St
                     if (test32 < K<float>::transmittance_floor()) break;
                     T32 = test32;
                     const SplatRec<double>& r = st.splats[st.vals[k]];
-                    const double power = gaussian_power(r.conic[0], r.conic[1], r.conic[2], r.mean2d[0] - double(px),
-                                                        r.mean2d[1] - double(py));
-                    const double alpha = std::min(clip, r.opacity * std::exp(power));
+                    double a2, b2, c2;
+                    conic_log2(r.conic[0], r.conic[1], r.conic[2], a2, b2, c2);
+                    const double alpha = std::min(
+                        clip, r.opacity * exp2(pair_power2(a2, b2, c2, r.mean2d[0] - double(px), r.mean2d[1] - double(py))));
                     const double a_t = alpha * Tr;
                     for (int ch = 0; ch < 3; ++ch) C[ch] = fma(r.rgb[ch], a_t, C[ch]);
                     Tr = Tr * (1.0 - alpha);
@@ -410,7 +411,9 @@ void run_blend_backward(const pht_gaussians& g, const pht_settings& s, const Sta
                         if (!pair_alpha(q.conic[0], q.conic[1], q.conic[2], q.opacity, q.mean2d[0] - float(px),
                                         q.mean2d[1] - float(py), float(s.alpha_clip), a32, G32))
                             continue;
-                        G = exp_(gaussian_power(r.conic[0], r.conic[1], r.conic[2], dx, dy));
+                        T a2, b2, c2;
+                        conic_log2(r.conic[0], r.conic[1], r.conic[2], a2, b2, c2);
+                        G = exp2_blend_(pair_power2(a2, b2, c2, dx, dy));
                         alpha = min_(clip, r.opacity * G);
                     } else if (!pair_alpha(r.conic[0], r.conic[1], r.conic[2], r.opacity, dx, dy, clip, alpha, G)) {
                         continue;
@@ -741,8 +744,8 @@ void oracle_set_inputs64(const double* means, const double* log_scales, const do
 void oracle_pexpf(const float* x, float* y, int64_t n) {
     for (int64_t i = 0; i < n; ++i) y[i] = pexpf(x[i]);
 }
-void oracle_pexpf_inrange(const float* x, float* y, int64_t n) {
-    for (int64_t i = 0; i < n; ++i) y[i] = pexpf_inrange(x[i]);
+void oracle_pexp2f_blend(const float* x, float* y, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) y[i] = pexp2f_blend(x[i]);
 }
 void oracle_plogf(const float* x, float* y, int64_t n) {
     for (int64_t i = 0; i < n; ++i) y[i] = plogf(x[i]);
@@ -753,9 +756,13 @@ void oracle_plog1pf(const float* x, float* y, int64_t n) {
 void oracle_pexpm1f(const float* x, float* y, int64_t n) {
     for (int64_t i = 0; i < n; ++i) y[i] = pexpm1f(x[i]);
 }
-void oracle_gaussian_power(const float* conic3, const float* d2, float* out, int64_t n) {
-    for (int64_t i = 0; i < n; ++i)
-        out[i] = gaussian_power(conic3[3 * i], conic3[3 * i + 1], conic3[3 * i + 2], d2[2 * i], d2[2 * i + 1]);
+// The blend's base-2 exponent of each (conic, offset) pair (conic_log2 + pair_power2).
+void oracle_pair_power2(const float* conic3, const float* d2, float* out, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) {
+        float a2, b2, c2;
+        conic_log2(conic3[3 * i], conic3[3 * i + 1], conic3[3 * i + 2], a2, b2, c2);
+        out[i] = pair_power2(a2, b2, c2, d2[2 * i], d2[2 * i + 1]);
+    }
 }
 void oracle_lp_roundtrip(const float* x, uint16_t* bits, float* back, int64_t n, int fmt) {
     for (int64_t i = 0; i < n; ++i) {

@@ -30,14 +30,14 @@ def load() -> C.CDLL:
         lib.oracle_pexpf.argtypes = [vp, vp, C.c_int64]
         lib.oracle_l1_ssim.argtypes = [C.c_int, vp, vp, vp, C.c_int, C.c_int, C.c_float, C.c_float, vp]
         lib.oracle_l1_ssim.restype = C.c_double
-        lib.oracle_pexpf_inrange.argtypes = [vp, vp, C.c_int64]
+        lib.oracle_pexp2f_blend.argtypes = [vp, vp, C.c_int64]
         lib.oracle_plog1pf.argtypes = [vp, vp, C.c_int64]
         lib.oracle_pexpm1f.argtypes = [vp, vp, C.c_int64]
         lib.oracle_sh_adam_quant_step.argtypes = [vp, vp, vp, vp, vp, C.c_int64, C.c_int, C.c_int, C.c_float,
                                                   C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
                                                   C.c_float]
         lib.oracle_plogf.argtypes = [vp, vp, C.c_int64]
-        lib.oracle_gaussian_power.argtypes = [vp, vp, vp, C.c_int64]
+        lib.oracle_pair_power2.argtypes = [vp, vp, vp, C.c_int64]
         lib.oracle_lp_roundtrip.argtypes = [vp, vp, vp, C.c_int64, C.c_int]
         lib.oracle_project.argtypes = [C.c_int, C.POINTER(_abi.pht_gaussians), C.POINTER(_abi.pht_camera),
                                        C.POINTER(_abi.pht_settings), vp]
@@ -266,10 +266,10 @@ def pexpf(x: np.ndarray) -> np.ndarray:
     return y
 
 
-def pexpf_inrange(x: np.ndarray) -> np.ndarray:
+def pexp2f_blend(x: np.ndarray) -> np.ndarray:
     x = f32(x)
     y = np.empty_like(x)
-    load().oracle_pexpf_inrange(_p(x), _p(y), x.size)
+    load().oracle_pexp2f_blend(_p(x), _p(y), x.size)
     return y
 
 
@@ -332,10 +332,11 @@ def sh_adam_quant_step(p: np.ndarray, g: Optional[np.ndarray], state: ShQuantSta
                                      state.stride, active, lr, rest_lr, b1, b2, c1, c2, eps, regularization)
 
 
-def gaussian_power(conic3: np.ndarray, d2: np.ndarray) -> np.ndarray:
+def pair_power2(conic3: np.ndarray, d2: np.ndarray) -> np.ndarray:
+    """The blend's base-2 exponent log2(e) * power of each (conic, offset) pair."""
     conic3, d2 = f32(conic3), f32(d2)
     out = np.empty((conic3.shape[0],), np.float32)
-    load().oracle_gaussian_power(_p(conic3), _p(d2), _p(out), conic3.shape[0])
+    load().oracle_pair_power2(_p(conic3), _p(d2), _p(out), conic3.shape[0])
     return out
 
 

@@ -31,10 +31,13 @@ struct Counters {
 
 // Blend record, one per (gaussian, view) slot (slot = g * vp + v), 64 bytes so that every
 // gather of it by the tile kernels is one aligned 64-byte HBM sector:
-//   a = (mean.x, mean.y, conic.x, conic.y)   b = (conic.z, opacity, r, g)
-//   c = (b, ext.x, ext.y, 0)                 d = 0
-// ext = half extents of the box around the alpha >= 1/255 ellipse (grown by 1% + 1 px), used
-// to cull 8x8 pixel patches; +inf = no culling, -inf = reaches no pixel.
+//   a = (mean.x, mean.y, a2, b2)          b = (c2, opacity, r, g)
+//   c = (b, ext.x, ext.y, qmax)           d = (conic.x, conic.y, conic.z, opacity)
+// (a2, b2, c2) = the conic scaled for base-2 exponents (conic_log2), read by the per-pair test;
+// ext = half extents of the box around the alpha >= 1/255 ellipse (grown by 1% + 1 px) and
+// qmax = the ellipse's grown bound on the quadratic form, used to cull 8x8 pixel patches
+// (+inf = no culling, -inf = reaches no pixel); the conic itself feeds the tile walk (which reads
+// a and d only) and the gradient terms.
 struct SplatRec {
     float4 a, b, c, d;
 };

@@ -36,7 +36,7 @@ __device__ __forceinline__ float form_min_box(float a, float b, float c, float i
 template <int Q>
 __device__ __forceinline__ uint32_t quadrant_mask(const SplatRec& r, int x0, int y0) {
     const float mx = r.a.x, my = r.a.y, ex = r.c.y, ey = r.c.z, qmax = r.c.w;
-    const float a = r.a.z, b = r.a.w, c = r.b.x;
+    const float a = r.d.x, b = r.d.y, c = r.d.z;
     const bool exact = qmax < __builtin_huge_valf();
     const float ia = exact ? 1.0f / a : 0.0f, ic = exact ? 1.0f / c : 0.0f;
     uint32_t m = 0;
@@ -91,14 +91,15 @@ __device__ __forceinline__ float wave_scatter10(const float (&r)[10], int lane) 
     return e + __shfl_xor(e, 32, 64);
 }
 
-// pair_alpha (splat_math.h) without early returns, so a wave evaluates it with no exec-mask
-// branches: identical G, alpha and decision for every pair pair_alpha accepts (the exp argument
-// is the same power), including a NaN power; other lanes get a placeholder G.
-__device__ __forceinline__ bool pair_alpha_flat(float ca, float cb, float cc, float op, float dx, float dy,
+// pair_alpha2 (splat_math.h) without early returns, so a wave evaluates it with no exec-mask
+// branches: identical G, alpha and decision for every pair pair_alpha2 accepts (the exponential's
+// argument is the same exponent), including a NaN exponent; other lanes get a placeholder G.
+// (a2, b2, c2) is the splat's base-2 conic.
+__device__ __forceinline__ bool pair_alpha_flat(float a2, float b2, float c2, float op, float dx, float dy,
                                                 float clip, float& alpha, float& G) {
-    const float power = gaussian_power(ca, cb, cc, dx, dy);
-    const bool ok = !(power > 0.0f) && !(power < kSkipPower);
-    G = pexpf_inrange(ok ? power : 0.0f);
+    const float p2 = pair_power2(a2, b2, c2, dx, dy);
+    const bool ok = !(p2 > 0.0f) && !(p2 < kSkipPower2);
+    G = pexp2f_blend(ok ? p2 : 0.0f);
     alpha = min_(clip, op * G);
     return ok && !(alpha < K<float>::alpha_floor());
 }
@@ -251,7 +252,7 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendArgs a) {
 
     __shared__ float4 sA[BATCH];
     __shared__ float4 sB[BATCH];
-    __shared__ float sC[BATCH];
+    __shared__ float4 sC[BATCH]; // (colour b, conic a, b, c): the gradient terms take the conic itself
     __shared__ uint32_t sM[BATCH];
     __shared__ uint32_t sS[BATCH];
 
@@ -306,7 +307,7 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendArgs a) {
                     const SplatRec& r = a.rec[slot];
                     sA[j] = r.a;
                     sB[j] = r.b;
-                    sC[j] = r.c.x;
+                    sC[j] = make_float4(r.c.x, r.d.x, r.d.y, r.d.z);
                     m = quadrant_mask<Q>(r, tc.tx * TS, tc.ty * TS);
                 }
                 sM[j] = m;
@@ -321,8 +322,8 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendArgs a) {
             for (int k = 0; k < Q; ++k)
                 if (idx >= qmax[k]) m &= ~(1u << k);
             if (m == 0) continue;
-            const float4 A = sA[j], B = sB[j];
-            const float cb = sC[j];
+            const float4 A = sA[j], B = sB[j], Cc = sC[j];
+            const float cb = Cc.x;
             float r[NG];
 #pragma unroll
             for (int g = 0; g < NG; ++g) r[g] = 0.0f;
@@ -348,8 +349,8 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendArgs a) {
                     const float dL_dG = B.y * dL_dalpha;
                     const float gdx = G * dx, gdy = G * dy;
                     // dL/dmean2d = dL/dG * dG/d(dx, dy), dG/ddx = -G (a dx + b dy), dG/ddy = -G (b dx + c dy)
-                    const float ddx = -dL_dG * fmaf(gdx, A.z, gdy * A.w);
-                    const float ddy = -dL_dG * fmaf(gdy, B.x, gdx * A.w);
+                    const float ddx = -dL_dG * fmaf(gdx, Cc.y, gdy * Cc.z);
+                    const float ddy = -dL_dG * fmaf(gdy, Cc.w, gdx * Cc.z);
                     r[0] += ddx;
                     r[1] += ddy;
                     // dL/dconic = -dL/dG * G * (dx^2, dx dy, dy^2) / 2

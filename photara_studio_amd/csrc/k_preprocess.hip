@@ -127,11 +127,13 @@ __global__ __launch_bounds__(256) void k_preprocess(PreArgs a, int64_t n_slots) 
                 }
                 float ex, ey, qmax;
                 patch_extent(p.conic[0], p.conic[1], p.conic[2], p.opacity, ex, ey, qmax);
+                float a2, b2, c2;
+                conic_log2(p.conic[0], p.conic[1], p.conic[2], a2, b2, c2);
                 SplatRec& r = a.rec[t];
-                r.a = make_float4(p.mean2d[0], p.mean2d[1], p.conic[0], p.conic[1]);
-                r.b = make_float4(p.conic[2], p.opacity, rgb[0], rgb[1]);
+                r.a = make_float4(p.mean2d[0], p.mean2d[1], a2, b2);
+                r.b = make_float4(c2, p.opacity, rgb[0], rgb[1]);
                 r.c = make_float4(rgb[2], ex, ey, qmax);
-                r.d = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                r.d = make_float4(p.conic[0], p.conic[1], p.conic[2], p.opacity);
                 a.meta[t] = make_uint4(0u, f2u(p.depth), touched, clamp | (fp64 ? 8u : 0u));
                 float4* sg = reinterpret_cast<float4*>(a.sgrad + grad_offset(t, a.vp_log2, a.g.count));
                 sg[0] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -192,10 +194,10 @@ __global__ __launch_bounds__(256) void k_emit(const uint32_t* sorted_slots, cons
         tw.ok = false;
         uint32_t tb = 0, gx = 0;
         if (valid && !over) {
-            const float4 A = rec[slot].a, B = rec[slot].b;
+            const float4 A = rec[slot].a, D = rec[slot].d;
             const ViewInfo& vi = views[slot & vmask].vi;
-            const float conic[3] = {A.z, A.w, B.x};
-            tw = tile_walk_setup(A.x, A.y, conic, B.y, vi.grid_x, vi.grid_y, ts);
+            const float conic[3] = {D.x, D.y, D.z};
+            tw = tile_walk_setup(A.x, A.y, conic, D.w, vi.grid_x, vi.grid_y, ts);
             tb = uint32_t(vi.tile_base);
             gx = uint32_t(vi.grid_x);
         }

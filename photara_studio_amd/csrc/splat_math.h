@@ -31,9 +31,6 @@ template <class T> struct K {
     static constexpr T alpha_floor() { return T(1) / T(255); }
     static constexpr T transmittance_floor() { return T(1.0e-4); }
 };
-// alpha = min(clip, o * exp(p)) < 1/255 for every o <= 1 once p < -5.6 (exp(-5.6) = 0.0037).
-// Used only to skip the exp of far pixels; the decision it shortcuts is unchanged.
-constexpr float kSkipPower = -5.6f;
 
 // ---------------------------------------------------------------- scalar helpers
 PHT_HD uint32_t f2u(float f) { return __builtin_bit_cast(uint32_t, f); }
@@ -79,25 +76,6 @@ PHT_HD float pexpf(float x) {
     return (p * u2f(uint32_t(k + 127 + 64) << 23)) * 5.42101086242752217e-20f; // 2^-64
 }
 
-// pexpf restricted to x in [-104, 88.7] with 2^k normal (the blend's powers lie in
-// [kSkipPower, 0]): the same operations as pexpf, so identical results there, without the
-// range branches. NaN propagates.
-PHT_HD float pexpf_inrange(float x) {
-    const float kf = rintf(x * 1.44269502162933350f);
-    float r = fma_(kf, -0.693147182464599609375f, x);
-    r = fma_(kf, 1.9046542121259336e-09f, r);
-    float p = 1.98412701138295233e-04f;
-    p = fma_(p, r, 1.38888892251998186e-03f);
-    p = fma_(p, r, 8.33333376795053482e-03f);
-    p = fma_(p, r, 4.16666679084300995e-02f);
-    p = fma_(p, r, 1.66666671633720398e-01f);
-    p = fma_(p, r, 0.5f);
-    p = fma_(p, r, 1.0f);
-    p = fma_(p, r, 1.0f);
-    const int k = x == x ? int(kf) : 0;
-    return p * u2f(uint32_t(k + 127) << 23);
-}
-
 // log(x): x = 2^e * m with m in [sqrt(1/2), sqrt(2)), log(m) = 2 atanh(s), s = (m-1)/(m+1),
 // atanh series to s^9 (|s| <= 0.1716, truncation < 1e-9 relative). ~1-2 ulp.
 PHT_HD float plogf(float x) {
@@ -124,9 +102,6 @@ PHT_HD float plogf(float x) {
 
 PHT_HD float exp_(float x) { return pexpf(x); }
 PHT_HD double exp_(double x) { return exp(x); }
-// exp for arguments already known to lie in [kSkipPower, 0] (or NaN)
-PHT_HD float exp_blend_(float x) { return pexpf_inrange(x); }
-PHT_HD double exp_blend_(double x) { return exp(x); }
 PHT_HD float log_(float x) { return plogf(x); }
 PHT_HD double log_(double x) { return log(x); }
 
@@ -629,24 +604,64 @@ PHT_HD void sh_backward(int degree, const T raw_dir[3], const T d[3], const T* c
 }
 
 // ---------------------------------------------------------------- per-pixel blend term
-// geo::gaussian_power (geometry.cuh:26-33) with its fixed rounding order.
-template <class T> PHT_HD T gaussian_power(T ca, T cb, T cc, T dx, T dy) {
-    const T xx = ca * dx;
-    const T yy = (cc * dy) * dy;
-    const T xy = (cb * dx) * dy;
-    return T(-0.5) * fma_(dx, xx, yy) - xy;
+// The blend evaluates each pair's Gaussian in base 2: with the conic scaled once per splat,
+// (a2, b2, c2) = (-log2(e)/2 a, -log2(e) b, -log2(e)/2 c), the exponent
+// p2 = a2 dx^2 + 2 (b2 / 2) dx dy + c2 dy^2 = log2(e) * power, and G = 2^p2 = exp(power).
+// Same value as the reference's exp(power) (geometry.cuh:26-33, render_forward.cu:380-388);
+// fewer operations per pair (one multiply per conic term is moved out of the pair loop, and the
+// exponential needs no ln 2 reduction).
+template <class T> struct Log2Consts {
+    static constexpr T neg_half_log2e = T(-0.72134752044448170368);
+    static constexpr T neg_log2e = T(-1.44269504088896340736);
+};
+template <class T> PHT_HD void conic_log2(T a, T b, T c, T& a2, T& b2, T& c2) {
+    a2 = Log2Consts<T>::neg_half_log2e * a;
+    b2 = Log2Consts<T>::neg_log2e * b;
+    c2 = Log2Consts<T>::neg_half_log2e * c;
 }
+template <class T> PHT_HD T pair_power2(T a2, T b2, T c2, T dx, T dy) {
+    return fma_(dx, fma_(a2, dx, b2 * dy), (c2 * dy) * dy);
+}
+// Skip bound in base-2 units: alpha = op 2^p2 >= 1/255 needs p2 >= log2(1/255) = -7.994 (op <= 1).
+constexpr float kSkipPower2 = -8.0f;
 
-// Alpha of one (instance, pixel) pair as in blend_tile (render_forward.cu:380-388).
-// Returns false when the pair is skipped (power > 0 or alpha < 1/255); G = exp(power).
+// 2^x for x in [-8, 0] (or NaN): x = k + r, |r| <= 1/2, a degree-6 polynomial for 2^r (fitted
+// to minimise the relative error on [-1/2, 1/2]; <= 0.94 ulp in float Horner form), exact scaling
+// by 2^k. NaN propagates.
+PHT_HD float pexp2f_blend(float x) {
+    const float kf = rintf(x);
+    const float r = x - kf;
+    float p = 0.00015345800784416497f;
+    p = fma_(p, r, 0.0013399932067841291f);
+    p = fma_(p, r, 0.009618489071726799f);
+    p = fma_(p, r, 0.05550328642129898f);
+    p = fma_(p, r, 0.24022646248340607f);
+    p = fma_(p, r, 0.6931471824645996f);
+    p = fma_(p, r, 1.0f);
+    const int k = x == x ? int(kf) : 0;
+    return p * u2f(uint32_t(k + 127) << 23);
+}
+PHT_HD float exp2_blend_(float x) { return pexp2f_blend(x); }
+PHT_HD double exp2_blend_(double x) { return exp2(x); }
+
+// Alpha of one (instance, pixel) pair as in blend_tile (render_forward.cu:380-388), from the
+// splat's base-2 conic. Returns false when the pair is skipped (exponent > 0 or alpha < 1/255);
+// G = exp(power).
 template <class T>
-PHT_HD bool pair_alpha(T ca, T cb, T cc, T op, T dx, T dy, T alpha_clip, T& alpha, T& G) {
-    const T power = gaussian_power(ca, cb, cc, dx, dy);
-    if (power > T(0)) return false;
-    if (power < T(kSkipPower)) return false;
-    G = exp_blend_(power);
+PHT_HD bool pair_alpha2(T a2, T b2, T c2, T op, T dx, T dy, T alpha_clip, T& alpha, T& G) {
+    const T p2 = pair_power2(a2, b2, c2, dx, dy);
+    if (p2 > T(0)) return false;
+    if (p2 < T(kSkipPower2)) return false;
+    G = exp2_blend_(p2);
     alpha = min_(alpha_clip, op * G);
     return !(alpha < K<T>::alpha_floor());
+}
+// The same from the splat's conic (a, b, c): the oracle's form.
+template <class T>
+PHT_HD bool pair_alpha(T ca, T cb, T cc, T op, T dx, T dy, T alpha_clip, T& alpha, T& G) {
+    T a2, b2, c2;
+    conic_log2(ca, cb, cc, a2, b2, c2);
+    return pair_alpha2(a2, b2, c2, op, dx, dy, alpha_clip, alpha, G);
 }
 
 // ---------------------------------------------------------------- per-Gaussian backward

@@ -145,9 +145,9 @@ def test_tile_walk_covers_every_contributing_pixel(oracle, ts):
         tiles = oracle.enumerate_tiles(mean, c3, op, gx, gy, ts)
         assert len(tiles) == len(set(tiles))
         d = (mean[None, :] - pix).astype(np.float32)
-        power = oracle.gaussian_power(np.tile(c3, (d.shape[0], 1)), d)
-        alpha = np.minimum(np.float32(0.99), op * oracle.pexpf(power))
-        hit = (power <= 0) & (alpha >= np.float32(1.0 / 255.0))
+        p2 = oracle.pair_power2(np.tile(c3, (d.shape[0], 1)), d)
+        alpha = np.minimum(np.float32(0.99), op * oracle.pexp2f_blend(np.maximum(p2, np.float32(-126.0))))
+        hit = (p2 <= 0) & (alpha >= np.float32(1.0 / 255.0))
         need = set(((pix[hit, 1].astype(int) // ts) * gx + pix[hit, 0].astype(int) // ts).tolist())
         assert need <= set(tiles), (trial, sorted(need - set(tiles)))
         thr = 2.0 * math.log(float(op) * 255.0)

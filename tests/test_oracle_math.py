@@ -31,15 +31,38 @@ def test_pexpf_error_bound(oracle):
     assert np.abs(ys.astype(np.float64) - want).max() <= 2 * 1.4e-45
 
 
-def test_pexpf_inrange_equals_pexpf_on_blend_powers(oracle):
-    """The blend's branch-free exp is bit-identical to pexpf on [-5.6, 0] (the powers that
-    reach it): every 128th float32 bit pattern of the interval (8.5M values) plus its ends."""
-    lo_bits = np.float32(-5.6).view(np.uint32)  # negative floats: larger bits = more negative
-    neg = np.arange(0x80000000, lo_bits + 1, 1 << 7, dtype=np.uint64).astype(np.uint32).view(np.float32)
-    x = np.concatenate([neg, np.array([0.0, -0.0, np.float32(-5.6)], np.float32)])
-    a, b = oracle.pexpf(x), oracle.pexpf_inrange(x)
-    np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
-    assert np.isnan(oracle.pexpf_inrange(np.array([np.nan], np.float32))[0])
+def test_pexp2f_blend_error_bound(oracle):
+    """The blend's 2^x on the exponents that reach it ([-8, 0]): every 64th float32 bit pattern
+    of the interval (16M values) plus the ends and the integers, within 1.5 ulp of float64 2^x;
+    2^integer exact; NaN propagates."""
+    lo_bits = np.float32(-8.0).view(np.uint32)  # negative floats: larger bits = more negative
+    neg = np.arange(0x80000000, lo_bits + 1, 1 << 6, dtype=np.uint64).astype(np.uint32).view(np.float32)
+    ints = -np.arange(0, 9, dtype=np.float32)
+    x = np.concatenate([neg, ints, np.array([0.0, -0.0], np.float32)])
+    y = oracle.pexp2f_blend(x).astype(np.float64)
+    want = np.exp2(x.astype(np.float64))
+    ulp = np.abs(y - want) / np.spacing(want.astype(np.float32)).astype(np.float64)
+    assert ulp.max() <= 1.5, float(ulp.max())
+    np.testing.assert_array_equal(oracle.pexp2f_blend(ints), np.exp2(ints))
+    assert np.isnan(oracle.pexp2f_blend(np.array([np.nan], np.float32))[0])
+
+
+def test_pair_power2_is_log2e_times_the_power(oracle):
+    """The base-2 exponent of a pair equals log2(e) * (-q/2), q = a dx^2 + 2 b dx dy + c dy^2,
+    evaluated exactly in float64 from the same float32 inputs, to float rounding."""
+    rng = np.random.default_rng(3)
+    n = 200000
+    c3 = np.stack([rng.uniform(0.001, 4.0, n), rng.uniform(-0.5, 0.5, n), rng.uniform(0.001, 4.0, n)], 1)
+    c3[:, 1] *= np.sqrt(c3[:, 0] * c3[:, 2])  # positive definite
+    c3 = c3.astype(np.float32)
+    d = rng.uniform(-12, 12, (n, 2)).astype(np.float32)
+    got = oracle.pair_power2(c3, d).astype(np.float64)
+    a, b, c = (c3[:, k].astype(np.float64) for k in range(3))
+    dx, dy = d[:, 0].astype(np.float64), d[:, 1].astype(np.float64)
+    q = a * dx * dx + 2 * b * dx * dy + c * dy * dy
+    want = -0.5 * np.log2(np.e) * q
+    scale = 0.5 * np.log2(np.e) * (a * dx * dx + 2 * np.abs(b * dx * dy) + c * dy * dy)
+    assert np.all(np.abs(got - want) <= 4 * 2.0 ** -24 * scale + 1e-30)
 
 
 def test_plogf_error_bound(oracle):
@@ -56,22 +79,6 @@ def test_plogf_error_bound(oracle):
     assert oracle.plogf(np.array([1.0], np.float32))[0] == 0.0
     sp = oracle.plogf(np.array([0.0, -1.0, np.inf], np.float32))
     assert sp[0] == -np.inf and np.isnan(sp[1]) and sp[2] == np.inf
-
-
-def test_gaussian_power_matches_exact(oracle):
-    """-1/2 (a dx^2 + c dy^2) - b dx dy, fixed rounding order (geometry.cuh:26-33)."""
-    rng = np.random.default_rng(2)
-    n = 100000
-    a = rng.uniform(0.01, 2.0, n)
-    c = rng.uniform(0.01, 2.0, n)
-    b = rng.uniform(-1, 1, n) * np.sqrt(a * c) * 0.99
-    conic = np.stack([a, b, c], 1).astype(np.float32)
-    d = rng.uniform(-8, 8, (n, 2)).astype(np.float32)
-    got = oracle.gaussian_power(conic, d).astype(np.float64)
-    c64, d64 = conic.astype(np.float64), d.astype(np.float64)
-    want = -0.5 * (c64[:, 0] * d64[:, 0] ** 2 + c64[:, 2] * d64[:, 1] ** 2) - c64[:, 1] * d64[:, 0] * d64[:, 1]
-    scale = 0.5 * (c64[:, 0] * d64[:, 0] ** 2 + c64[:, 2] * d64[:, 1] ** 2) + np.abs(c64[:, 1] * d64[:, 0] * d64[:, 1])
-    assert (np.abs(got - want) <= 4e-7 * scale + 1e-30).all()
 
 
 def test_reduced_precision_conversions_match_torch(oracle):
