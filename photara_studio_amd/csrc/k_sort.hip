@@ -135,10 +135,10 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_write(Src src, Sink sink,
 // downsweep ranks tiles of kSortTile keys in LDS: each wave ranks its 1024 keys row by row with a
 // wave-level multi-split (8 ballots) against per-wave running digit counters, the tile is
 // reordered by digit in LDS, and each digit's run is written out contiguously. A block of BT
-// threads ranks BT * 16 keys per iteration: 256 threads for the depth sort (three payload arrays
-// in LDS), 512 for the tile sort, whose 8192-key tiles give each digit runs of 32 keys on average,
-// i.e. whole 128-byte lines per digit and array in the write-out.
-constexpr int kSortRows = 16; // keys per lane and tile
+// threads ranks BT * R keys per iteration (R keys per lane): the tile sort uses 512 x 16, whose
+// 8192-key tiles give each digit runs of 32 keys on average (whole 128-byte lines per digit and
+// array in the write-out); the depth sort, with a third array in LDS, 512 x 8, which keeps two
+// blocks (16 waves) resident per CU.
 
 __global__ __launch_bounds__(kScanThreads) void k_radix_upsweep(const uint32_t* keys, const uint32_t* n_ptr,
                                                                 uint32_t n_const, uint32_t cap, int shift, int nb,
@@ -185,7 +185,7 @@ __global__ __launch_bounds__(1024) void k_radix_scan(uint32_t* hist, int nb, uin
     if (t == 0) digit_total[blockIdx.x] = total;
 }
 
-template <bool AUX, int BT>
+template <bool AUX, int BT, int R>
 __global__ __launch_bounds__(BT) void k_radix_downsweep(const uint32_t* keys_in, const uint32_t* vals_in,
                                                                   const uint32_t* aux_in, uint32_t* keys_out,
                                                                   uint32_t* vals_out, uint32_t* aux_out,
@@ -193,7 +193,7 @@ __global__ __launch_bounds__(BT) void k_radix_downsweep(const uint32_t* keys_in,
                                                                   uint32_t cap, int shift, int nb,
                                                                   const uint32_t* hist, const uint32_t* digit_total) {
     constexpr int NW = BT / 64;
-    constexpr int kSortTile = BT * kSortRows;
+    constexpr int kSortTile = BT * R;
     __shared__ uint32_t s_key[kSortTile];
     __shared__ uint32_t s_val[kSortTile];
     __shared__ uint32_t s_aux[AUX ? kSortTile : 1];
@@ -217,11 +217,11 @@ __global__ __launch_bounds__(BT) void k_radix_downsweep(const uint32_t* keys_in,
     const uint64_t lt = __lanemask_lt();
     // The next tile's keys are loaded into the same registers as soon as the current tile has
     // been reordered into LDS, so its global loads overlap the current tile's write-out.
-    uint32_t key[kSortRows], val[kSortRows], aux[kSortRows], rank[kSortRows];
+    uint32_t key[R], val[R], aux[R], rank[R];
     auto load_tile = [&](uint32_t b) {
         const uint32_t wb = b + uint32_t(w) * (kSortTile / NW);
 #pragma unroll
-        for (int r = 0; r < kSortRows; ++r) {
+        for (int r = 0; r < R; ++r) {
             const uint32_t i = wb + uint32_t(r) * 64u + uint32_t(lane);
             const bool valid = i < hi;
             key[r] = valid ? keys_in[i] : 0u;
@@ -235,7 +235,7 @@ __global__ __launch_bounds__(BT) void k_radix_downsweep(const uint32_t* keys_in,
         __syncthreads();
         const uint32_t wbase = base + uint32_t(w) * (kSortTile / NW);
 #pragma unroll
-        for (int r = 0; r < kSortRows; ++r) {
+        for (int r = 0; r < R; ++r) {
             const uint32_t i = wbase + uint32_t(r) * 64u + uint32_t(lane);
             const bool valid = i < hi;
             const uint32_t d = (key[r] >> shift) & 255u;
@@ -273,7 +273,7 @@ __global__ __launch_bounds__(BT) void k_radix_downsweep(const uint32_t* keys_in,
         }
         __syncthreads();
 #pragma unroll
-        for (int r = 0; r < kSortRows; ++r) {
+        for (int r = 0; r < R; ++r) {
             const uint32_t i = wbase + uint32_t(r) * 64u + uint32_t(lane);
             if (i < hi) {
                 const uint32_t d = (key[r] >> shift) & 255u;
@@ -337,19 +337,19 @@ void launch_tile_offsets(const uint32_t* sorted_tiles, const uint32_t* visible_p
 
 int launch_radix_sort(uint32_t* keys[2], uint32_t* vals[2], uint32_t* aux[2], const uint32_t* n_ptr, uint32_t cap,
                       int bits, uint32_t* hist, uint32_t* digit_total, hipStream_t st) {
-    // the key-value sort takes 512-thread blocks: two resident per CU, so at most 512 chunks
-    const int nb = aux ? blocks_for(cap) : (blocks_for(cap) < 512 ? blocks_for(cap) : 512);
+    // 512-thread blocks, two resident per CU: at most 512 chunks
+    const int nb = blocks_for(cap) < 512 ? blocks_for(cap) : 512;
     int cur = 0;
     for (int shift = 0; shift < bits; shift += 8) {
         hipLaunchKernelGGL(k_radix_upsweep, dim3(nb), dim3(kScanThreads), 0, st, keys[cur], n_ptr, 0u, cap, shift,
                            nb, hist);
         hipLaunchKernelGGL(k_radix_scan, dim3(256), dim3(1024), 0, st, hist, nb, digit_total);
         if (aux)
-            hipLaunchKernelGGL((k_radix_downsweep<true, 256>), dim3(nb), dim3(256), 0, st, keys[cur], vals[cur],
+            hipLaunchKernelGGL((k_radix_downsweep<true, 512, 8>), dim3(nb), dim3(512), 0, st, keys[cur], vals[cur],
                                aux[cur], keys[cur ^ 1], vals[cur ^ 1], aux[cur ^ 1], n_ptr, 0u, cap, shift, nb,
                                hist, digit_total);
         else
-            hipLaunchKernelGGL((k_radix_downsweep<false, 512>), dim3(nb), dim3(512), 0, st, keys[cur], vals[cur],
+            hipLaunchKernelGGL((k_radix_downsweep<false, 512, 16>), dim3(nb), dim3(512), 0, st, keys[cur], vals[cur],
                                nullptr, keys[cur ^ 1], vals[cur ^ 1], nullptr, n_ptr, 0u, cap, shift, nb, hist,
                                digit_total);
         cur ^= 1;
