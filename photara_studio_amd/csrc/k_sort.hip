@@ -150,16 +150,26 @@ __global__ __launch_bounds__(kScanThreads) void k_radix_upsweep(const uint32_t* 
     const uint32_t n = load_count(n_ptr, n_const, cap);
     uint32_t lo, hi;
     chunk_of(n, nb, blockIdx.x, lo, hi);
-    // lo is a multiple of 256 keys, so the uint4 view of [lo, hi4) is 16-byte aligned
+    // lo is a multiple of 256 keys, so the uint4 view of [lo, hi4) is 16-byte aligned; four
+    // uint4 loads per thread are issued before any of them is counted (bytes in flight)
     const uint32_t hi4 = lo + ((hi - lo) & ~3u);
     const uint4* k4 = reinterpret_cast<const uint4*>(keys);
-    for (uint32_t i = lo + 4u * t; i < hi4; i += 4u * kScanThreads) {
-        const uint4 q = k4[i >> 2];
+    constexpr uint32_t kStep = 4u * kScanThreads;
+    auto count4 = [&](const uint4& q) {
         atomicAdd(&s_h[w][(q.x >> shift) & 255u], 1u);
         atomicAdd(&s_h[w][(q.y >> shift) & 255u], 1u);
         atomicAdd(&s_h[w][(q.z >> shift) & 255u], 1u);
         atomicAdd(&s_h[w][(q.w >> shift) & 255u], 1u);
+    };
+    uint32_t i = lo + 4u * t;
+    for (; i + 3u * kStep < hi4; i += 4u * kStep) {
+        uint4 q[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) q[u] = k4[(i + uint32_t(u) * kStep) >> 2];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) count4(q[u]);
     }
+    for (; i < hi4; i += kStep) count4(k4[i >> 2]);
     for (uint32_t i = hi4 + t; i < hi; i += kScanThreads) atomicAdd(&s_h[w][(keys[i] >> shift) & 255u], 1u);
     __syncthreads();
     hist[size_t(t) * nb + blockIdx.x] = (s_h[0][t] + s_h[1][t]) + (s_h[2][t] + s_h[3][t]);
