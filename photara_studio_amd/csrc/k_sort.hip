@@ -132,8 +132,8 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_write(Src src, Sink sink,
 // ---------------------------------------------------------------- radix sort (8-bit digits)
 // One pass = upsweep (per-block digit histogram of its chunk), digit scan (one block per digit
 // scans that digit's counts over the blocks and stores the digit total) and downsweep. The
-// downsweep ranks tiles of kSortTile keys in LDS: each wave ranks its 1024 keys row by row with a
-// wave-level multi-split (8 ballots) against per-wave running digit counters, the tile is
+// downsweep ranks tiles of kSortTile keys in LDS: each wave ranks its keys row by row (the lanes
+// sharing a digit found through an LDS lane-mask table) against per-wave running digit counters, the tile is
 // reordered by digit in LDS, and each digit's run is written out contiguously. A block of BT
 // threads ranks BT * R keys per iteration (R keys per lane): the tile sort uses 512 x 16, whose
 // 8192-key tiles give each digit runs of 32 keys on average (whole 128-byte lines per digit and
@@ -204,7 +204,8 @@ __global__ __launch_bounds__(BT) void k_radix_downsweep(const uint32_t* keys_in,
                                                                   const uint32_t* hist, const uint32_t* digit_total) {
     constexpr int NW = BT / 64;
     constexpr int kSortTile = BT * R;
-    __shared__ uint32_t s_key[kSortTile];
+    static_assert(kSortTile * 4 >= BT / 64 * 256 * 8, "the digit-match tables live in s_key");
+    __shared__ __align__(8) uint32_t s_key[kSortTile];
     __shared__ uint32_t s_val[kSortTile];
     __shared__ uint32_t s_aux[AUX ? kSortTile : 1];
     __shared__ uint32_t s_cnt[NW][256]; // per-wave running digit counts, then per-wave offsets
@@ -240,8 +241,13 @@ __global__ __launch_bounds__(BT) void k_radix_downsweep(const uint32_t* keys_in,
         }
     };
     if (lo < hi) load_tile(lo);
+    // per-wave digit-match tables (256 lane masks), in s_key's storage while the tile is ranked
+    uint64_t* s_tab = reinterpret_cast<uint64_t*>(s_key) + w * 256;
     for (uint32_t base = lo; base < hi; base += kSortTile) {
-        for (int k = t; k < NW * 256; k += BT) (&s_cnt[0][0])[k] = 0;
+        for (int k = t; k < NW * 256; k += BT) {
+            (&s_cnt[0][0])[k] = 0;
+            reinterpret_cast<uint64_t*>(s_key)[k] = 0ull;
+        }
         __syncthreads();
         const uint32_t wbase = base + uint32_t(w) * (kSortTile / NW);
 #pragma unroll
@@ -249,12 +255,14 @@ __global__ __launch_bounds__(BT) void k_radix_downsweep(const uint32_t* keys_in,
             const uint32_t i = wbase + uint32_t(r) * 64u + uint32_t(lane);
             const bool valid = i < hi;
             const uint32_t d = (key[r] >> shift) & 255u;
-            uint64_t peers = __ballot(valid);
-#pragma unroll
-            for (int bit = 0; bit < 8; ++bit) {
-                const bool set = (d >> bit) & 1u;
-                const uint64_t bal = __ballot(valid && set);
-                peers &= set ? bal : ~bal;
+            // lanes of this row holding digit d: each lane ORs its bit into the wave's table entry,
+            // reads the entry back and clears it (the wave's LDS operations complete in order;
+            // OR is order-free, so the mask and the ranks below do not depend on atomic order)
+            uint64_t peers = 0;
+            if (valid) {
+                atomicOr(reinterpret_cast<unsigned long long*>(&s_tab[d]), 1ull << lane);
+                peers = s_tab[d];
+                s_tab[d] = 0ull;
             }
             const uint32_t before = valid ? s_cnt[w][d] : 0u;
             rank[r] = before + uint32_t(__popcll(peers & lt));
