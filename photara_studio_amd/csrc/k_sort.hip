@@ -138,7 +138,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_write(Src src, Sink sink,
 // threads ranks BT * R keys per iteration (R keys per lane): the tile sort uses 1024 x 16, whose
 // 16384-key tiles give each digit runs of 64 keys on average (whole 128-byte lines per digit and
 // array in the write-out, and half the per-tile bookkeeping of 8192-key tiles); the depth sort,
-// with a third array in LDS, 512 x 8, which keeps two blocks (16 waves) resident per CU.
+// with a third array in LDS, 1024 x 8.
 
 __global__ __launch_bounds__(kScanThreads) void k_radix_upsweep(const uint32_t* keys, const uint32_t* n_ptr,
                                                                 uint32_t n_const, uint32_t cap, int shift, int nb,
@@ -355,17 +355,15 @@ void launch_tile_offsets(const uint32_t* sorted_tiles, const uint32_t* visible_p
 
 int launch_radix_sort(uint32_t* keys[2], uint32_t* vals[2], uint32_t* aux[2], const uint32_t* n_ptr, uint32_t cap,
                       int bits, uint32_t* hist, uint32_t* digit_total, hipStream_t st) {
-    // depth sort: 512-thread blocks, two resident per CU (at most 512 chunks); tile sort:
-    // 1024-thread blocks with 16384-key tiles, one resident per CU (at most 256 chunks)
-    const int cap_blocks = aux ? 512 : 256;
-    const int nb = blocks_for(cap) < cap_blocks ? blocks_for(cap) : cap_blocks;
+    // 1024-thread blocks, one resident per CU: at most 256 chunks
+    const int nb = blocks_for(cap) < 256 ? blocks_for(cap) : 256;
     int cur = 0;
     for (int shift = 0; shift < bits; shift += 8) {
         hipLaunchKernelGGL(k_radix_upsweep, dim3(nb), dim3(kScanThreads), 0, st, keys[cur], n_ptr, 0u, cap, shift,
                            nb, hist);
         hipLaunchKernelGGL(k_radix_scan, dim3(256), dim3(1024), 0, st, hist, nb, digit_total);
         if (aux)
-            hipLaunchKernelGGL((k_radix_downsweep<true, 512, 8>), dim3(nb), dim3(512), 0, st, keys[cur], vals[cur],
+            hipLaunchKernelGGL((k_radix_downsweep<true, 1024, 8>), dim3(nb), dim3(1024), 0, st, keys[cur], vals[cur],
                                aux[cur], keys[cur ^ 1], vals[cur ^ 1], aux[cur ^ 1], n_ptr, 0u, cap, shift, nb,
                                hist, digit_total);
         else
