@@ -267,7 +267,6 @@ int pht_forward(pht_workspace* ws, const pht_gaussians* g, const pht_camera* cam
     pa.rec = ws->rec;
     pa.meta = ws->meta;
     pa.vis_flag = ws->vis_flag;
-    pa.sgrad = ws->sgrad;
     pa.view_mask = ws->view_mask;
     pa.counters = ws->counters;
     const int64_t n_slots = g->count << ws->vp_log2;

@@ -121,6 +121,16 @@ __device__ __forceinline__ TileCoord tile_of_block(const BlendArgs& a, const Dev
 template <int TS, int BATCH>
 __global__ __launch_bounds__(64) void k_blend_fwd(BlendArgs a) {
     constexpr int Q = (TS / 8) * (TS / 8);
+    {
+        // Clear the splat-gradient records the backward accumulates into (n_views * N records of
+        // 12 floats, view-major): each block streams zeros over its share. The stores retire in
+        // the background of this VALU-bound kernel instead of taking HBM time of their own.
+        const int64_t n4 = int64_t(a.n_views) * a.n_gauss * (kGradStride / 4);
+        const int64_t nb = gridDim.x, b = blockIdx.x;
+        const int64_t lo = n4 * b / nb, hi = n4 * (b + 1) / nb;
+        float4* g4 = reinterpret_cast<float4*>(a.sgrad);
+        for (int64_t j = lo + threadIdx.x; j < hi; j += 64) g4[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
     const DevView* dvp;
     const TileCoord tc = tile_of_block(a, dvp);
     const DevView& dv = *dvp;

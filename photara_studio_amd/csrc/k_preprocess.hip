@@ -135,10 +135,6 @@ __global__ __launch_bounds__(256) void k_preprocess(PreArgs a, int64_t n_slots) 
                 r.c = make_float4(rgb[2], ex, ey, qmax);
                 r.d = make_float4(p.conic[0], p.conic[1], p.conic[2], p.opacity);
                 a.meta[t] = make_uint4(0u, f2u(p.depth), touched, clamp | (fp64 ? 8u : 0u));
-                float4* sg = reinterpret_cast<float4*>(a.sgrad + grad_offset(t, a.vp_log2, a.g.count));
-                sg[0] = make_float4(0.f, 0.f, 0.f, 0.f);
-                sg[1] = make_float4(0.f, 0.f, 0.f, 0.f);
-                sg[2] = make_float4(0.f, 0.f, 0.f, 0.f);
             }
         }
         if (dv.radii) dv.radii[gi] = visible ? radius : 0;

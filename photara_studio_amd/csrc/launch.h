@@ -15,7 +15,6 @@ struct PreArgs {
     SplatRec* rec;
     uint4* meta;
     uint32_t* vis_flag;
-    float* sgrad;
     uint32_t* view_mask;
     Counters* counters;
 };
