@@ -120,7 +120,8 @@ typedef struct pht_model_grads {
     float* opacity_logits; /* [N]   */
     float* sh;             /* [N,K,3] */
     float* colors_precomp; /* [N,3] only when forward used colors_precomp */
-    float* refine_weight;  /* [N] */
+    float* refine_weight;  /* [N]; when NULL (and on the fused-Adam path) the backward skips the
+                            * refine statistic, whose per-splat value then reads 0 */
 } pht_model_grads;
 
 /* Fused Adam over every Gaussian parameter (StructureAdamUpdate rasterizer.hpp:84-104 plus the

@@ -304,6 +304,7 @@ int pht_forward(pht_workspace* ws, const pht_gaussians* g, const pht_camera* cam
     ba.counters = ws->counters;
     ba.slot_cap = ws->slot_cap;
     ba.n_gauss = g->count;
+    ba.want_refine = 0;
     ws->mark(4, 0, st);
     launch_blend_forward(ba, int(tiles), ts, s->batch, st);
     ws->mark(4, 1, st);
@@ -370,6 +371,7 @@ int pht_backward(pht_workspace* ws, const pht_gaussians* g, const pht_view_grads
     ba.counters = ws->counters;
     ba.slot_cap = ws->slot_cap;
     ba.n_gauss = ws->gaussians;
+    ba.want_refine = grads && grads->refine_weight ? 1 : 0;
     ws->mark(5, 0, st);
     launch_blend_backward(ba, int(ws->tiles_total), ws->settings.tile_size, ws->settings.batch, st);
     ws->mark(5, 1, st);

@@ -247,7 +247,7 @@ __global__ __launch_bounds__(64) void k_blend_fwd(BlendArgs a) {
     }
 }
 
-template <int TS, int BATCH>
+template <int TS, int BATCH, bool REFINE>
 __global__ __launch_bounds__(64) void k_blend_bwd(BlendArgs a) {
     constexpr int Q = (TS / 8) * (TS / 8);
     constexpr int NG = 10;
@@ -293,7 +293,7 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendArgs a) {
             const float ga = dv.grad_alpha ? dv.grad_alpha[pid] : 0.0f;
             const float* bg = dv.vi.cam.background;
             dL_dTfin = ((bg[0] * gp0[k] + bg[1] * gp1[k]) + bg[2] * gp2[k]) - ga;
-            refine_den[k] = 1.0f / fmaxf(1.0f - Tfin, 1.0e-5f);
+            if constexpr (REFINE) refine_den[k] = 1.0f / fmaxf(1.0f - Tfin, 1.0e-5f);
         }
         T[k] = Tfin;
         E[k] = Tfin * dL_dTfin;
@@ -373,8 +373,10 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendArgs a) {
                     r[6] = fmaf(wgt, gp0[k], r[6]);
                     r[7] = fmaf(wgt, gp1[k], r[7]);
                     r[8] = fmaf(wgt, gp2[k], r[8]);
-                    const float rx = ddx * fW, ry = ddy * fH;
-                    r[9] = fmaf(__builtin_amdgcn_sqrtf(fmaf(rx, rx, ry * ry)), refine_den[k], r[9]);
+                    if constexpr (REFINE) {
+                        const float rx = ddx * fW, ry = ddy * fH;
+                        r[9] = fmaf(__builtin_amdgcn_sqrtf(fmaf(rx, rx, ry * ry)), refine_den[k], r[9]);
+                    }
                     E[k] = fmaf(c_dot, wgt, E[k]);
                     T[k] = Tb;
                 }
@@ -391,7 +393,8 @@ template <int TS, int BATCH> static void fwd_launch(const BlendArgs& a, int n_ti
     hipLaunchKernelGGL((k_blend_fwd<TS, BATCH>), dim3(n_tiles), dim3(64), 0, st, a);
 }
 template <int TS, int BATCH> static void bwd_launch(const BlendArgs& a, int n_tiles, hipStream_t st) {
-    hipLaunchKernelGGL((k_blend_bwd<TS, BATCH>), dim3(n_tiles), dim3(64), 0, st, a);
+    if (a.want_refine) hipLaunchKernelGGL((k_blend_bwd<TS, BATCH, true>), dim3(n_tiles), dim3(64), 0, st, a);
+    else hipLaunchKernelGGL((k_blend_bwd<TS, BATCH, false>), dim3(n_tiles), dim3(64), 0, st, a);
 }
 
 void launch_blend_forward(const BlendArgs& a, int n_tiles, int tile_size, int batch, hipStream_t st) {

@@ -33,6 +33,7 @@ struct BlendArgs {
     const Counters* counters;
     uint32_t slot_cap;
     int64_t n_gauss;
+    int want_refine; // backward: accumulate the refine statistic (only when a caller reads it)
 };
 
 struct GaussArgs {
