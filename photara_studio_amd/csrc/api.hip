@@ -541,17 +541,27 @@ int pht_step_graph_create(pht_workspace* ws, const pht_gaussians* g, const pht_c
         if (loss_sum) launch_zero_f32(loss_sum, 1, cap_stream);
         rc = pht_forward(ws, g, cams, n_views, s, out, 0, cap_stream);
         std::vector<pht_view_grads> dl(static_cast<size_t>(n_views));
+        L1Views l1{};
         for (int v = 0; rc == PHT_OK && v < n_views; ++v) {
             const int64_t P = int64_t(cams[v].width) * cams[v].height;
-            if (lc.kind == PHT_LOSS_L1_SSIM)
+            if (lc.kind == PHT_LOSS_L1_SSIM) {
                 rc = pht_l1_ssim_loss(out[v].color, targets[v], nullptr, grad_color_bufs[v], 0, loss_sum,
                                       loss_sum ? lc.ssim_scratch[v] : nullptr, cams[v].width, cams[v].height,
                                       lc.ssim_weight, lc.photometric_weight, cap_stream);
-            else
-                rc = pht_l1_loss(out[v].color, targets[v], grad_color_bufs[v], loss_sum, P, lc.photometric_weight,
-                                 cap_stream);
+            } else {
+                if (!out[v].color || !targets[v] || !grad_color_bufs[v] || P <= 0) rc = fail(PHT_EINVAL, "bad L1 view");
+                l1.color[l1.n] = out[v].color;
+                l1.target[l1.n] = targets[v];
+                l1.grad[l1.n] = grad_color_bufs[v];
+                l1.pixels[l1.n] = P;
+                ++l1.n;
+            }
             dl[size_t(v)].grad_color = grad_color_bufs[v];
             dl[size_t(v)].grad_alpha = nullptr;
+        }
+        if (rc == PHT_OK && l1.n > 0) {
+            launch_l1_views(l1, loss_sum, lc.photometric_weight, cap_stream);
+            rc = check_hip(hipGetLastError(), "l1 launch");
         }
         if (rc == PHT_OK) rc = pht_backward(ws, g, dl.data(), nullptr, adam, cap_stream);
         hipGraph_t graph = nullptr;

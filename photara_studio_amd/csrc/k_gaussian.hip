@@ -493,17 +493,18 @@ __device__ __forceinline__ float l1_term(float c, float t, float scale, float& g
     return fabsf(d);
 }
 
+// One view's L1 term over a bounded grid of `nblocks` blocks (this block is `block`).
 template <bool VEC>
-__global__ __launch_bounds__(256) void k_l1(const float* color, const float* target, float* grad, float* loss_sum,
-                                            int64_t pixels, float weight) {
+__device__ __forceinline__ void l1_view(const float* color, const float* target, float* grad, float* loss_sum,
+                                        int64_t pixels, float weight, int64_t block, int64_t nblocks) {
     __shared__ float s_w[4];
     const float inv_p = 1.0f / float(pixels);
     const float scale = (weight * inv_p) / 3.0f;
-    const int64_t stride = int64_t(gridDim.x) * 256;
+    const int64_t stride = nblocks * 256;
     float part = 0.0f;
     if constexpr (VEC) {
         const int64_t n4 = pixels >> 2;
-        for (int64_t q = int64_t(blockIdx.x) * 256 + threadIdx.x; q < n4; q += stride) {
+        for (int64_t q = block * 256 + threadIdx.x; q < n4; q += stride) {
             for (int ch = 0; ch < 3; ++ch) {
                 const int64_t o = (ch * pixels >> 2) + q;
                 const float4 c = reinterpret_cast<const float4*>(color)[o];
@@ -517,7 +518,7 @@ __global__ __launch_bounds__(256) void k_l1(const float* color, const float* tar
             }
         }
     } else {
-        for (int64_t p = int64_t(blockIdx.x) * 256 + threadIdx.x; p < pixels; p += stride) {
+        for (int64_t p = block * 256 + threadIdx.x; p < pixels; p += stride) {
             for (int ch = 0; ch < 3; ++ch) {
                 const int64_t o = ch * pixels + p;
                 float g;
@@ -534,6 +535,19 @@ __global__ __launch_bounds__(256) void k_l1(const float* color, const float* tar
     }
 }
 
+// All views of a step in one launch: grid (blocks per view, views).
+template <bool VEC>
+__global__ __launch_bounds__(256) void k_l1_views(L1Views v, float* loss_sum, float weight) {
+    const int k = blockIdx.y;
+    l1_view<VEC>(v.color[k], v.target[k], v.grad[k], loss_sum, v.pixels[k], weight, blockIdx.x, gridDim.x);
+}
+
+template <bool VEC>
+__global__ __launch_bounds__(256) void k_l1(const float* color, const float* target, float* grad, float* loss_sum,
+                                            int64_t pixels, float weight) {
+    l1_view<VEC>(color, target, grad, loss_sum, pixels, weight, blockIdx.x, gridDim.x);
+}
+
 void launch_l1_loss(const float* color, const float* target, float* grad, float* loss_sum, int64_t pixels,
                     float weight, hipStream_t st) {
     if (pixels <= 0) return;
@@ -544,6 +558,25 @@ void launch_l1_loss(const float* color, const float* target, float* grad, float*
     if (blocks > 512) blocks = 512;
     if (vec) hipLaunchKernelGGL(k_l1<true>, dim3(unsigned(blocks)), dim3(256), 0, st, color, target, grad, loss_sum, pixels, weight);
     else hipLaunchKernelGGL(k_l1<false>, dim3(unsigned(blocks)), dim3(256), 0, st, color, target, grad, loss_sum, pixels, weight);
+}
+
+void launch_l1_views(const L1Views& v, float* loss_sum, float weight, hipStream_t st) {
+    if (v.n <= 0) return;
+    bool vec = true;
+    int64_t most = 0;
+    for (int k = 0; k < v.n; ++k) {
+        vec = vec && (v.pixels[k] & 3) == 0 &&
+              ((reinterpret_cast<uintptr_t>(v.color[k]) | reinterpret_cast<uintptr_t>(v.target[k]) |
+                reinterpret_cast<uintptr_t>(v.grad[k])) & 15u) == 0;
+        most = v.pixels[k] > most ? v.pixels[k] : most;
+    }
+    if (most <= 0) return;
+    const int64_t items = vec ? most >> 2 : most;
+    int64_t blocks = (items + 255) / 256;
+    if (blocks > 128) blocks = 128;
+    const dim3 grid(unsigned(blocks), unsigned(v.n));
+    if (vec) hipLaunchKernelGGL(k_l1_views<true>, grid, dim3(256), 0, st, v, loss_sum, weight);
+    else hipLaunchKernelGGL(k_l1_views<false>, grid, dim3(256), 0, st, v, loss_sum, weight);
 }
 
 __global__ void k_zero_f32(float* p, int n) {

@@ -104,6 +104,15 @@ void launch_l1_ssim(const float* color, const float* target, const float* mask, 
                     float* loss_sum, float* scratch, int width, int height, float ssim_weight,
                     float photometric_weight, hipStream_t st);
 
+// The L1 term of several views in one launch (the captured step).
+struct L1Views {
+    const float* color[PHT_MAX_VIEWS];
+    const float* target[PHT_MAX_VIEWS];
+    float* grad[PHT_MAX_VIEWS];
+    int64_t pixels[PHT_MAX_VIEWS];
+    int n;
+};
+void launch_l1_views(const L1Views& v, float* loss_sum, float weight, hipStream_t st);
 void launch_l1_loss(const float* color, const float* target, float* grad, float* loss_sum, int64_t pixels,
                     float weight, hipStream_t st);
 void launch_zero_f32(float* p, int n, hipStream_t st);
