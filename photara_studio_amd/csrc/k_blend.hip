@@ -347,39 +347,40 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendArgs a) {
                     pair_alpha_flat(A.z, A.w, B.x, B.y, dx, dy, clip, alpha, G) && idx < last[k];
                 if (__ballot(contrib) == 0) continue;
                 any = true;
-                if (contrib) {
-                    // the recovered T and the gradients are not compared bit for bit, so the
-                    // hardware reciprocal / square root (1 ulp) replace the IEEE sequences
-                    // (fused multiply-adds throughout: these sums are compared to a tolerance)
-                    const float inv_1ma = __builtin_amdgcn_rcpf(1.0f - alpha);
-                    const float Tb = T[k] * inv_1ma;
-                    const float wgt = alpha * Tb;
-                    const float c_dot = fmaf(cb, gp2[k], fmaf(B.w, gp1[k], B.z * gp0[k]));
-                    const float dL_dalpha = fmaf(Tb, c_dot, -E[k] * inv_1ma);
-                    const float dL_dG = B.y * dL_dalpha;
-                    const float gdx = G * dx, gdy = G * dy;
-                    // dL/dmean2d = dL/dG * dG/d(dx, dy), dG/ddx = -G (a dx + b dy), dG/ddy = -G (b dx + c dy)
-                    const float ddx = -dL_dG * fmaf(gdx, Cc.y, gdy * Cc.z);
-                    const float ddy = -dL_dG * fmaf(gdy, Cc.w, gdx * Cc.z);
-                    r[0] += ddx;
-                    r[1] += ddy;
-                    // dL/dconic = -dL/dG * G * (dx^2, dx dy, dy^2) / 2
-                    const float h = -0.5f * dL_dG;
-                    const float hx = h * gdx;
-                    r[2] = fmaf(hx, dx, r[2]);
-                    r[3] = fmaf(hx, dy, r[3]);
-                    r[4] = fmaf(h * gdy, dy, r[4]);
-                    r[5] = fmaf(G, dL_dalpha, r[5]);
-                    r[6] = fmaf(wgt, gp0[k], r[6]);
-                    r[7] = fmaf(wgt, gp1[k], r[7]);
-                    r[8] = fmaf(wgt, gp2[k], r[8]);
-                    if constexpr (REFINE) {
-                        const float rx = ddx * fW, ry = ddy * fH;
-                        r[9] = fmaf(__builtin_amdgcn_sqrtf(fmaf(rx, rx, ry * ry)), refine_den[k], r[9]);
-                    }
-                    E[k] = fmaf(c_dot, wgt, E[k]);
-                    T[k] = Tb;
+                // Branch-free over the wave: lanes without a contributing pair zero the three
+                // factors every term is a product of (dL/dalpha, dL/dG, the blend weight) and
+                // their offsets, so their terms are exact zeros and T, E keep their values.
+                // (The recovered T and the gradients are not compared bit for bit: the hardware
+                // reciprocal / square root and fused multiply-adds are used throughout.)
+                const float ox = contrib ? dx : 0.0f, oy = contrib ? dy : 0.0f;
+                const float inv_1ma = __builtin_amdgcn_rcpf(1.0f - alpha);
+                const float Tb = T[k] * inv_1ma;
+                const float wgt = contrib ? alpha * Tb : 0.0f;
+                const float c_dot = fmaf(cb, gp2[k], fmaf(B.w, gp1[k], B.z * gp0[k]));
+                const float dL_dalpha = contrib ? fmaf(Tb, c_dot, -E[k] * inv_1ma) : 0.0f;
+                const float dL_dG = B.y * dL_dalpha;
+                const float gdx = G * ox, gdy = G * oy;
+                // dL/dmean2d = dL/dG * dG/d(dx, dy), dG/ddx = -G (a dx + b dy), dG/ddy = -G (b dx + c dy)
+                const float ddx = -dL_dG * fmaf(gdx, Cc.y, gdy * Cc.z);
+                const float ddy = -dL_dG * fmaf(gdy, Cc.w, gdx * Cc.z);
+                r[0] += ddx;
+                r[1] += ddy;
+                // dL/dconic = -dL/dG * G * (dx^2, dx dy, dy^2) / 2
+                const float h = -0.5f * dL_dG;
+                const float hx = h * gdx;
+                r[2] = fmaf(hx, ox, r[2]);
+                r[3] = fmaf(hx, oy, r[3]);
+                r[4] = fmaf(h * gdy, oy, r[4]);
+                r[5] = fmaf(G, dL_dalpha, r[5]);
+                r[6] = fmaf(wgt, gp0[k], r[6]);
+                r[7] = fmaf(wgt, gp1[k], r[7]);
+                r[8] = fmaf(wgt, gp2[k], r[8]);
+                if constexpr (REFINE) {
+                    const float rx = ddx * fW, ry = ddy * fH;
+                    r[9] = fmaf(__builtin_amdgcn_sqrtf(fmaf(rx, rx, ry * ry)), refine_den[k], r[9]);
                 }
+                E[k] = fmaf(c_dot, wgt, E[k]);
+                T[k] = contrib ? Tb : T[k];
             }
             if (!any) continue;
             const float tot = wave_scatter10(r, lane);
