@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define PHT_ABI_VERSION 2
+#define PHT_ABI_VERSION 3
 #define PHT_MAX_VIEWS 32
 
 /* Status codes. The reference throws std::invalid_argument for bad inputs
@@ -291,6 +291,142 @@ int pht_step_graph_create(pht_workspace* ws, const pht_gaussians* g, const pht_c
                           void* stream, pht_step_graph** out_graph);
 int pht_step_graph_launch(pht_step_graph* graph, void* stream);
 void pht_step_graph_destroy(pht_step_graph* graph);
+
+/* ======================================================================================
+ * Densification (ADC-IGS refinement, photara/src/splat/densification_igs.cpp:291-515): the
+ * per-row device work of a refinement. The host sequences these calls (and reads the few
+ * counts the refinement branches on), as the reference's IgsStrategy::refine does.
+ * ====================================================================================== */
+
+/* detail::DensificationStats (photara/src/splat/adc_ops.cpp:8-19), [N] each. */
+typedef struct pht_densify_stats {
+    float* gradient;
+    float* count;
+    float* max_screen_radius;
+    float* priority;
+    float* max_geometry_gradient;
+    int32_t* first_view;
+    float* view_support;
+    float* accumulated_error;
+} pht_densify_stats;
+
+/* One view's observation for the statistics (accumulate_densification_stats,
+ * photara/src/splat/adc_ops.cpp:23-62; kernel photara/src/splat/cuda_ops.cu:1548-1606).
+ * Optional arrays may be NULL. */
+typedef struct pht_densify_observation {
+    const float* refine_weight;     /* [N] */
+    const float* visibility;        /* [N] */
+    const int32_t* radii;           /* [N] */
+    const float* geometry_gradient; /* [N] or NULL */
+    const float* image_error;       /* [N] or NULL */
+    const float* normalized_size;   /* [N] or NULL: replaces radius / min(width, height) */
+    int32_t width, height;
+    int32_t use_maximum;            /* gradient = max(gradient, w) instead of += w */
+    int32_t require_contribution_visibility;
+    float step_score_power;         /* w = max(refine_weight, 0) ^ power */
+    float oversize_screen_threshold; /* > 0: priority += max(log2(screen / threshold), 0) */
+    int32_t view_index;             /* >= 0: distinct-view support */
+    int32_t reserved;
+} pht_densify_observation;
+
+/* Accumulates n_obs observations (in order: the same as n_obs calls of the reference's
+ * accumulate_densification_stats) into stats in one pass over the N rows. */
+int pht_densify_accumulate(const pht_densify_stats* stats, const pht_densify_observation* obs, int n_obs,
+                           int64_t n, void* stream);
+
+/* prune_masks (photara/src/splat/ops.hpp:131-141; kernel adc_plus_prune_kernel,
+ * photara/src/splat/cuda_ops.cu:263-302) over the FP32 master parameters of g: keep = !hard &&
+ * sigmoid(logit) >= minimum_opacity; hard = any non-finite parameter (SH included, sh_stride
+ * floats per row), a mean outside the box |mean - center| <= maximum_bounds, or max
+ * exp(log_scale) > maximum_bounds; opacities = sigmoid(logit). */
+int pht_densify_prune_masks(const pht_gaussians* g, int32_t sh_stride, float minimum_opacity,
+                            float maximum_bounds, const float center[3], uint8_t* keep, uint8_t* hard,
+                            float* opacities, void* stream);
+
+/* Row selection by radix select over 64-bit keys (no sort of the n rows).
+ *   PHT_SELECT_LARGEST / PHT_SELECT_SMALLEST: the k rows with the largest / smallest value,
+ *     ties to the lower index: select_front's sort-and-take-front (densification_igs.cpp:91-121).
+ *   PHT_SELECT_GUMBEL: weighted sampling without replacement by Gumbel top-k, keys of
+ *     igs_weighted_sample_cuda (photara/src/splat/igs_selection_cuda.cu:31-70): rows with a
+ *     positive finite weight, score log(w) - log(-log(u)), u from splitmix64(seed + golden * (i+1)),
+ *     ties to the higher index.
+ *   PHT_SELECT_LARGEST_POSITIVE: LARGEST over the rows with a positive finite value.
+ * A row is eligible when mask is NULL or mask[i] != 0, exclude is NULL or exclude[i] == 0, its
+ * value is not NaN and (GUMBEL, LARGEST_POSITIVE) positive and finite. k is clamped to the eligible count. Each selected
+ * row gets out[i] = out_value (out may alias mask or exclude). counts (device, optional) receives
+ * {eligible, selected}; threshold (device, optional) the key of the last selected row, whose high
+ * 32 bits give its value for LARGEST / SMALLEST (pht_select_key_value). */
+#define PHT_SELECT_LARGEST 0
+#define PHT_SELECT_SMALLEST 1
+#define PHT_SELECT_GUMBEL 2
+#define PHT_SELECT_LARGEST_POSITIVE 3
+size_t pht_select_workspace_bytes(int64_t n);
+int pht_select_rows(const float* values, const uint8_t* mask, const uint8_t* exclude, int64_t n, int64_t k,
+                    int mode, uint64_t seed, uint8_t* out, uint8_t out_value, uint32_t* counts,
+                    uint64_t* threshold, void* workspace, size_t workspace_bytes, void* stream);
+/* The value encoded in a LARGEST / SMALLEST key (host helper). */
+float pht_select_key_value(uint64_t key, int mode);
+
+/* IGS parent scores over the retained rows r < retained (row src = keep_rows[r], or r when
+ * keep_rows is NULL), densification_igs.cpp:439-497:
+ *   pht_igs_prepare: priority[r] = stats.priority / max(count, 1); positive[r] = count > 0 &&
+ *     priority > 0; counts (device) += {positive, growth candidates (count > 0 && gradient >
+ *     gradient_threshold), oversized (count > 0 && max_screen_radius > screen_threshold)};
+ *   pht_igs_weights: edge = 1 + 0.25 min(max(priority, 0) / median, 10); replacement = opacity *
+ *     edge (candidates); growth = gradient * edge * (oversized ? 2 : 1) (growth candidates) and
+ *     oversize = max_screen_radius * edge (oversized), both only when allow_growth (else 0). */
+int pht_igs_prepare(const pht_densify_stats* stats, const int32_t* keep_rows, int64_t retained,
+                    float gradient_threshold, float screen_threshold, float* priority, uint8_t* positive,
+                    uint32_t* counts, void* stream);
+int pht_igs_weights(const pht_densify_stats* stats, const int32_t* keep_rows, int64_t retained,
+                    const float* opacities, float median, int32_t allow_growth,
+                    float gradient_threshold, float screen_threshold, float* replacement, float* growth,
+                    float* oversize, void* stream);
+
+/* Row remap of model and optimizer arrays: dst[r] = src[rows[r]] (rows[r] < 0: zero row) for
+ * r < n_rows, for each buffer (row_bytes per row, any multiple of 2). Replaces the reference's
+ * index_select / cat / zero_adam_rows sequence (densification_adc_plus.cpp:18-95, 148-196;
+ * zero_adam_rows_kernel, cuda_ops.cu:1407-1421) with one launch. dst must not alias src. */
+#define PHT_MAX_ROW_BUFFERS 24
+typedef struct pht_row_buffer {
+    const void* src;
+    void* dst;
+    int64_t row_bytes;
+} pht_row_buffer;
+int pht_gather_rows(const pht_row_buffer* buffers, int n_buffers, const int32_t* rows, int64_t n_rows,
+                    void* stream);
+
+/* split_gaussians_kernel (photara/src/splat/cuda_ops.cu:1720-1845) in place on FP32 masters:
+ * for j < count, parent row parents[j] and child row child_base + j, both holding the parent's
+ * parameters on entry, receive the split parameters. samples [count,3] normal variates;
+ * screen_sizes [count] or NULL (ADC covariance mode). When lp_format != PHT_LP_NONE the rows'
+ * reduced-precision copies of log-scales and logits are refreshed too. */
+#define PHT_SPLIT_DENSE_TANGENT 0
+#define PHT_SPLIT_ADC_COVARIANCE 1
+#define PHT_SPLIT_IGS_RANDOM 2
+#define PHT_SPLIT_LONG_AXIS 3
+typedef struct pht_split_params {
+    float* means;
+    float* log_scales;
+    float* opacity_logits;
+    const float* quaternions;
+    uint16_t* log_scales_lp;
+    uint16_t* opacity_logits_lp;
+    int32_t lp_format;
+    int32_t mode;
+    float minimum_opacity;
+    float split_at_screen_size;
+    float split_opacity_k;
+    int32_t reserved;
+} pht_split_params;
+int pht_densify_split(const pht_split_params* p, const int32_t* parents, int64_t count, int64_t child_base,
+                      const float* samples, const float* screen_sizes, void* stream);
+
+/* adc_decay_kernel (photara/src/splat/cuda_ops.cu:1847-1861): opacity -= opacity_decay
+ * (clamped to [1e-12, 1 - 1e-12]), log_scales += log_scale_decay, over n rows; refreshes the
+ * reduced-precision copies when given (lp_format != PHT_LP_NONE). */
+int pht_adc_decay(float* log_scales, float* opacity_logits, uint16_t* log_scales_lp, uint16_t* opacity_logits_lp,
+                  int32_t lp_format, int64_t n, float opacity_decay, float log_scale_decay, void* stream);
 
 #ifdef __cplusplus
 }

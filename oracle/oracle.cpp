@@ -1057,4 +1057,96 @@ double oracle_train_step(const pht_gaussians* g, const pht_camera* cams, int n_v
     return loss;
 }
 
+// ---------------------------------------------------------------- densification rows
+// The refinement's per-row steps (photara/src/splat/cuda_ops.cu:263-302, 1548-1606, 1720-1861;
+// igs_selection_cuda.cu:31-70; densification_igs.cpp:439-497), one row at a time over the same
+// shared row functions as the kernels. Selection itself is done by the Python oracle by sorting
+// the keys (a stable ordering, no radix select).
+void oracle_densify_accumulate(const pht_densify_stats* s, const pht_densify_observation* obs, int n_obs, int64_t n) {
+    for (int v = 0; v < n_obs; ++v) {
+        const int32_t side = std::max<int32_t>(1, std::min(obs[v].width, obs[v].height));
+        const float inv = 1.0f / float(side);
+        for (int64_t i = 0; i < n; ++i) densify_accumulate_row(*s, obs[v], inv, i);
+    }
+}
+
+void oracle_prune_masks(const pht_gaussians* g, int sh_stride, float min_opacity, float max_bounds,
+                        const float* center, uint8_t* keep, uint8_t* hard, float* opacities) {
+    for (int64_t i = 0; i < g->count; ++i) {
+        bool k, h;
+        opacities[i] = densify_prune_row(g->means, g->log_scales, g->quaternions, g->opacity_logits, g->sh, sh_stride,
+                                         i, min_opacity, max_bounds, center, k, h);
+        keep[i] = k ? 1 : 0;
+        hard[i] = h ? 1 : 0;
+    }
+}
+
+// Keys of pht_select_rows (0 = not eligible).
+void oracle_select_keys(const float* values, const uint8_t* mask, const uint8_t* exclude, int64_t n, int mode,
+                        uint64_t seed, uint64_t* keys) {
+    for (int64_t i = 0; i < n; ++i) {
+        uint64_t key = 0;
+        if ((mask == nullptr || mask[i] != 0) && (exclude == nullptr || exclude[i] == 0)) {
+            const float v = values[i];
+            if (mode == PHT_SELECT_GUMBEL)
+                key = gumbel_key(v, seed, uint32_t(i));
+            else if (v == v && (mode != PHT_SELECT_LARGEST_POSITIVE || (v > 0.0f && finite_(v))))
+                key = select_value_key(v, mode != PHT_SELECT_SMALLEST, uint32_t(i));
+        }
+        keys[i] = key;
+    }
+}
+
+void oracle_densify_split(const pht_split_params* p, const int32_t* parents, int64_t count, int64_t child_base,
+                          const float* samples, const float* screens) {
+    for (int64_t j = 0; j < count; ++j) {
+        const int64_t parent = parents[j], child = child_base + j;
+        densify_split_row(p->means, p->log_scales, p->opacity_logits, p->quaternions, parent, child, samples + 3 * j,
+                          screens ? screens[j] : 0.0f, screens != nullptr, p->mode, p->minimum_opacity,
+                          p->split_at_screen_size, p->split_opacity_k);
+        if (p->lp_format != PHT_LP_NONE) {
+            for (int k = 0; k < 3; ++k) {
+                p->log_scales_lp[3 * parent + k] = f32_to_lp(p->log_scales[3 * parent + k], p->lp_format);
+                p->log_scales_lp[3 * child + k] = f32_to_lp(p->log_scales[3 * child + k], p->lp_format);
+            }
+            p->opacity_logits_lp[parent] = f32_to_lp(p->opacity_logits[parent], p->lp_format);
+            p->opacity_logits_lp[child] = f32_to_lp(p->opacity_logits[child], p->lp_format);
+        }
+    }
+}
+
+void oracle_adc_decay(float* log_scales, float* logits, uint16_t* ls_lp, uint16_t* logit_lp, int fmt, int64_t n,
+                      float opacity_decay, float log_scale_decay) {
+    for (int64_t i = 0; i < n; ++i) {
+        densify_decay_row(log_scales, logits, i, opacity_decay, log_scale_decay);
+        if (fmt != PHT_LP_NONE) {
+            for (int k = 0; k < 3; ++k) ls_lp[3 * i + k] = f32_to_lp(log_scales[3 * i + k], fmt);
+            logit_lp[i] = f32_to_lp(logits[i], fmt);
+        }
+    }
+}
+
+// pht_igs_prepare + pht_igs_weights over the retained rows.
+void oracle_igs_rows(const pht_densify_stats* s, const int32_t* keep_rows, int64_t retained, const float* opacities,
+                     float median, int allow_growth, float gthr, float sthr, float* priority, uint8_t* positive,
+                     uint32_t* counts, float* replacement, float* growth, float* oversize) {
+    counts[0] = counts[1] = counts[2] = 0;
+    for (int64_t r = 0; r < retained; ++r) {
+        const int64_t src = keep_rows ? keep_rows[r] : r;
+        const IgsRow row = igs_row(*s, src, gthr, sthr);
+        priority[r] = row.priority;
+        positive[r] = row.positive ? 1 : 0;
+        counts[0] += row.positive;
+        counts[1] += row.growth;
+        counts[2] += row.oversized;
+        if (replacement) {
+            float rw, gw, ow;
+            igs_weights_row(*s, src, opacities[src], median, allow_growth != 0, gthr, sthr, rw, gw, ow);
+            replacement[r] = rw;
+            growth[r] = gw;
+            oversize[r] = ow;
+        }
+    }
+}
+
 } // extern "C"

@@ -63,6 +63,15 @@ def load() -> C.CDLL:
         lib.oracle_images.argtypes = [C.c_int, vp, vp]
         lib.oracle_sh_eval.argtypes = [C.c_int, vp, vp, vp, C.c_int64]
         lib.oracle_bruteforce.argtypes = [vp, vp, vp]
+        lib.oracle_densify_accumulate.argtypes = [C.POINTER(_abi.pht_densify_stats),
+                                                  C.POINTER(_abi.pht_densify_observation), C.c_int, C.c_int64]
+        lib.oracle_prune_masks.argtypes = [C.POINTER(_abi.pht_gaussians), C.c_int, C.c_float, C.c_float, vp, vp, vp,
+                                           vp]
+        lib.oracle_select_keys.argtypes = [vp, vp, vp, C.c_int64, C.c_int, C.c_uint64, vp]
+        lib.oracle_densify_split.argtypes = [C.POINTER(_abi.pht_split_params), vp, C.c_int64, C.c_int64, vp, vp]
+        lib.oracle_adc_decay.argtypes = [vp, vp, vp, vp, C.c_int, C.c_int64, C.c_float, C.c_float]
+        lib.oracle_igs_rows.argtypes = [C.POINTER(_abi.pht_densify_stats), vp, C.c_int64, vp, C.c_float, C.c_int,
+                                        C.c_float, C.c_float, vp, vp, vp, vp, vp, vp]
         _lib = lib
     return _lib
 
@@ -442,3 +451,236 @@ class OracleTrainer:
         return float(load().oracle_train_step(C.byref(g), self.cams, len(self.cameras), C.byref(self.settings),
                                               self.outs, self.tp, self.gp, C.byref(a), self.threads, self.loss_kind,
                                               self.ssim_weight, self.photometric_weight))
+
+
+# ---------------------------------------------------------------- densification (ADC-IGS)
+STATS_FIELDS = ("gradient", "count", "max_screen_radius", "priority", "max_geometry_gradient", "first_view",
+                "view_support", "accumulated_error")
+
+
+def make_stats(n: int) -> Dict[str, np.ndarray]:
+    return {k: np.zeros(n, np.int32 if k == "first_view" else np.float32) for k in STATS_FIELDS}
+
+
+def _stats_abi(st: Dict[str, np.ndarray]) -> _abi.pht_densify_stats:
+    s = _abi.pht_densify_stats()
+    for k in STATS_FIELDS:
+        setattr(s, k, st[k].ctypes.data)
+    return s
+
+
+def densify_accumulate(st: Dict[str, np.ndarray], observations: Sequence[dict]) -> None:
+    """observations: dicts with the pht_densify_observation fields (arrays as numpy)."""
+    n = len(st["gradient"])
+    obs = (_abi.pht_densify_observation * len(observations))()
+    keep = []
+    for k, o in enumerate(observations):
+        for name in ("refine_weight", "visibility", "radii", "geometry_gradient", "image_error", "normalized_size"):
+            a = o.get(name)
+            if a is not None:
+                a = np.ascontiguousarray(a, np.int32 if name == "radii" else np.float32)
+                keep.append(a)
+                setattr(obs[k], name, a.ctypes.data)
+        for name in ("width", "height", "use_maximum", "require_contribution_visibility", "view_index"):
+            setattr(obs[k], name, int(o.get(name, -1 if name == "view_index" else 0)))
+        obs[k].step_score_power = float(o.get("step_score_power", 1.0))
+        obs[k].oversize_screen_threshold = float(o.get("oversize_screen_threshold", 0.0))
+    s = _stats_abi(st)
+    load().oracle_densify_accumulate(C.byref(s), obs, len(observations), n)
+
+
+def prune_masks(model: Dict[str, np.ndarray], min_opacity: float, max_bounds: float, center):
+    n = len(model["opacity_logits"])
+    g = _abi.pht_gaussians()
+    g.count = n
+    for k in ("means", "log_scales", "quaternions", "opacity_logits", "sh"):
+        setattr(g, k, model[k].ctypes.data)
+    keep, hard, op = np.zeros(n, np.uint8), np.zeros(n, np.uint8), np.zeros(n, np.float32)
+    c = f32(center)
+    load().oracle_prune_masks(C.byref(g), int(model["sh"].size // max(n, 1)), min_opacity, max_bounds, _p(c), _p(keep),
+                              _p(hard), _p(op))
+    return keep, hard, op
+
+
+def select_keys(values, mask=None, exclude=None, mode=0, seed=0) -> np.ndarray:
+    v = f32(values)
+    keys = np.zeros(v.size, np.uint64)
+    m = None if mask is None else np.ascontiguousarray(mask, np.uint8)
+    e = None if exclude is None else np.ascontiguousarray(exclude, np.uint8)
+    load().oracle_select_keys(_p(v), _p(m), _p(e), v.size, mode, C.c_uint64(seed & 0xFFFFFFFFFFFFFFFF), _p(keys))
+    return keys
+
+
+def select_rows(values, k, mode, mask=None, exclude=None, seed=0):
+    """The k eligible rows with the largest keys, by a full sort (the checker of the radix
+    select): (rows ascending, eligible count, threshold key)."""
+    keys = select_keys(values, mask, exclude, mode, seed)
+    elig = np.flatnonzero(keys != 0)
+    k = min(max(int(k), 0), len(elig))
+    order = elig[np.argsort(keys[elig], kind="stable")[::-1]]
+    rows = np.sort(order[:k])
+    return rows, len(elig), (int(keys[order[k - 1]]) if k else 0)
+
+
+def key_value(key: int, mode: int) -> float:
+    hi = (key >> 32) & 0xFFFFFFFF
+    o = (~hi & 0xFFFFFFFF) if mode == 1 else hi
+    bits = (o ^ 0x80000000) if o & 0x80000000 else (~o & 0xFFFFFFFF)
+    return float(np.array([bits], np.uint32).view(np.float32)[0])
+
+
+def densify_split(model: Dict[str, np.ndarray], parents, child_base, samples, mode, min_opacity, split_at=0.0,
+                  screens=None, split_k=0.0, lp: Optional[Dict[str, np.ndarray]] = None, lp_format: int = 0) -> None:
+    p = _abi.pht_split_params()
+    p.means, p.log_scales = model["means"].ctypes.data, model["log_scales"].ctypes.data
+    p.opacity_logits, p.quaternions = model["opacity_logits"].ctypes.data, model["quaternions"].ctypes.data
+    p.lp_format = lp_format
+    if lp_format:
+        p.log_scales_lp, p.opacity_logits_lp = lp["log_scales"].ctypes.data, lp["opacity_logits"].ctypes.data
+    p.mode, p.minimum_opacity, p.split_at_screen_size, p.split_opacity_k = mode, min_opacity, split_at, split_k
+    par = np.ascontiguousarray(parents, np.int32)
+    smp = f32(samples)
+    scr = None if screens is None else f32(screens)
+    load().oracle_densify_split(C.byref(p), _p(par), len(par), child_base, _p(smp), _p(scr))
+
+
+def adc_decay(model: Dict[str, np.ndarray], opacity_decay: float, log_scale_decay: float,
+              lp: Optional[Dict[str, np.ndarray]] = None, lp_format: int = 0) -> None:
+    n = len(model["opacity_logits"])
+    load().oracle_adc_decay(_p(model["log_scales"]), _p(model["opacity_logits"]),
+                            _p(lp["log_scales"]) if lp_format else None, _p(lp["opacity_logits"]) if lp_format else None,
+                            lp_format, n, opacity_decay, log_scale_decay)
+
+
+def igs_rows(st, keep_rows, retained, opacities, median, allow_growth, gthr, sthr, weights=True):
+    pr, pos = np.zeros(retained, np.float32), np.zeros(retained, np.uint8)
+    counts = np.zeros(3, np.uint32)
+    rw, gw, ow = (np.zeros(retained, np.float32) for _ in range(3))
+    kr = None if keep_rows is None else np.ascontiguousarray(keep_rows, np.int32)
+    s = _stats_abi(st)
+    load().oracle_igs_rows(C.byref(s), _p(kr), retained, _p(f32(opacities)), median, int(allow_growth), gthr, sthr,
+                           _p(pr), _p(pos), _p(counts), _p(rw) if weights else None, _p(gw), _p(ow))
+    return pr, pos, [int(c) for c in counts], rw, gw, ow
+
+
+def select_igs_parents(rw, ow, gw, slots, desired, capacity, random):
+    """select_igs_parents (densification_igs.cpp:226-277) by sorting: (parents ascending,
+    replacement, oversized, growth)."""
+    from photara_studio_amd import densify as D
+    rw = f32(rw)
+    chosen = np.zeros(len(rw), bool)
+    k = min(slots, capacity)
+    n_rep = n_over = n_grow = 0
+    if k and len(rw):
+        rows, _, _ = select_rows(rw, k, 2, seed=D.gumbel_seed(random))
+        chosen[rows] = True
+        n_rep = len(rows)
+    remaining = capacity - n_rep
+    if remaining and ow is not None:
+        rows, _, _ = select_rows(ow, remaining, 3, exclude=chosen.astype(np.uint8))
+        chosen[rows] = True
+        n_over = len(rows)
+        remaining -= n_over
+    if remaining and desired and gw is not None:
+        rows, _, _ = select_rows(gw, min(remaining, desired), 2, exclude=chosen.astype(np.uint8),
+                                 seed=D.gumbel_seed(random))
+        chosen[rows] = True
+        n_grow = len(rows)
+    return np.flatnonzero(chosen).astype(np.int32), n_rep, n_over, n_grow
+
+
+MOMENT_NAMES = ("means_first", "means_second", "scales_first", "scales_second", "rotations_first", "rotations_second",
+                "opacity_first", "opacity_second", "sh_first", "sh_second")
+MODEL_NAMES = ("means", "log_scales", "quaternions", "opacity_logits", "sh")
+
+
+def igs_refine(model: Dict[str, np.ndarray], stats: Dict[str, np.ndarray], iteration: int, scene_extent: float,
+               center, options, random, moments: Dict[str, np.ndarray], quant: Optional[Dict[str, np.ndarray]] = None,
+               lp: Optional[Dict[str, np.ndarray]] = None, lp_format: int = 0, growth_cap: int = 0):
+    """IgsStrategy::refine (densification_igs.cpp:279-515) on host arrays, selection by sorting.
+    Mutates the dicts (rows replaced); returns (grown, pruned, info)."""
+    from photara_studio_amd import densify as D
+    old = len(model["opacity_logits"])
+    if not D.is_refinement_iteration(iteration, options) or old == 0:
+        return 0, 0, {}
+    keep0, hard, op = prune_masks(model, options.prune_opacity, 100.0 * scene_extent, center)
+    recycled = (keep0 == 0) & (hard == 0)
+    keep = hard == 0
+    hard_count = int(hard.sum())
+    frac = float(min(max(np.float32(options.dense_recycle_fraction), np.float32(0)), np.float32(1)))
+    limit = int(np.ceil(float(old) * frac))
+    budget = limit - hard_count if limit > hard_count else 0
+    if budget:
+        rows, _, _ = select_rows(op, budget, 1, mask=recycled)
+        keep[rows] = False
+    best_rows, _, _ = select_rows(op, 1, 0)
+    best = np.zeros(old, bool)
+    best[best_rows] = True
+    keep |= best & (hard == 0)
+    retained = int(keep.sum())
+    if retained > options.densification_cap:
+        rows, _, _ = select_rows(op, retained - options.densification_cap, 1, mask=keep, exclude=best)
+        keep[rows] = False
+        retained -= len(rows)
+    keep_rows = np.flatnonzero(keep).astype(np.int32) if retained != old else None
+    pruned = old - retained
+    gcap = options.densification_cap if growth_cap == 0 else min(options.densification_cap, max(old, growth_cap))
+    capacity = gcap - retained if gcap > retained else 0
+    rp = np.float32(1.0) - np.float32(iteration) / max(np.float32(1.0), np.float32(options.iterations))
+    prog = max(rp, np.float32(0.0))
+    od, lsd = D.adc_decay_factors(float(np.float32(options.opacity_decay) * prog),
+                                  float(np.float32(options.scale_decay) * prog))
+
+    def remap(param_rows, adam_rows):
+        def take(a, rows):
+            out = np.zeros((len(rows),) + a.shape[1:], a.dtype)
+            ok = rows >= 0
+            out[ok] = a[rows[ok]]
+            return out
+        for k in list(model):
+            model[k] = take(model[k], param_rows)
+        if lp is not None:
+            for k in list(lp):
+                lp[k] = take(lp[k], param_rows)
+        for k in list(moments):
+            moments[k] = take(moments[k], adam_rows)
+        if quant is not None:
+            for k in list(quant):
+                quant[k] = take(quant[k], adam_rows)
+
+    info = {}
+    if capacity == 0:
+        if keep_rows is not None:
+            remap(keep_rows, keep_rows)
+        adc_decay(model, od, lsd, lp, lp_format)
+        return 0, pruned, info
+    gthr, sthr = float(options.densify_gradient_threshold), float(options.densify_screen_threshold)
+    pr, pos, counts, _, _, _ = igs_rows(stats, keep_rows, retained, op, 1.0, False, gthr, sthr, weights=False)
+    allow = iteration < D.grow_stop_iteration(options)
+    growth_candidates = counts[1] if allow else 0
+    median = 1.0
+    if counts[0]:
+        rows, _, thr = select_rows(pr, counts[0] // 2 + 1, 1, mask=pos)
+        median = max(key_value(thr, 1), 1e-9)
+    _, _, _, rw, gw, ow = igs_rows(stats, keep_rows, retained, op, median, allow, gthr, sthr)
+    desired = int(D.round_half_away(growth_candidates * float(np.float32(options.densify_select_fraction)))) \
+        if allow else 0
+    info = dict(median=median, positive=counts[0])
+    parents, _, _, _ = select_igs_parents(rw, ow if allow else None, gw if allow else None, pruned, desired, capacity,
+                                          random)
+    count = len(parents)
+    if count == 0:
+        if keep_rows is not None:
+            remap(keep_rows, keep_rows)
+    else:
+        samples = D.normal_floats(random, 3 * count).reshape(count, 3)
+        base = keep_rows if keep_rows is not None else np.arange(old, dtype=np.int32)
+        param_rows = np.concatenate([base, base[parents]]).astype(np.int32)
+        adam_rows = param_rows.copy()
+        adam_rows[parents] = -1
+        adam_rows[retained:] = -1
+        remap(param_rows, adam_rows)
+        densify_split(model, parents, retained, samples, 2, options.prune_opacity, lp=lp, lp_format=lp_format)
+    adc_decay(model, od, lsd, lp, lp_format)
+    info.update(parents=parents)
+    return count, pruned, info

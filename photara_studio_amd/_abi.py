@@ -114,6 +114,43 @@ class pht_loss_config(C.Structure):
     ]
 
 
+
+# densification (ADC-IGS refinement)
+PHT_SELECT_LARGEST, PHT_SELECT_SMALLEST, PHT_SELECT_GUMBEL, PHT_SELECT_LARGEST_POSITIVE = 0, 1, 2, 3
+PHT_SPLIT_DENSE_TANGENT, PHT_SPLIT_ADC_COVARIANCE, PHT_SPLIT_IGS_RANDOM, PHT_SPLIT_LONG_AXIS = 0, 1, 2, 3
+PHT_MAX_ROW_BUFFERS = 24
+
+
+class pht_densify_stats(C.Structure):
+    _fields_ = [(name, C.c_void_p) for name in ("gradient", "count", "max_screen_radius", "priority",
+                                                  "max_geometry_gradient", "first_view", "view_support",
+                                                  "accumulated_error")]
+
+
+class pht_densify_observation(C.Structure):
+    _fields_ = [
+        ("refine_weight", C.c_void_p), ("visibility", C.c_void_p), ("radii", C.c_void_p),
+        ("geometry_gradient", C.c_void_p), ("image_error", C.c_void_p), ("normalized_size", C.c_void_p),
+        ("width", C.c_int32), ("height", C.c_int32),
+        ("use_maximum", C.c_int32), ("require_contribution_visibility", C.c_int32),
+        ("step_score_power", C.c_float), ("oversize_screen_threshold", C.c_float),
+        ("view_index", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+
+class pht_row_buffer(C.Structure):
+    _fields_ = [("src", C.c_void_p), ("dst", C.c_void_p), ("row_bytes", C.c_int64)]
+
+
+class pht_split_params(C.Structure):
+    _fields_ = [
+        ("means", C.c_void_p), ("log_scales", C.c_void_p), ("opacity_logits", C.c_void_p),
+        ("quaternions", C.c_void_p), ("log_scales_lp", C.c_void_p), ("opacity_logits_lp", C.c_void_p),
+        ("lp_format", C.c_int32), ("mode", C.c_int32),
+        ("minimum_opacity", C.c_float), ("split_at_screen_size", C.c_float), ("split_opacity_k", C.c_float),
+        ("reserved", C.c_int32),
+    ]
+
 # Every symbol the header declares, with its ctypes signature.
 SIGNATURES = {
     "pht_abi_version": (C.c_int, []),
@@ -149,6 +186,25 @@ SIGNATURES = {
                                         C.POINTER(C.c_void_p)]),
     "pht_step_graph_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
     "pht_step_graph_destroy": (None, [C.c_void_p]),
+    "pht_densify_accumulate": (C.c_int, [C.POINTER(pht_densify_stats), C.POINTER(pht_densify_observation), C.c_int,
+                                         C.c_int64, C.c_void_p]),
+    "pht_densify_prune_masks": (C.c_int, [C.POINTER(pht_gaussians), C.c_int32, C.c_float, C.c_float,
+                                          C.POINTER(C.c_float), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pht_select_workspace_bytes": (C.c_size_t, [C.c_int64]),
+    "pht_select_key_value": (C.c_float, [C.c_uint64, C.c_int]),
+    "pht_select_rows": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_int, C.c_uint64,
+                                  C.c_void_p, C.c_uint8, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
+                                  C.c_void_p]),
+    "pht_igs_prepare": (C.c_int, [C.POINTER(pht_densify_stats), C.c_void_p, C.c_int64, C.c_float, C.c_float,
+                                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pht_igs_weights": (C.c_int, [C.POINTER(pht_densify_stats), C.c_void_p, C.c_int64, C.c_void_p, C.c_float,
+                                  C.c_int32, C.c_float, C.c_float, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_void_p]),
+    "pht_gather_rows": (C.c_int, [C.POINTER(pht_row_buffer), C.c_int, C.c_void_p, C.c_int64, C.c_void_p]),
+    "pht_densify_split": (C.c_int, [C.POINTER(pht_split_params), C.c_void_p, C.c_int64, C.c_int64, C.c_void_p,
+                                    C.c_void_p, C.c_void_p]),
+    "pht_adc_decay": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int64, C.c_float,
+                                C.c_float, C.c_void_p]),
 }
 
 _lib = None

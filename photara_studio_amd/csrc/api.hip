@@ -465,6 +465,130 @@ int pht_sh_adam_quant_step(float* parameter, const float* gradient, uint16_t* fi
     return check_hip(hipGetLastError(), "quantised SH Adam launch");
 }
 
+// ---------------------------------------------------------------- densification
+int pht_densify_accumulate(const pht_densify_stats* stats, const pht_densify_observation* obs, int n_obs, int64_t n,
+                           void* stream) {
+    if (!stats || (n_obs > 0 && !obs) || n_obs < 0 || n_obs > PHT_MAX_VIEWS || n < 0)
+        return fail(PHT_EINVAL, "densify_accumulate: bad arguments (at most 32 observations)");
+    const pht_densify_stats& s = *stats;
+    if (n > 0 && (!s.gradient || !s.count || !s.max_screen_radius || !s.priority || !s.max_geometry_gradient ||
+                  !s.first_view || !s.view_support || !s.accumulated_error))
+        return fail(PHT_EINVAL, "densify_accumulate: null statistics array");
+    DensifyObsSet set{};
+    set.stats = s;
+    set.n = n_obs;
+    for (int v = 0; v < n_obs; ++v) {
+        const pht_densify_observation& o = obs[v];
+        if (n > 0 && (!o.refine_weight || !o.visibility || !o.radii))
+            return fail(PHT_EINVAL, "densify_accumulate: refine_weight, visibility and radii are required");
+        set.obs[v] = o;
+        const int32_t side = std::max<int32_t>(1, std::min(o.width, o.height));
+        set.inverse_resolution[v] = 1.0f / float(side);
+    }
+    launch_densify_accumulate(set, n, static_cast<hipStream_t>(stream));
+    return check_hip(hipGetLastError(), "densify_accumulate launch");
+}
+
+int pht_densify_prune_masks(const pht_gaussians* g, int32_t sh_stride, float minimum_opacity, float maximum_bounds,
+                            const float center[3], uint8_t* keep, uint8_t* hard, float* opacities, void* stream) {
+    if (!g || !center || g->count < 0 || sh_stride < 0) return fail(PHT_EINVAL, "prune_masks: bad arguments");
+    if (g->count > 0 && (!g->means || !g->log_scales || !g->quaternions || !g->opacity_logits ||
+                         (sh_stride > 0 && !g->sh) || !keep || !hard || !opacities))
+        return fail(PHT_EINVAL, "prune_masks: null array");
+    launch_prune_masks(*g, sh_stride, minimum_opacity, maximum_bounds, center, keep, hard, opacities,
+                       static_cast<hipStream_t>(stream));
+    return check_hip(hipGetLastError(), "prune_masks launch");
+}
+
+size_t pht_select_workspace_bytes(int64_t n) { return select_workspace_bytes(n); }
+
+float pht_select_key_value(uint64_t key, int mode) {
+    const uint32_t hi = uint32_t(key >> 32);
+    return float_from_order_bits(mode == PHT_SELECT_SMALLEST ? ~hi : hi);
+}
+
+int pht_select_rows(const float* values, const uint8_t* mask, const uint8_t* exclude, int64_t n, int64_t k, int mode,
+                    uint64_t seed, uint8_t* out, uint8_t out_value, uint32_t* counts, uint64_t* threshold,
+                    void* workspace, size_t workspace_bytes, void* stream) {
+    if (n < 0 || n > int64_t(0xfffffffe) || mode < PHT_SELECT_LARGEST || mode > PHT_SELECT_LARGEST_POSITIVE)
+        return fail(PHT_EINVAL, "select_rows: n must be in [0, 2^32 - 2] and mode one of PHT_SELECT_*");
+    if (!workspace || workspace_bytes < select_workspace_bytes(n))
+        return fail(PHT_EINVAL, "select_rows: workspace smaller than pht_select_workspace_bytes(n)");
+    if (n > 0 && (!values || !out)) return fail(PHT_EINVAL, "select_rows: null values or out");
+    launch_select_rows(values, mask, exclude, n, k, mode, seed, out, out_value, counts, threshold, workspace,
+                       static_cast<hipStream_t>(stream));
+    return check_hip(hipGetLastError(), "select_rows launch");
+}
+
+int pht_igs_prepare(const pht_densify_stats* stats, const int32_t* keep_rows, int64_t retained,
+                    float gradient_threshold, float screen_threshold, float* priority, uint8_t* positive,
+                    uint32_t* counts, void* stream) {
+    if (!stats || retained < 0 || !counts) return fail(PHT_EINVAL, "igs_prepare: bad arguments");
+    if (retained > 0 && (!stats->count || !stats->priority || !stats->gradient || !stats->max_screen_radius ||
+                         !priority || !positive))
+        return fail(PHT_EINVAL, "igs_prepare: null array");
+    launch_igs_prepare(*stats, keep_rows, retained, gradient_threshold, screen_threshold, priority, positive, counts,
+                       static_cast<hipStream_t>(stream));
+    return check_hip(hipGetLastError(), "igs_prepare launch");
+}
+
+int pht_igs_weights(const pht_densify_stats* stats, const int32_t* keep_rows, int64_t retained,
+                    const float* opacities, float median, int32_t allow_growth,
+                    float gradient_threshold, float screen_threshold, float* replacement, float* growth,
+                    float* oversize, void* stream) {
+    if (!stats || retained < 0) return fail(PHT_EINVAL, "igs_weights: bad arguments");
+    if (retained > 0 && (!opacities || !replacement || !stats->count || !stats->priority || !stats->gradient ||
+                         !stats->max_screen_radius || (allow_growth && (!growth || !oversize))))
+        return fail(PHT_EINVAL, "igs_weights: null array");
+    launch_igs_weights(*stats, keep_rows, retained, opacities, median, allow_growth, gradient_threshold,
+                       screen_threshold, replacement, growth, oversize, static_cast<hipStream_t>(stream));
+    return check_hip(hipGetLastError(), "igs_weights launch");
+}
+
+int pht_gather_rows(const pht_row_buffer* buffers, int n_buffers, const int32_t* rows, int64_t n_rows,
+                    void* stream) {
+    if (n_buffers < 0 || n_buffers > PHT_MAX_ROW_BUFFERS || n_rows < 0 || (n_buffers > 0 && !buffers) ||
+        (n_rows > 0 && !rows))
+        return fail(PHT_EINVAL, "gather_rows: bad arguments (at most 24 buffers)");
+    RowBuffers set{};
+    set.n = n_buffers;
+    for (int i = 0; i < n_buffers; ++i) {
+        const pht_row_buffer& b = buffers[i];
+        if (b.row_bytes <= 0 || (b.row_bytes & 1) || !b.dst || (n_rows > 0 && !b.src))
+            return fail(PHT_EINVAL, "gather_rows: each buffer needs src, dst and a positive even row size");
+        const uintptr_t a = reinterpret_cast<uintptr_t>(b.src) | reinterpret_cast<uintptr_t>(b.dst);
+        int64_t unit = 2;
+        if (b.row_bytes % 16 == 0 && a % 16 == 0) unit = 16;
+        else if (b.row_bytes % 4 == 0 && a % 4 == 0) unit = 4;
+        else if (a % 2 != 0) return fail(PHT_EINVAL, "gather_rows: buffers must be 2-byte aligned");
+        set.b[i] = RowBuffer{b.src, b.dst, b.row_bytes, unit};
+    }
+    launch_gather_rows(set, rows, n_rows, static_cast<hipStream_t>(stream));
+    return check_hip(hipGetLastError(), "gather_rows launch");
+}
+
+int pht_densify_split(const pht_split_params* p, const int32_t* parents, int64_t count, int64_t child_base,
+                      const float* samples, const float* screen_sizes, void* stream) {
+    if (!p || count < 0 || child_base < 0 || p->mode < PHT_SPLIT_DENSE_TANGENT || p->mode > PHT_SPLIT_LONG_AXIS)
+        return fail(PHT_EINVAL, "densify_split: bad arguments");
+    if (count > 0 && (!parents || !samples || !p->means || !p->log_scales || !p->opacity_logits || !p->quaternions))
+        return fail(PHT_EINVAL, "densify_split: null array");
+    if (p->lp_format != PHT_LP_NONE && (!p->log_scales_lp || !p->opacity_logits_lp))
+        return fail(PHT_EINVAL, "densify_split: lp_format set without reduced-precision copies");
+    launch_densify_split(*p, parents, count, child_base, samples, screen_sizes, static_cast<hipStream_t>(stream));
+    return check_hip(hipGetLastError(), "densify_split launch");
+}
+
+int pht_adc_decay(float* log_scales, float* opacity_logits, uint16_t* log_scales_lp, uint16_t* opacity_logits_lp,
+                  int32_t lp_format, int64_t n, float opacity_decay, float log_scale_decay, void* stream) {
+    if (n < 0 || (n > 0 && (!log_scales || !opacity_logits)) ||
+        (lp_format != PHT_LP_NONE && n > 0 && (!log_scales_lp || !opacity_logits_lp)))
+        return fail(PHT_EINVAL, "adc_decay: bad arguments");
+    launch_adc_decay(log_scales, opacity_logits, log_scales_lp, opacity_logits_lp, lp_format, n, opacity_decay,
+                     log_scale_decay, static_cast<hipStream_t>(stream));
+    return check_hip(hipGetLastError(), "adc_decay launch");
+}
+
 int pht_get_counts(pht_workspace* ws, pht_counts* out, void* stream) {
     if (!ws || !out) return fail(PHT_EINVAL, "null argument");
     hipStream_t st = static_cast<hipStream_t>(stream);

@@ -117,4 +117,48 @@ void launch_l1_loss(const float* color, const float* target, float* grad, float*
                     float weight, hipStream_t st);
 void launch_zero_f32(float* p, int n, hipStream_t st);
 
+// k_densify.hip
+struct DensifyObsSet {
+    pht_densify_stats stats;
+    pht_densify_observation obs[PHT_MAX_VIEWS];
+    float inverse_resolution[PHT_MAX_VIEWS]; // 1 / max(1, min(width, height)) per observation
+    int n;
+};
+void launch_densify_accumulate(const DensifyObsSet& set, int64_t n, hipStream_t st);
+void launch_prune_masks(const pht_gaussians& g, int sh_stride, float min_opacity, float max_bounds,
+                        const float center[3], uint8_t* keep, uint8_t* hard, float* opacities, hipStream_t st);
+// Device state of one radix select (in the caller's workspace, after the keys).
+struct SelectState {
+    uint32_t hist[256];
+    uint32_t eligible;  // rows with a key
+    uint32_t remaining; // rank still to find below the current prefix
+    uint32_t chosen;    // min(k, eligible)
+    uint32_t pad;
+    uint64_t prefix;    // digits chosen so far (the full threshold key after eight passes)
+};
+size_t select_workspace_bytes(int64_t n);
+void launch_select_rows(const float* values, const uint8_t* mask, const uint8_t* exclude, int64_t n, int64_t k,
+                        int mode, uint64_t seed, uint8_t* out, uint8_t out_value, uint32_t* counts,
+                        uint64_t* threshold, void* workspace, hipStream_t st);
+void launch_igs_prepare(const pht_densify_stats& s, const int32_t* keep_rows, int64_t retained, float gthr,
+                        float sthr, float* priority, uint8_t* positive, uint32_t* counts, hipStream_t st);
+void launch_igs_weights(const pht_densify_stats& s, const int32_t* keep_rows, int64_t retained,
+                        const float* opacities, float median, int allow_growth, float gthr, float sthr,
+                        float* replacement, float* growth, float* oversize, hipStream_t st);
+struct RowBuffer {
+    const void* src;
+    void* dst;
+    int64_t row_bytes;
+    int64_t unit; // bytes moved per access: 16, 4 or 2
+};
+struct RowBuffers {
+    RowBuffer b[PHT_MAX_ROW_BUFFERS];
+    int n;
+};
+void launch_gather_rows(const RowBuffers& set, const int32_t* rows, int64_t n_rows, hipStream_t st);
+void launch_densify_split(const pht_split_params& p, const int32_t* parents, int64_t count, int64_t child_base,
+                          const float* samples, const float* screens, hipStream_t st);
+void launch_adc_decay(float* log_scales, float* logits, uint16_t* ls_lp, uint16_t* logit_lp, int fmt, int64_t n,
+                      float opacity_decay, float log_scale_decay, hipStream_t st);
+
 } // namespace pht

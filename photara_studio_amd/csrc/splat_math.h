@@ -1040,4 +1040,223 @@ PHT_HD uint16_t f32_to_fp16(float f) { // round to nearest even
 PHT_HD float lp_to_f32(uint16_t h, int fmt) { return fmt == PHT_LP_FP16 ? fp16_to_f32(h) : bf16_to_f32(h); }
 PHT_HD uint16_t f32_to_lp(float f, int fmt) { return fmt == PHT_LP_FP16 ? f32_to_fp16(f) : f32_to_bf16(f); }
 
+// ---------------------------------------------------------------- densification (per row)
+// Shared by the device kernels and the CPU oracle, so both compute the same bits.
+
+PHT_HD float psigmoidf(float x) { return 1.0f / (1.0f + pexpf(-x)); }
+// log2 through plogf (~2 ulp), and x^y for x >= 0 through exp(y log x).
+PHT_HD float plog2f(float x) { return plogf(x) * 1.44269502162933350f; }
+PHT_HD float ppowf(float x, float y) { return x == 0.0f ? 0.0f : pexpf(y * plogf(x)); }
+// logit(p) = log(p / (1 - p)).
+PHT_HD float plogitf(float p) { return plogf(p / (1.0f - p)); }
+
+// accumulate_densification_kernel (photara/src/splat/cuda_ops.cu:1548-1606): row i, one view.
+PHT_HD void densify_accumulate_row(const pht_densify_stats& s, const pht_densify_observation& o,
+                                   float inverse_resolution, int64_t i) {
+    const int32_t radius = o.radii[i];
+    if (radius <= 0 || (o.require_contribution_visibility && o.visibility[i] <= 0.0f)) return;
+    const float rw = o.refine_weight[i];
+    const float raw = finite_(rw) ? fmaxf(rw, 0.0f) : 0.0f;
+    if (o.geometry_gradient != nullptr && finite_(o.geometry_gradient[i]))
+        s.max_geometry_gradient[i] = fmaxf(s.max_geometry_gradient[i], o.geometry_gradient[i]);
+    const float w = o.step_score_power == 1.0f ? raw : ppowf(raw, o.step_score_power);
+    s.gradient[i] = o.use_maximum ? fmaxf(s.gradient[i], w) : s.gradient[i] + w;
+    s.count[i] += 1.0f;
+    if (o.image_error != nullptr && finite_(o.image_error[i]))
+        s.accumulated_error[i] += fmaxf(o.image_error[i], 0.0f);
+    if (o.view_index >= 0) {
+        if (s.first_view[i] == 0) {
+            s.first_view[i] = o.view_index + 1;
+            s.view_support[i] = 1.0f;
+        } else if (s.first_view[i] != o.view_index + 1) {
+            s.view_support[i] = 2.0f;
+        }
+    }
+    const float screen = o.normalized_size != nullptr ? o.normalized_size[i] : float(radius) * inverse_resolution;
+    s.max_screen_radius[i] = fmaxf(s.max_screen_radius[i], screen);
+    const float thr = o.oversize_screen_threshold;
+    s.priority[i] += thr > 0.0f ? fmaxf(plog2f(screen / thr), 0.0f) : w * (1.0f + screen);
+}
+
+// adc_plus_prune_kernel (photara/src/splat/cuda_ops.cu:263-302): row i. Returns opacity.
+PHT_HD float densify_prune_row(const float* means, const float* log_scales, const float* quats, const float* logits,
+                               const float* sh, int sh_stride, int64_t i, float minimum_opacity,
+                               float maximum_bounds, const float center[3], bool& keep, bool& hard) {
+    const float logit = logits[i];
+    const float opacity = psigmoidf(logit);
+    bool bad = !finite_(logit);
+    float max_scale = 0.0f;
+    for (int k = 0; k < 3; ++k) {
+        const float m = means[3 * i + k], ls = log_scales[3 * i + k];
+        bad = bad || !finite_(m) || !finite_(ls);
+        max_scale = fmaxf(max_scale, pexpf(ls));
+    }
+    for (int k = 0; k < 4; ++k) bad = bad || !finite_(quats[4 * i + k]);
+    for (int k = 0; k < sh_stride; ++k) bad = bad || !finite_(sh[i * sh_stride + k]);
+    const bool outside = fabsf(means[3 * i] - center[0]) > maximum_bounds ||
+                         fabsf(means[3 * i + 1] - center[1]) > maximum_bounds ||
+                         fabsf(means[3 * i + 2] - center[2]) > maximum_bounds;
+    hard = bad || outside || max_scale > maximum_bounds;
+    keep = !hard && opacity >= minimum_opacity;
+    return opacity;
+}
+
+// Selection keys: larger key = selected first; key 0 = not eligible.
+// Total order of floats as unsigned integers (negative values flipped, positive values above).
+PHT_HD uint32_t float_order_bits(float v) {
+    const uint32_t u = f2u(v);
+    return u ^ ((u & 0x80000000u) ? 0xffffffffu : 0x80000000u);
+}
+PHT_HD float float_from_order_bits(uint32_t o) {
+    return u2f((o & 0x80000000u) ? (o ^ 0x80000000u) : ~o);
+}
+// Largest / smallest value first, ties to the lower row (a stable sort's front).
+PHT_HD uint64_t select_value_key(float v, bool largest, uint32_t i) {
+    const uint32_t o = float_order_bits(v);
+    return (uint64_t(largest ? o : ~o) << 32) | uint64_t(0xffffffffu - i);
+}
+// splitmix64's output function (Steele, Lea, Flood 2014): the generator's i-th output for seed s
+// is mix(s + 0x9e3779b97f4a7c15 * i).
+PHT_HD uint64_t splitmix64_mix(uint64_t x) {
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+    return x ^ (x >> 31);
+}
+// Gumbel top-k key of igs_weighted_sample_cuda (photara/src/splat/igs_selection_cuda.cu:31-70):
+// score = log(w) - log(-log(u)) for positive finite w (subnormal w through its integer bits),
+// u uniform in (0, 1) from splitmix64; ties to the higher row.
+PHT_HD uint64_t gumbel_key(float w, uint64_t seed, uint32_t i) {
+    const uint32_t bits = f2u(w);
+    if (bits == 0u || bits >= 0x7f800000u) return 0;
+    const uint64_t r = splitmix64_mix(seed + 0x9e3779b97f4a7c15ull * (uint64_t(i) + 1));
+    const float u = fminf(1.0f - 1e-7f, fmaxf(1e-7f, (float(uint32_t(r >> 41)) + 0.5f) * 1.1920928955078125e-07f));
+    const float log_w = bits < 0x00800000u ? plogf(float(bits)) - 149.0f * 0.6931471805599453f : plogf(w);
+    const float score = log_w - plogf(-plogf(u));
+    return (uint64_t(float_order_bits(score)) << 32) | uint64_t(i);
+}
+
+// Quaternion rotation v' = v + w t + q x t, t = 2 q x v, of the normalised scalar-first q.
+PHT_HD void quat_rotate(const float* raw, const float v[3], float out[3]) {
+    const float inv = 1.0f / sqrtf(fmaxf(((raw[0] * raw[0] + raw[1] * raw[1]) + raw[2] * raw[2]) + raw[3] * raw[3],
+                                         1e-20f));
+    const float w = raw[0] * inv, x = raw[1] * inv, y = raw[2] * inv, z = raw[3] * inv;
+    const float tx = 2.0f * (y * v[2] - z * v[1]);
+    const float ty = 2.0f * (z * v[0] - x * v[2]);
+    const float tz = 2.0f * (x * v[1] - y * v[0]);
+    out[0] = (v[0] + w * tx) + (y * tz - z * ty);
+    out[1] = (v[1] + w * ty) + (z * tx - x * tz);
+    out[2] = (v[2] + w * tz) + (x * ty - y * tx);
+}
+
+// split_gaussians_kernel (photara/src/splat/cuda_ops.cu:1720-1845): the parent row and the child
+// row both hold the parent's parameters on entry; both receive the split parameters.
+PHT_HD void densify_split_row(float* means, float* log_scales, float* logits, const float* quats, int64_t parent,
+                              int64_t child, const float sample[3], float screen, bool has_screen, int mode,
+                              float minimum_opacity, float split_at_screen_size, float split_opacity_k) {
+    const float* ls = log_scales + 3 * parent;
+    float local[3] = {0.0f, 0.0f, 0.0f}, delta[3] = {0.0f, 0.0f, 0.0f};
+    if (mode == PHT_SPLIT_DENSE_TANGENT) {
+        // tangent-plane split: offsets along local x / y, both shrink by 1/sqrt(2), normal kept
+        local[0] = (pexpf(ls[0]) * sample[0]) * 0.5f;
+        local[1] = (pexpf(ls[1]) * sample[1]) * 0.5f;
+        delta[0] = delta[1] = plogf(0.70710678118654752f);
+    } else if (mode == PHT_SPLIT_ADC_COVARIANCE) {
+        // rank-one split along the major axis that keeps the mixture covariance
+        float scale2[3], max2 = 0.0f;
+        int major = 0;
+        for (int k = 0; k < 3; ++k) {
+            const float s = pexpf(ls[k]);
+            scale2[k] = s * s;
+            if (scale2[k] > max2) { max2 = scale2[k]; major = k; }
+        }
+        const float standard_k = 0.70710678118654752f;
+        const float sc = has_screen ? fmaxf(screen, 1e-6f) : 1e-6f;
+        const float max_k = split_at_screen_size > 0.0f ? fminf(standard_k, split_at_screen_size / sc) : standard_k;
+        for (int k = 0; k < 3; ++k) {
+            const float kk = k == major ? max_k : 1.0f;
+            local[k] = sqrtf(fmaxf(1.0f - kk * kk, 0.0f)) * sqrtf(scale2[k]);
+            delta[k] = plogf(fmaxf(kk, 1e-12f));
+        }
+    } else if (mode == PHT_SPLIT_IGS_RANDOM) {
+        // one shared variate scales every axis; only the largest axis halves
+        int largest = 0;
+        if (ls[1] > ls[largest]) largest = 1;
+        if (ls[2] > ls[largest]) largest = 2;
+        for (int k = 0; k < 3; ++k) {
+            local[k] = pexpf(ls[k]) * sample[0];
+            delta[k] = k == largest ? plogf(0.5f) : 0.0f;
+        }
+    } else { // PHT_SPLIT_LONG_AXIS: offset of half the major scale along it; major halves, others x0.85
+        int largest = 0;
+        for (int k = 1; k < 3; ++k)
+            if (ls[k] > ls[largest]) largest = k;
+        const float offset = 0.5f * pexpf(ls[largest]);
+        for (int k = 0; k < 3; ++k) {
+            local[k] = k == largest ? offset : 0.0f;
+            delta[k] = k == largest ? plogf(0.5f) : plogf(0.85f);
+        }
+    }
+    float off[3];
+    quat_rotate(quats + 4 * parent, local, off);
+    for (int k = 0; k < 3; ++k) {
+        const float c = means[3 * parent + k];
+        means[3 * parent + k] = c - off[k];
+        means[3 * child + k] = c + off[k];
+        log_scales[3 * parent + k] += delta[k];
+        log_scales[3 * child + k] += delta[k];
+    }
+    float logit;
+    if (mode == PHT_SPLIT_LONG_AXIS) {
+        // a' = k / (1 + exp(-l) - k): two children approximately composite back to the parent
+        const float den = 1.0f + pexpf(-logits[parent]) - split_opacity_k;
+        const float mapped = den > 1e-6f ? fminf(fmaxf(split_opacity_k / den, 1e-6f), 1.0f - 1e-6f) : 1.0f - 1e-6f;
+        logit = plogitf(mapped);
+    } else {
+        // 1 - (1 - a)^p with p = 1/2 (two children reproduce the parent's alpha exactly) or
+        // 1/sqrt(2) (covariance mode), clamped to [min, 1 - min]
+        const float opacity = psigmoidf(logits[parent]);
+        const float base = fmaxf(1.0f - opacity, 0.0f);
+        const float powered = mode == PHT_SPLIT_ADC_COVARIANCE ? ppowf(base, 0.70710678118654752f) : sqrtf(base);
+        const float revised = fminf(fmaxf(1.0f - powered, minimum_opacity), 1.0f - minimum_opacity);
+        logit = plogitf(revised);
+    }
+    logits[parent] = logit;
+    logits[child] = logit;
+}
+
+// adc_decay_kernel (photara/src/splat/cuda_ops.cu:1847-1861): row i.
+PHT_HD void densify_decay_row(float* log_scales, float* logits, int64_t i, float opacity_decay, float log_scale_decay) {
+    const float op = fminf(fmaxf(psigmoidf(logits[i]) - opacity_decay, 1e-12f), 1.0f - 1e-12f);
+    logits[i] = plogitf(op);
+    for (int k = 0; k < 3; ++k) log_scales[3 * i + k] += log_scale_decay;
+}
+
+// IGS parent scores of retained row r (densification_igs.cpp:439-497).
+struct IgsRow {
+    float priority;     // priority / max(count, 1)
+    bool candidate;     // count > 0
+    bool positive;      // candidate && priority > 0
+    bool growth;        // candidate && gradient > gradient threshold
+    bool oversized;     // candidate && max screen radius > screen threshold
+};
+PHT_HD IgsRow igs_row(const pht_densify_stats& s, int64_t src, float gradient_threshold, float screen_threshold) {
+    IgsRow r;
+    const float c = s.count[src];
+    r.priority = s.priority[src] / fmaxf(c, 1.0f);
+    r.candidate = c > 0.0f;
+    r.positive = r.candidate && r.priority > 0.0f;
+    r.growth = r.candidate && s.gradient[src] > gradient_threshold;
+    r.oversized = r.candidate && s.max_screen_radius[src] > screen_threshold;
+    return r;
+}
+PHT_HD void igs_weights_row(const pht_densify_stats& s, int64_t src, float opacity, float median, bool allow_growth,
+                            float gradient_threshold, float screen_threshold, float& replacement, float& growth,
+                            float& oversize) {
+    const IgsRow r = igs_row(s, src, gradient_threshold, screen_threshold);
+    const float edge = fminf((r.priority > 0.0f ? r.priority : 0.0f) / median, 10.0f) * 0.25f + 1.0f;
+    replacement = r.candidate ? opacity * edge : 0.0f;
+    growth = allow_growth && r.growth ? (s.gradient[src] * edge) * (r.oversized ? 2.0f : 1.0f) : 0.0f;
+    oversize = allow_growth && r.oversized ? s.max_screen_radius[src] * edge : 0.0f;
+}
+
 } // namespace pht

@@ -33,7 +33,8 @@ def test_library_exports_every_declared_symbol():
 
 
 STRUCTS = ["pht_camera", "pht_settings", "pht_gaussians", "pht_view_outputs", "pht_view_grads", "pht_model_grads",
-           "pht_adam", "pht_capacity", "pht_counts", "pht_loss_config"]
+           "pht_adam", "pht_capacity", "pht_counts", "pht_loss_config", "pht_densify_stats",
+           "pht_densify_observation", "pht_row_buffer", "pht_split_params"]
 
 
 def test_struct_layouts_match_header(tmp_path):
@@ -60,7 +61,7 @@ def test_struct_layouts_match_header(tmp_path):
 def test_workspace_bytes_host_only():
     from photara_studio_amd import _abi
     lib = _abi.load()
-    assert lib.pht_abi_version() == 2
+    assert lib.pht_abi_version() == 3
     cap = _abi.pht_capacity(10000, 1, 16, 256, 65536, 1 << 20)
     b1 = lib.pht_workspace_bytes(C.byref(cap))
     cap2 = _abi.pht_capacity(10000, 1, 16, 256, 65536, 1 << 21)
@@ -72,6 +73,25 @@ def test_workspace_bytes_host_only():
     # invalid workspace creation is rejected before touching a device
     h = C.c_void_p()
     assert lib.pht_workspace_create(None, 0, C.byref(cap), C.byref(h)) == _abi.PHT_EINVAL
+
+
+def test_select_host_helpers():
+    """Host-only parts of the row selection: workspace size and the value behind a key."""
+    import struct
+    from photara_studio_amd import _abi
+    lib = _abi.load()
+    assert lib.pht_select_workspace_bytes(1000) >= 8 * 1000 + 1024
+    for v in (0.25, -3.5, 0.0, 2.0 ** -140):
+        bits = struct.unpack("<I", struct.pack("<f", v))[0]
+        order = bits ^ (0xFFFFFFFF if bits & 0x80000000 else 0x80000000)
+        assert lib.pht_select_key_value((order << 32) | 5, _abi.PHT_SELECT_LARGEST) == pytest.approx(v, abs=0)
+        assert lib.pht_select_key_value(((~order & 0xFFFFFFFF) << 32) | 5, _abi.PHT_SELECT_SMALLEST) == \
+            pytest.approx(v, abs=0)
+    # bad arguments are rejected before any launch
+    ws = C.create_string_buffer(64)
+    assert lib.pht_select_rows(None, None, None, 10, 1, 7, 0, None, 1, None, None, ws, 64, None) == _abi.PHT_EINVAL
+    assert lib.pht_select_rows(None, None, None, 10, 1, 0, 0, None, 1, None, None, ws, 64, None) == _abi.PHT_EINVAL
+    assert b"workspace" in lib.pht_last_error()
 
 
 def test_product_path_fails_loudly_without_library(monkeypatch, tmp_path):

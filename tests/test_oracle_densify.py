@@ -1,0 +1,385 @@
+"""The densification oracle (CPU) against the reference's own densification tests and the closed
+forms the split / decay / statistics satisfy. The fixtures restate, case by case, photara's
+test_igs_refinement_decisions (splat_test.cpp:6613-6803), test_igs_budgeted_selection_and_panorama_sizes
+(:6805-6878, the selection part), test_densification_cap_stops_igs_growth (:6880-6913),
+test_densify_statistics_and_oversize_weights (:7607-7640, the statistics part),
+test_adc_plus_split_preserves_covariance (:6977-7050) and test_emc_long_axis_split_math (:7166-7213)."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from photara_studio_amd import densify as D
+
+
+# ---------------------------------------------------------------- random numbers
+def test_mt19937_matches_the_cpp_standard():
+    # [rand.predef]: the 10000th output of a default-constructed std::mt19937 is 4123659995
+    r = D.Mt19937()
+    v = r.raw(10000)
+    assert int(v[-1]) == 4123659995
+    r = D.Mt19937(42)
+    first = r()
+    assert first == int(D.Mt19937(42).raw(1)[0])
+
+
+def test_splitmix64_keys_follow_the_published_sequence(oracle):
+    # splitmix64 seeded with 0 outputs 0xe220a8397b1dcdaf, 0x6e789e6aa1b965f4, 0x06c45d188009454f;
+    # the Gumbel key of row i uses output i+1 (u from its top 23 bits)
+    known = [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+    w = np.array([0.5, 2.0, 1e-3], np.float32)
+    keys = oracle.select_keys(w, mode=2, seed=0)
+    for i, z in enumerate(known):
+        u = min(1 - 1e-7, max(1e-7, ((z >> 41) + 0.5) * 2.0 ** -23))
+        expected = math.log(w[i]) - math.log(-math.log(u))
+        assert int(keys[i]) & 0xFFFFFFFF == i
+        assert oracle.key_value(int(keys[i]), 0) == pytest.approx(expected, rel=2e-6, abs=2e-6)
+
+
+def test_gumbel_keys_reject_non_positive_and_non_finite(oracle):
+    w = np.array([0.0, -1.0, np.inf, np.nan, 1e-42, 3.0], np.float32)
+    keys = oracle.select_keys(w, mode=2, seed=123)
+    assert list(keys[:4]) == [0, 0, 0, 0]
+    assert keys[4] != 0 and keys[5] != 0  # a subnormal weight is eligible (scored through its bits)
+    # score = log(w) - log(-log(u)) with u in [1e-7, 1 - 1e-7]: within [log w - 2.78, log w + 16.12]
+    for i in (4, 5):
+        lw = math.log(float(w[i]))
+        assert lw - 2.79 <= oracle.key_value(int(keys[i]), 0) <= lw + 16.13
+
+
+def test_normal_floats_match_the_sequential_polar_method():
+    def sequential(random, count):
+        out, saved = [], None
+        for _ in range(count):
+            if saved is not None:
+                out.append(saved)
+                saved = None
+                continue
+            while True:
+                x = np.float32(2.0) * D._canonical(random.raw(1))[0] - np.float32(1.0)
+                y = np.float32(2.0) * D._canonical(random.raw(1))[0] - np.float32(1.0)
+                r2 = x * x + y * y
+                if r2 <= 1.0 and r2 != 0.0:
+                    break
+            m = np.sqrt(np.float32(-2.0) * np.log(r2) / r2)
+            saved = np.float32(x * m)
+            out.append(np.float32(y * m))
+        return np.array(out, np.float32)
+
+    for count in (1, 2, 7, 300):
+        a, b = D.Mt19937(42), D.Mt19937(42)
+        assert np.array_equal(D.normal_floats(a, count), sequential(b, count)), count
+        assert a() == b()  # both consumed the same draws
+    big = D.normal_floats(D.Mt19937(7), 200000)
+    assert abs(float(big.mean())) < 0.01 and abs(float(big.std()) - 1.0) < 0.01
+
+
+# ---------------------------------------------------------------- selection
+def test_select_rows_orders_and_ties(oracle):
+    v = np.array([0.3, 0.1, 0.3, 0.9, np.nan, 0.1, -2.0], np.float32)
+    rows, elig, thr = oracle.select_rows(v, 3, 0)           # largest, ties to the lower row
+    assert elig == 6 and list(rows) == [0, 2, 3] and oracle.key_value(thr, 0) == pytest.approx(0.3)
+    rows, _, thr = oracle.select_rows(v, 2, 1)               # smallest
+    assert list(rows) == [1, 6] and oracle.key_value(thr, 1) == pytest.approx(0.1)
+    rows, _, _ = oracle.select_rows(v, 3, 1)
+    assert list(rows) == [1, 5, 6]
+    rows, elig, _ = oracle.select_rows(v, 10, 3)             # positive finite only
+    assert elig == 5 and list(rows) == [0, 1, 2, 3, 5]
+    mask = np.array([1, 1, 1, 0, 1, 1, 1], np.uint8)
+    excl = np.array([1, 0, 0, 0, 0, 0, 0], np.uint8)
+    rows, elig, _ = oracle.select_rows(v, 1, 0, mask=mask, exclude=excl)
+    assert elig == 4 and list(rows) == [2]
+
+
+def test_select_rows_is_a_stable_sort_front(oracle):
+    rng = np.random.default_rng(3)
+    for trial in range(20):
+        n = int(rng.integers(1, 200))
+        v = rng.integers(0, 6, n).astype(np.float32) * 0.25  # many ties
+        mask = (rng.random(n) < 0.8).astype(np.uint8)
+        k = int(rng.integers(0, n + 3))
+        for mode, desc in ((0, True), (1, False)):
+            rows, elig, _ = oracle.select_rows(v, k, mode, mask=mask)
+            idx = np.flatnonzero(mask)
+            order = idx[np.argsort(-v[idx] if desc else v[idx], kind="stable")]
+            assert elig == len(idx)
+            assert list(rows) == sorted(order[:min(k, len(idx))]), (trial, mode)
+
+
+def test_igs_budgeted_selection_reference_cases(oracle):
+    v = lambda *a: np.array(a, np.float32)  # noqa: E731
+    parents, rep, over, grow = oracle.select_igs_parents(v(0, 0, 0, 0, 1), v(1, 2, 9, 3, 0), v(1, 1, 1, 1, 1), 1, 5, 2,
+                                                         D.Mt19937(0))
+    assert list(parents) == [2, 4] and (rep, over, grow) == (1, 1, 0)
+    parents, *_ = oracle.select_igs_parents(v(1, 0, 0, 0, 0), v(0, 3, 9, 2, 1), None, 1, 0, 2, D.Mt19937(0))
+    assert list(parents) == [0, 2]
+    parents, rep, over, grow = oracle.select_igs_parents(v(0, 0, 0, 0, 1), v(0, 0, 9, 0, 0), v(1, 1, 1, 1, 1), 1, 5, 5,
+                                                         D.Mt19937(0))
+    assert len(parents) == 5 and grow == 3
+
+
+# ---------------------------------------------------------------- statistics
+def test_statistics_reference_cases(oracle):
+    n = 2
+    st = oracle.make_stats(n)
+    vis = np.ones(n, np.float32)
+    radii = np.ones(n, np.int32)
+    obs = dict(refine_weight=vis, visibility=vis, radii=radii, width=100, height=100, use_maximum=0,
+               require_contribution_visibility=1, step_score_power=1.0, oversize_screen_threshold=0.0)
+    for _ in range(3):
+        oracle.densify_accumulate(st, [dict(obs, view_index=7)])
+    assert list(st["view_support"]) == [1.0, 1.0]
+    oracle.densify_accumulate(st, [dict(obs, visibility=np.array([1, 0], np.float32), view_index=8)])
+    assert list(st["view_support"]) == [2.0, 1.0] and list(st["count"]) == [4.0, 3.0]
+    over = oracle.make_stats(n)
+    for r in ([20, 20], [1, 20]):
+        oracle.densify_accumulate(over, [dict(obs, radii=np.array(r, np.int32), oversize_screen_threshold=0.1)])
+    assert over["priority"][0] == pytest.approx(1.0, abs=1e-5) and over["priority"][1] == pytest.approx(2.0, abs=1e-5)
+    # normalized sizes replace radius / min(W, H) (the panorama path's angular size)
+    sz = oracle.make_stats(n)
+    oracle.densify_accumulate(sz, [dict(obs, radii=np.full(n, 10000, np.int32), width=1920, height=960,
+                                        normalized_size=np.array([0.0472, 0.3], np.float32), use_maximum=1,
+                                        oversize_screen_threshold=0.5, view_index=0)])
+    assert sz["max_screen_radius"][0] == pytest.approx(0.0472)
+
+
+def test_statistics_closed_form(oracle):
+    rng = np.random.default_rng(5)
+    n = 500
+    st = oracle.make_stats(n)
+    exp_grad, exp_count = np.zeros(n), np.zeros(n)
+    exp_screen, exp_pri = np.zeros(n), np.zeros(n)
+    for view in range(3):
+        rw = rng.normal(0, 1, n).astype(np.float32)
+        rw[::17] = np.nan
+        vis = (rng.random(n) < 0.7).astype(np.float32)
+        radii = rng.integers(-2, 60, n).astype(np.int32)
+        oracle.densify_accumulate(st, [dict(refine_weight=rw, visibility=vis, radii=radii, width=640, height=480,
+                                            use_maximum=1, require_contribution_visibility=1, step_score_power=1.0,
+                                            oversize_screen_threshold=0.05, view_index=view)])
+        on = (radii > 0) & (vis > 0)
+        w = np.where(np.isfinite(rw), np.maximum(rw, 0), 0)
+        exp_grad = np.where(on, np.maximum(exp_grad, w), exp_grad)
+        exp_count += on
+        screen = radii / 480.0
+        exp_screen = np.where(on, np.maximum(exp_screen, screen), exp_screen)
+        exp_pri += np.where(on, np.maximum(np.log2(np.maximum(screen, 1e-30) / 0.05), 0), 0)
+    assert np.array_equal(st["count"], exp_count.astype(np.float32))
+    assert np.allclose(st["gradient"], exp_grad, atol=0)
+    assert np.allclose(st["max_screen_radius"], exp_screen, rtol=1e-6)
+    assert np.allclose(st["priority"], exp_pri, rtol=2e-6, atol=2e-6)
+
+
+# ---------------------------------------------------------------- prune masks, split, decay
+def _model(n, logits=None, means=None, log_scale=-3.0):
+    m = {
+        "means": np.zeros((n, 3), np.float32) if means is None else np.asarray(means, np.float32).reshape(n, 3),
+        "log_scales": np.full((n, 3), log_scale, np.float32),
+        "quaternions": np.tile(np.array([1, 0, 0, 0], np.float32), (n, 1)),
+        "opacity_logits": np.zeros(n, np.float32) if logits is None else np.asarray(logits, np.float32),
+        "sh": np.zeros((n, 1, 3), np.float32),
+    }
+    return m
+
+
+def test_prune_masks_cases(oracle):
+    m = _model(6, logits=[-8, 0, 0, 0, 0, np.nan], means=[[0, 0, 0], [0, 150, 0], [0, 0, 0], [0, 0, 0],
+                                                             [0, 0, 0], [0, 0, 0]])
+    m["log_scales"][2, 1] = math.log(200.0)   # larger than the bound
+    m["sh"][3, 0, 2] = np.inf
+    keep, hard, op = oracle.prune_masks(m, 0.05, 100.0, [0, 0, 0])
+    assert list(hard) == [0, 1, 1, 1, 0, 1]
+    assert list(keep) == [0, 0, 0, 0, 1, 0]
+    assert op[4] == pytest.approx(0.5) and op[0] == pytest.approx(1 / (1 + math.exp(8)), rel=1e-5)
+
+
+def _rot(q):
+    w, x, y, z = np.asarray(q, np.float64) / np.linalg.norm(q)
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+def test_igs_random_split_keeps_alpha_and_halves_the_largest_axis(oracle):
+    rng = np.random.default_rng(11)
+    n = 64
+    m = _model(2 * n)
+    m["means"][:n] = rng.normal(0, 1, (n, 3))
+    m["log_scales"][:n] = rng.normal(-3, 0.5, (n, 3))
+    q = rng.normal(0, 1, (n, 4)).astype(np.float32)
+    m["quaternions"][:n] = q
+    m["opacity_logits"][:n] = rng.normal(0, 2, n)
+    for k in m:
+        m[k][n:] = m[k][:n]
+    before = {k: v.copy() for k, v in m.items()}
+    samples = rng.normal(0, 1, (n, 3)).astype(np.float32)
+    oracle.densify_split(m, np.arange(n), n, samples, 2, 1.0 / 255.0)
+    a = 1 / (1 + np.exp(-before["opacity_logits"][:n].astype(np.float64)))
+    a_new = 1 / (1 + np.exp(-m["opacity_logits"][:n].astype(np.float64)))
+    clamped = (a_new <= 1 / 255 + 1e-6) | (a_new >= 1 - 1 / 255 - 1e-6)
+    assert np.allclose((1 - (1 - a_new) ** 2)[~clamped], a[~clamped], atol=2e-6)   # two children composite the parent
+    assert np.array_equal(m["opacity_logits"][:n], m["opacity_logits"][n:])
+    for j in range(n):
+        ls = before["log_scales"][j]
+        big = int(np.argmax(ls))
+        d = m["log_scales"][j] - ls
+        assert d[big] == pytest.approx(-math.log(2), abs=2e-6) and np.all(np.delete(d, big) == 0)
+        off = _rot(q[j]) @ (np.exp(ls.astype(np.float64)) * samples[j, 0])
+        assert np.allclose(m["means"][n + j] - before["means"][j], off, atol=2e-6)
+        assert np.allclose(before["means"][j] - m["means"][j], off, atol=2e-6)
+
+
+def test_adc_covariance_split_reference_case(oracle):
+    m = _model(2)
+    m["log_scales"][:] = [math.log(2.0), 0.0, math.log(0.5)]
+    oracle.densify_split(m, [0], 1, np.zeros((1, 3), np.float32), 1, 1.0 / 255.0, split_at=0.5,
+                         screens=np.array([1.0], np.float32))
+    k, scale = [0.5, 1.0, 1.0], [2.0, 1.0, 0.5]
+    for axis in range(3):
+        off = math.sqrt(1 - k[axis] ** 2) * scale[axis]
+        assert abs(m["means"][0, axis] + off) < 1e-5 and abs(m["means"][1, axis] - off) < 1e-5
+        assert abs(m["log_scales"][0, axis] - math.log(scale[axis] * k[axis])) < 1e-5
+        assert abs(m["log_scales"][1, axis] - math.log(scale[axis] * k[axis])) < 1e-5
+    pm, cm, ps = m["means"][0], m["means"][1], m["log_scales"][0]
+    for r in range(3):
+        for c in range(3):
+            cov = 0.5 * (pm[r] * pm[c] + cm[r] * cm[c]) + (math.exp(2 * ps[r]) if r == c else 0.0)
+            assert abs(cov - (scale[r] ** 2 if r == c else 0.0)) < 1e-5
+    op = 1 - 0.5 ** (math.sqrt(2) / 2)
+    assert abs(m["opacity_logits"][0] - math.log(op / (1 - op))) < 1e-5
+    assert abs(m["opacity_logits"][1] - math.log(op / (1 - op))) < 1e-5
+
+
+def test_long_axis_split_reference_case(oracle):
+    m = _model(2)
+    m["log_scales"][:] = [math.log(0.001), math.log(0.05), math.log(0.002)]
+    oracle.densify_split(m, [0], 1, np.array([[0.3, 0.2, 0.7]], np.float32), 3, 1e-3, split_k=0.5)
+    assert abs(m["means"][0, 1] + 0.025) < 1e-6 and abs(m["means"][1, 1] - 0.025) < 1e-6
+    assert abs(m["means"][0, 0]) < 1e-6 and abs(m["means"][1, 2]) < 1e-6
+    assert abs(m["log_scales"][0, 0] - math.log(0.001 * 0.85)) < 1e-5
+    assert abs(m["log_scales"][0, 1] - math.log(0.05 * 0.5)) < 1e-5
+    assert abs(m["log_scales"][0, 2] - math.log(0.002 * 0.85)) < 1e-5
+    assert abs(m["log_scales"][1, 1] - m["log_scales"][0, 1]) < 1e-6
+    assert abs(m["opacity_logits"][0] - math.log(0.5)) < 1e-4 and abs(m["opacity_logits"][1] - math.log(0.5)) < 1e-4
+
+
+def test_dense_tangent_split_keeps_the_normal_axis(oracle):
+    m = _model(2)
+    m["log_scales"][:] = [math.log(0.2), math.log(0.1), math.log(0.01)]
+    oracle.densify_split(m, [0], 1, np.array([[1.0, -2.0, 5.0]], np.float32), 0, 1.0 / 255.0)
+    assert m["means"][1, 0] == pytest.approx(0.1, rel=1e-5) and m["means"][1, 1] == pytest.approx(-0.1, rel=1e-5)
+    assert m["means"][1, 2] == 0.0 and m["log_scales"][0, 2] == pytest.approx(math.log(0.01), abs=1e-6)
+    assert m["log_scales"][0, 0] == pytest.approx(math.log(0.2 / math.sqrt(2)), abs=1e-5)
+    a = 1 / (1 + math.exp(-float(m["opacity_logits"][0])))
+    assert 1 - (1 - a) ** 2 == pytest.approx(0.5, abs=1e-6)
+
+
+def test_decay_closed_form(oracle):
+    rng = np.random.default_rng(2)
+    m = _model(1000)
+    m["opacity_logits"][:] = rng.normal(0, 3, 1000)
+    m["log_scales"][:] = rng.normal(-3, 1, (1000, 3))
+    before = {k: v.copy() for k, v in m.items()}
+    od, lsd = D.adc_decay_factors(0.004, 0.05)
+    assert lsd == pytest.approx(math.log(0.95), rel=1e-6)
+    oracle.adc_decay(m, od, lsd)
+    a0 = 1 / (1 + np.exp(-before["opacity_logits"].astype(np.float64)))
+    a1 = 1 / (1 + np.exp(-m["opacity_logits"].astype(np.float64)))
+    expect = np.clip(a0 - 0.004, 1e-12, 1)
+    assert np.allclose(a1, expect, rtol=2e-5, atol=1e-9)
+    assert np.array_equal(m["log_scales"], before["log_scales"] + np.float32(lsd))
+
+
+# ---------------------------------------------------------------- the refinement
+def _options(**kw):
+    o = D.DensifyOptions(refine_start_iter=0, refine_stop_iter=200, refine_every=100)
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def _harness(logits, means):
+    n = len(logits)
+    m = _model(n, logits=logits, means=means)
+    moments = {}
+    for name, src in (("means", "means"), ("scales", "log_scales"), ("rotations", "quaternions"),
+                      ("opacity", "opacity_logits"), ("sh", "sh")):
+        moments[name + "_first"] = np.zeros_like(m[src])
+        moments[name + "_second"] = np.zeros_like(m[src])
+    return m, moments
+
+
+def _indexed_means(n, out_of_bounds=None):
+    return [[200.0 if i == out_of_bounds else float(i), 0.0, 0.0] for i in range(n)]
+
+
+def _stats(n, gradient=1.0, count=10.0, priority=10.0, screen=0.01):
+    st = {"gradient": np.full(n, gradient, np.float32), "count": np.full(n, count, np.float32),
+          "max_screen_radius": np.full(n, screen, np.float32), "priority": np.full(n, priority, np.float32),
+          "max_geometry_gradient": np.zeros(n, np.float32), "first_view": np.zeros(n, np.int32),
+          "view_support": np.full(n, 2.0, np.float32), "accumulated_error": np.zeros(n, np.float32)}
+    return st
+
+
+def test_igs_refinement_reference_cases(oracle):
+    # row 5 lies outside the 100x bound, rows 0..2 below the floor; the recycle budget is
+    # ceil(8 * 0.25) = 2 with one slot spent on the hard row: only row 0 is recycled
+    m, mo = _harness([-8, -7, -6, -1, 0, 1, 2, 3], _indexed_means(8, 5))
+    grown, pruned, _ = oracle.igs_refine(m, _stats(8), 100, 1.0, [0, 0, 0],
+                                         _options(prune_opacity=0.05, dense_recycle_fraction=0.25,
+                                                  densification_cap=6), D.Mt19937(42), mo)
+    assert (pruned, grown, len(m["means"])) == (2, 0, 6)
+    assert list(m["means"][:, 0]) == [1, 2, 3, 4, 6, 7]
+    # cap clamp: the two least opaque rows go, the strongest survives
+    m, mo = _harness([-5, -4, -3, 0, 2, 3], _indexed_means(6))
+    grown, pruned, _ = oracle.igs_refine(m, _stats(6), 100, 1.0, [0, 0, 0],
+                                         _options(prune_opacity=0.001, dense_recycle_fraction=0.0, densification_cap=4),
+                                         D.Mt19937(42), mo)
+    assert (pruned, grown, len(m["means"])) == (2, 0, 4) and list(m["means"][:, 0]) == [2, 3, 4, 5]
+    # growth budget: half of six gradient-qualified parents
+    m, mo = _harness([0] * 6, _indexed_means(6))
+    grown, pruned, _ = oracle.igs_refine(m, _stats(6), 100, 1.0, [0, 0, 0],
+                                         _options(densification_cap=12, densify_select_fraction=0.5,
+                                                  densify_gradient_threshold=0.0, densify_screen_threshold=10.0,
+                                                  grow_stop_iter=1000), D.Mt19937(42), mo)
+    assert (pruned, grown, len(m["means"])) == (0, 3, 9)
+    assert all(v.shape[0] == 9 for v in mo.values())
+    assert np.all(np.isfinite(m["means"])) and np.all(np.isfinite(m["log_scales"]))
+    # zero prune, no capacity: rows and storage stay
+    m, mo = _harness([0] * 6, _indexed_means(6))
+    grown, pruned, _ = oracle.igs_refine(m, oracle.make_stats(6), 100, 1.0, [0, 0, 0],
+                                         _options(dense_recycle_fraction=0.0, densification_cap=6), D.Mt19937(42), mo)
+    assert (pruned, grown) == (0, 0) and list(m["means"][:, 0]) == [0, 1, 2, 3, 4, 5]
+    # recycling refills its budget; parents and children both restart from zero moments
+    m, mo = _harness([-8, -7, -6, -1, 0, 1, 2, 3], _indexed_means(8, 5))
+    for k in mo:
+        mo[k][:] = 1.0 if k.endswith("first") else 2.0
+    st = _stats(8, gradient=0.0)
+    grown, pruned, _ = oracle.igs_refine(m, st, 100, 1.0, [0, 0, 0],
+                                         _options(prune_opacity=0.05, dense_recycle_fraction=0.25,
+                                                  densification_cap=8), D.Mt19937(42), mo)
+    assert (pruned, grown, len(m["means"])) == (2, 2, 8)
+    for k, v in mo.items():
+        width = v.size // 8
+        assert int((v == 0).sum()) == 4 * width, k
+        assert np.all(v[-2:] == 0), k
+
+
+def test_densification_cap_stops_growth(oracle):
+    m, mo = _harness([0] * 4, [[0, 0, 0]] * 4)
+    o = D.apply_strategy_defaults(D.DensifyOptions())
+    o.refine_every, o.densification_cap, o.grow_stop_iter = 100, 4, 10000
+    grown, pruned, _ = oracle.igs_refine(m, _stats(4), 600, 1.0, [0, 0, 0], o, D.Mt19937(42), mo)
+    assert len(m["means"]) == 4 and grown == 0
+
+
+def test_schedule():
+    o = D.DensifyOptions(refine_start_iter=0, refine_stop_iter=200, refine_every=100)
+    assert D.strategy_schedule(o) == (0, 27500, 100)
+    assert D.is_refinement_iteration(100, o) and not D.is_refinement_iteration(150, o)
+    assert not D.is_refinement_iteration(0, o) and not D.is_refinement_iteration(27500, o)
+    d = D.DensifyOptions()
+    assert D.strategy_schedule(d) == (0, 27500, 200) and D.grow_stop_iteration(d) == 15000
+    assert D.round_half_away(2.5) == 3 and D.round_half_away(1.4999) == 1
