@@ -144,6 +144,57 @@ def cpu_baseline(arrays, cams, opts, seconds):
                       f"in {el:.1f}s"}
 
 
+def densify_bench(step, refines=3, seed=3):
+    """Densification costs on the benchmark model: the statistics of all views of one step (one
+    launch), and ADC-IGS refinements (prune / recycle, Gumbel and ranked parent selection, row
+    remap of the model, bf16 copies, FP32 moments and quantised SH state, split, decay), wall
+    clock including the host readbacks the refinement branches on."""
+    from photara_studio_amd import densify as D
+    dev = step.model.means.device
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+
+    def random_stats(n):
+        st = D.make_densification_stats(n, dev)
+        st.gradient.copy_(torch.rand((n,), generator=gen, device=dev) * 0.01)
+        st.count.copy_(torch.randint(0, 9, (n,), generator=gen, device=dev).float())
+        st.max_screen_radius.copy_(torch.rand((n,), generator=gen, device=dev) * 0.6)
+        st.priority.copy_(torch.rand((n,), generator=gen, device=dev) * 4.0)
+        return st
+
+    n = step.model.size()
+    stats = D.make_densification_stats(n, dev)
+    obs = []
+    for v, cam in enumerate(step.cameras):
+        rw = torch.rand((n,), generator=gen, device=dev) * 0.01
+        obs.append(D.Observation(rw, step.visibility[v], step.radii[v], cam.width, cam.height, True, True, 1.0, 0.5,
+                                 rw, v))
+    D.accumulate_views(stats, obs)
+    reps = 20
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        D.accumulate_views(stats, obs)
+    e1.record()
+    torch.cuda.synchronize()
+    acc_ms = e0.elapsed_time(e1) / reps
+    opts = D.apply_strategy_defaults(D.DensifyOptions(densification_cap=n + 400_000 * refines))
+    states = D.AdamStates(step.adam, step.sh_state)
+    rows = []
+    for r in range(refines):
+        stats = random_stats(step.model.size())
+        before = step.model.size()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        counts, stats = D.refine_gaussians(step.model, stats, 200 * (r + 2), 1.0, [0.0, 0.0, 0.0], opts,
+                                           D.Mt19937(42 + r), states)
+        torch.cuda.synchronize()
+        rows.append({"ms": round((time.perf_counter() - t0) * 1e3, 2), "rows_in": before, "pruned": counts.pruned,
+                     "grown": counts.grown})
+    return {"accumulate_views_ms": round(acc_ms, 4), "views": len(obs), "refines": rows,
+            "refine_ms_median": sorted(r["ms"] for r in rows)[len(rows) // 2]}
+
+
 def main():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -154,6 +205,7 @@ def main():
     p.add_argument("--plain-steps", type=int, default=0, help="plain-step count (default: 4 x steps)")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-densify", action="store_true", help="skip the densification timings")
     p.add_argument("--workload", choices=("ring", "c2"), default="ring",
                    help="ring: --views 1920x1080 views on a radius-8 ring; c2: 2x1080p + 3x720p + 3x540p")
     p.add_argument("--tile", type=int, default=16)
@@ -242,6 +294,16 @@ def main():
     traffic, traffic_src = profiled_traffic(dominant, workload_key)
     print(f"[bench] stages ms: " + ", ".join(f"{k}={v:.3f}" for k, v in acc.items()), flush=True)
 
+    dens = None
+    if not args.no_densify:
+        # runs last on the batched step's model: a refinement replaces its arrays
+        dens = densify_bench(batched)
+        print(f"[bench] densify: statistics of {dens['views']} views {dens['accumulate_views_ms']:.3f} ms; refinements "
+              + ", ".join(f"{r['ms']:.1f} ms ({r['rows_in']} rows, -{r['pruned']} +{r['grown']})"
+                          for r in dens["refines"]), flush=True)
+    del batched
+    torch.cuda.empty_cache()
+
     cpu = None
     if not args.no_cpu_baseline:
         import numpy as np
@@ -279,6 +341,7 @@ def main():
                      "algorithmic_bytes": int(ab[dominant]),
                      "traffic_source": f"profiles/{traffic_src}" if traffic_src else None},
         "cpu_baseline": cpu,
+        "densify": dens,
     }
     print(json.dumps(line), flush=True)
 

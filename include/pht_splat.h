@@ -428,6 +428,11 @@ int pht_densify_split(const pht_split_params* p, const int32_t* parents, int64_t
 int pht_adc_decay(float* log_scales, float* opacity_logits, uint16_t* log_scales_lp, uint16_t* opacity_logits_lp,
                   int32_t lp_format, int64_t n, float opacity_decay, float log_scale_decay, void* stream);
 
+/* Host only: count draws of a fresh std::normal_distribution<float>(0, 1) over the std::mt19937
+ * whose state (624 words, position in [0, 624]) the caller holds and which is advanced in place:
+ * the split offsets of grow_igs_random_gpu (densification_adc_plus.cpp:113-120). */
+int pht_host_normal_floats(uint32_t* mt_key, int32_t* mt_pos, float* out, int64_t count);
+
 #ifdef __cplusplus
 }
 #endif

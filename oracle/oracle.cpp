@@ -1066,7 +1066,11 @@ void oracle_densify_accumulate(const pht_densify_stats* s, const pht_densify_obs
     for (int v = 0; v < n_obs; ++v) {
         const int32_t side = std::max<int32_t>(1, std::min(obs[v].width, obs[v].height));
         const float inv = 1.0f / float(side);
-        for (int64_t i = 0; i < n; ++i) densify_accumulate_row(*s, obs[v], inv, i);
+        for (int64_t i = 0; i < n; ++i) {
+            DensifyRow r = densify_load_row(*s, i);
+            densify_accumulate_row(r, obs[v], inv, i);
+            densify_store_row(*s, i, r);
+        }
     }
 }
 

@@ -203,6 +203,7 @@ SIGNATURES = {
     "pht_gather_rows": (C.c_int, [C.POINTER(pht_row_buffer), C.c_int, C.c_void_p, C.c_int64, C.c_void_p]),
     "pht_densify_split": (C.c_int, [C.POINTER(pht_split_params), C.c_void_p, C.c_int64, C.c_int64, C.c_void_p,
                                     C.c_void_p, C.c_void_p]),
+    "pht_host_normal_floats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.c_void_p, C.c_int64]),
     "pht_adc_decay": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int64, C.c_float,
                                 C.c_float, C.c_void_p]),
 }

@@ -9,6 +9,7 @@
 // stream, so the step can be captured as one hipGraph.
 #include <algorithm>
 #include <cstring>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -587,6 +588,44 @@ int pht_adc_decay(float* log_scales, float* opacity_logits, uint16_t* log_scales
     launch_adc_decay(log_scales, opacity_logits, log_scales_lp, opacity_logits_lp, lp_format, n, opacity_decay,
                      log_scale_decay, static_cast<hipStream_t>(stream));
     return check_hip(hipGetLastError(), "adc_decay launch");
+}
+
+namespace {
+// MT19937 (Matsumoto & Nishimura 1998) over a caller-held state of 624 words and a position, the
+// state layout of std::mt19937 and of numpy's MT19937; a UniformRandomBitGenerator, so the
+// standard library's distributions consume it exactly as they consume std::mt19937.
+struct Mt19937State {
+    using result_type = uint32_t;
+    uint32_t* key;
+    int32_t pos;
+    static constexpr result_type min() { return 0u; }
+    static constexpr result_type max() { return 0xffffffffu; }
+    void regenerate() {
+        for (int i = 0; i < 624; ++i) {
+            const uint32_t y = (key[i] & 0x80000000u) | (key[(i + 1) % 624] & 0x7fffffffu);
+            key[i] = key[(i + 397) % 624] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+        }
+        pos = 0;
+    }
+    result_type operator()() {
+        if (pos >= 624) regenerate();
+        uint32_t y = key[pos++];
+        y ^= y >> 11;
+        y ^= (y << 7) & 0x9d2c5680u;
+        y ^= (y << 15) & 0xefc60000u;
+        return y ^ (y >> 18);
+    }
+};
+} // namespace
+
+int pht_host_normal_floats(uint32_t* mt_key, int32_t* mt_pos, float* out, int64_t count) {
+    if (!mt_key || !mt_pos || count < 0 || (count > 0 && !out) || *mt_pos < 0 || *mt_pos > 624)
+        return fail(PHT_EINVAL, "host_normal_floats: bad arguments");
+    Mt19937State eng{mt_key, *mt_pos};
+    std::normal_distribution<float> normal(0.0f, 1.0f);
+    for (int64_t i = 0; i < count; ++i) out[i] = normal(eng);
+    *mt_pos = eng.pos;
+    return PHT_OK;
 }
 
 int pht_get_counts(pht_workspace* ws, pht_counts* out, void* stream) {

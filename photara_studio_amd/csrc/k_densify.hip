@@ -42,8 +42,11 @@ __device__ uint32_t block_count(uint32_t v, uint32_t* lds) {
 
 // ---------------------------------------------------------------- statistics
 __global__ __launch_bounds__(kThreads) void k_densify_accumulate(DensifyObsSet set, int64_t n) {
-    for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads)
-        for (int v = 0; v < set.n; ++v) densify_accumulate_row(set.stats, set.obs[v], set.inverse_resolution[v], i);
+    for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads) {
+        DensifyRow r = densify_load_row(set.stats, i);
+        for (int v = 0; v < set.n; ++v) densify_accumulate_row(r, set.obs[v], set.inverse_resolution[v], i);
+        densify_store_row(set.stats, i, r);
+    }
 }
 
 void launch_densify_accumulate(const DensifyObsSet& set, int64_t n, hipStream_t st) {

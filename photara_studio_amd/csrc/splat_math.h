@@ -1050,32 +1050,55 @@ PHT_HD float ppowf(float x, float y) { return x == 0.0f ? 0.0f : pexpf(y * plogf
 // logit(p) = log(p / (1 - p)).
 PHT_HD float plogitf(float p) { return plogf(p / (1.0f - p)); }
 
-// accumulate_densification_kernel (photara/src/splat/cuda_ops.cu:1548-1606): row i, one view.
-PHT_HD void densify_accumulate_row(const pht_densify_stats& s, const pht_densify_observation& o,
-                                   float inverse_resolution, int64_t i) {
+// accumulate_densification_kernel (photara/src/splat/cuda_ops.cu:1548-1606). One row's statistics
+// held in registers while the observations of several views are folded in, in order.
+struct DensifyRow {
+    float gradient, count, max_screen_radius, priority, max_geometry_gradient, view_support, accumulated_error;
+    int32_t first_view;
+};
+PHT_HD DensifyRow densify_load_row(const pht_densify_stats& s, int64_t i) {
+    return DensifyRow{s.gradient[i], s.count[i], s.max_screen_radius[i], s.priority[i], s.max_geometry_gradient[i],
+                      s.view_support[i], s.accumulated_error[i], s.first_view[i]};
+}
+PHT_HD void densify_store_row(const pht_densify_stats& s, int64_t i, const DensifyRow& r) {
+    s.gradient[i] = r.gradient;
+    s.count[i] = r.count;
+    s.max_screen_radius[i] = r.max_screen_radius;
+    s.priority[i] = r.priority;
+    s.max_geometry_gradient[i] = r.max_geometry_gradient;
+    s.view_support[i] = r.view_support;
+    s.accumulated_error[i] = r.accumulated_error;
+    s.first_view[i] = r.first_view;
+}
+PHT_HD void densify_accumulate_row(DensifyRow& s, const pht_densify_observation& o, float inverse_resolution,
+                                   int64_t i) {
     const int32_t radius = o.radii[i];
     if (radius <= 0 || (o.require_contribution_visibility && o.visibility[i] <= 0.0f)) return;
     const float rw = o.refine_weight[i];
     const float raw = finite_(rw) ? fmaxf(rw, 0.0f) : 0.0f;
-    if (o.geometry_gradient != nullptr && finite_(o.geometry_gradient[i]))
-        s.max_geometry_gradient[i] = fmaxf(s.max_geometry_gradient[i], o.geometry_gradient[i]);
+    if (o.geometry_gradient != nullptr) {
+        const float gg = o.geometry_gradient[i];
+        if (finite_(gg)) s.max_geometry_gradient = fmaxf(s.max_geometry_gradient, gg);
+    }
     const float w = o.step_score_power == 1.0f ? raw : ppowf(raw, o.step_score_power);
-    s.gradient[i] = o.use_maximum ? fmaxf(s.gradient[i], w) : s.gradient[i] + w;
-    s.count[i] += 1.0f;
-    if (o.image_error != nullptr && finite_(o.image_error[i]))
-        s.accumulated_error[i] += fmaxf(o.image_error[i], 0.0f);
+    s.gradient = o.use_maximum ? fmaxf(s.gradient, w) : s.gradient + w;
+    s.count += 1.0f;
+    if (o.image_error != nullptr) {
+        const float e = o.image_error[i];
+        if (finite_(e)) s.accumulated_error += fmaxf(e, 0.0f);
+    }
     if (o.view_index >= 0) {
-        if (s.first_view[i] == 0) {
-            s.first_view[i] = o.view_index + 1;
-            s.view_support[i] = 1.0f;
-        } else if (s.first_view[i] != o.view_index + 1) {
-            s.view_support[i] = 2.0f;
+        if (s.first_view == 0) {
+            s.first_view = o.view_index + 1;
+            s.view_support = 1.0f;
+        } else if (s.first_view != o.view_index + 1) {
+            s.view_support = 2.0f;
         }
     }
     const float screen = o.normalized_size != nullptr ? o.normalized_size[i] : float(radius) * inverse_resolution;
-    s.max_screen_radius[i] = fmaxf(s.max_screen_radius[i], screen);
+    s.max_screen_radius = fmaxf(s.max_screen_radius, screen);
     const float thr = o.oversize_screen_threshold;
-    s.priority[i] += thr > 0.0f ? fmaxf(plog2f(screen / thr), 0.0f) : w * (1.0f + screen);
+    s.priority += thr > 0.0f ? fmaxf(plog2f(screen / thr), 0.0f) : w * (1.0f + screen);
 }
 
 // adc_plus_prune_kernel (photara/src/splat/cuda_ops.cu:263-302): row i. Returns opacity.

@@ -10,9 +10,9 @@ the refinement branches on, as the reference does; mask algebra on byte masks an
 compaction of the keep / chosen masks use torch device ops.
 
 Random numbers follow the reference's: ``Mt19937`` is std::mt19937 (the Gumbel seeds are two of
-its 32-bit outputs, high word first, igs_weighted_sample_cuda's seed), and ``normal_floats``
-restates libstdc++'s std::normal_distribution<float> (Marsaglia polar method over
-std::generate_canonical<float>), whose float log may differ from glibc's logf in the last place.
+its 32-bit outputs, high word first, igs_weighted_sample_cuda's seed), and ``normal_floats`` draws
+the split offsets with the C++ standard library's std::normal_distribution<float> over that
+engine's state, as grow_igs_random_gpu does.
 """
 from __future__ import annotations
 
@@ -53,38 +53,16 @@ class Mt19937:
         self._rs.set_state(state)
 
 
-def _canonical(raw: np.ndarray) -> np.ndarray:
-    # std::generate_canonical<float, 24> over a 32-bit engine: float(x) / 2^32, below 1.
-    u = raw.astype(np.float32) / np.float32(4294967296.0)
-    return np.where(u >= np.float32(1.0), np.nextafter(np.float32(1.0), np.float32(0.0)), u).astype(np.float32)
-
-
 def normal_floats(random: Mt19937, count: int) -> np.ndarray:
-    """``count`` draws of a fresh std::normal_distribution<float>(0, 1) (libstdc++): pairs
-    (x, y) = 2u - 1 are drawn until 0 < x^2 + y^2 <= 1; the pair yields y*m, then x*m, with
-    m = sqrt(-2 log(r2) / r2)."""
+    """``count`` draws of a fresh std::normal_distribution<float>(0, 1) over the engine, by the
+    standard library itself (pht_host_normal_floats); the engine advances in place."""
+    _, key, pos, _, _ = random.get_state()
+    key = np.ascontiguousarray(key, np.uint32).copy()
+    p = C.c_int32(int(pos))
     out = np.empty(count, np.float32)
-    filled = 0
-    while filled < count:
-        need_pairs = (count - filled + 1) // 2
-        state = random.get_state()
-        attempt = max(16, int(need_pairs * 1.35) + 8)
-        raw = random.raw(2 * attempt)
-        x = np.float32(2.0) * _canonical(raw[0::2]) - np.float32(1.0)
-        y = np.float32(2.0) * _canonical(raw[1::2]) - np.float32(1.0)
-        r2 = x * x + y * y
-        ok = np.flatnonzero((r2 <= np.float32(1.0)) & (r2 != np.float32(0.0)))
-        take = ok[:need_pairs]
-        used = int(take[-1]) + 1 if len(take) == need_pairs else attempt
-        random.set_state(state)
-        random.raw(2 * used)  # consume exactly the draws the sequential method makes
-        mult = np.sqrt(np.float32(-2.0) * np.log(r2[take]) / r2[take]).astype(np.float32)
-        pair = np.empty(2 * len(take), np.float32)
-        pair[0::2] = y[take] * mult
-        pair[1::2] = x[take] * mult
-        n = min(len(pair), count - filled)
-        out[filled:filled + n] = pair[:n]
-        filled += n
+    check(_abi.load().pht_host_normal_floats(C.c_void_p(key.ctypes.data), C.byref(p),
+                                             C.c_void_p(out.ctypes.data) if count else None, int(count)))
+    random.set_state(("MT19937", key, int(p.value), 0, 0.0))
     return out
 
 

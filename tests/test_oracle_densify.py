@@ -49,31 +49,46 @@ def test_gumbel_keys_reject_non_positive_and_non_finite(oracle):
         assert lw - 2.79 <= oracle.key_value(int(keys[i]), 0) <= lw + 16.13
 
 
-def test_normal_floats_match_the_sequential_polar_method():
-    def sequential(random, count):
-        out, saved = [], None
-        for _ in range(count):
-            if saved is not None:
-                out.append(saved)
-                saved = None
-                continue
-            while True:
-                x = np.float32(2.0) * D._canonical(random.raw(1))[0] - np.float32(1.0)
-                y = np.float32(2.0) * D._canonical(random.raw(1))[0] - np.float32(1.0)
-                r2 = x * x + y * y
-                if r2 <= 1.0 and r2 != 0.0:
-                    break
-            m = np.sqrt(np.float32(-2.0) * np.log(r2) / r2)
-            saved = np.float32(x * m)
-            out.append(np.float32(y * m))
-        return np.array(out, np.float32)
+def _canonical(raw):
+    # std::generate_canonical<float, 24> over a 32-bit engine: float(x) / 2^32, kept below 1
+    u = raw.astype(np.float32) / np.float32(4294967296.0)
+    return np.where(u >= np.float32(1.0), np.nextafter(np.float32(1.0), np.float32(0.0)), u).astype(np.float32)
 
-    for count in (1, 2, 7, 300):
+
+def _polar(random, count):
+    """libstdc++'s std::normal_distribution<float>::operator() (Marsaglia's polar method): pairs
+    x, y = 2u - 1 until 0 < x^2 + y^2 <= 1, then y*m and (cached) x*m, m = sqrt(-2 ln r2 / r2)."""
+    out, saved = [], None
+    for _ in range(count):
+        if saved is not None:
+            out.append(saved)
+            saved = None
+            continue
+        while True:
+            x = np.float32(2.0) * _canonical(random.raw(1))[0] - np.float32(1.0)
+            y = np.float32(2.0) * _canonical(random.raw(1))[0] - np.float32(1.0)
+            r2 = x * x + y * y
+            if r2 <= 1.0 and r2 != 0.0:
+                break
+        m = np.sqrt(np.float32(-2.0) * np.float32(np.log(np.float64(r2))) / r2)  # a correctly rounded logf
+        saved = np.float32(x * m)
+        out.append(np.float32(y * m))
+    return np.array(out, np.float32)
+
+
+def test_normal_floats_are_the_polar_method_over_mt19937():
+    for count in (1, 2, 7, 2000):
         a, b = D.Mt19937(42), D.Mt19937(42)
-        assert np.array_equal(D.normal_floats(a, count), sequential(b, count)), count
-        assert a() == b()  # both consumed the same draws
+        got, want = D.normal_floats(a, count), _polar(b, count)
+        # equal up to the last place of the float logarithm (the library's logf vs numpy's)
+        assert np.allclose(got, want, rtol=3e-7, atol=0), count
+        assert int(np.sum(got != want)) <= max(2, count // 100)
+        assert a() == b(), count  # the engine advanced by exactly the draws the method makes
     big = D.normal_floats(D.Mt19937(7), 200000)
     assert abs(float(big.mean())) < 0.01 and abs(float(big.std()) - 1.0) < 0.01
+    r = D.Mt19937(3)
+    D.normal_floats(r, 0)
+    assert r() == D.Mt19937(3)()
 
 
 # ---------------------------------------------------------------- selection
