@@ -428,6 +428,34 @@ int pht_densify_split(const pht_split_params* p, const int32_t* parents, int64_t
 int pht_adc_decay(float* log_scales, float* opacity_logits, uint16_t* log_scales_lp, uint16_t* opacity_logits_lp,
                   int32_t lp_format, int64_t n, float opacity_decay, float log_scale_decay, void* stream);
 
+/* Statistics of a batched training step, accumulated inside its backward: view v of the batch is
+ * observed as accumulate_densification_stats would observe its render (its radii, visibility and
+ * refine weight, the trainer's call at photara/src/splat/trainer.cpp:2441-2449), views in order.
+ * view_index[v] is the dataset index of the batch's view v (or -1). geometry_from_refine: the
+ * refine weight is also the geometry gradient (the trainer's call). The forward must have
+ * written radii and visibility for every view. */
+typedef struct pht_densify_step {
+    pht_densify_stats stats;
+    int32_t view_index[PHT_MAX_VIEWS];
+    int32_t use_maximum;
+    int32_t require_contribution_visibility;
+    float step_score_power;
+    float oversize_screen_threshold;
+    int32_t geometry_from_refine;
+    int32_t reserved;
+} pht_densify_step;
+/* pht_backward + the step's densification statistics (computes the refine statistic). */
+int pht_backward_densify(pht_workspace* ws, const pht_gaussians* g, const pht_view_grads* dl,
+                         pht_model_grads* grads, const pht_adam* adam, const pht_densify_step* densify,
+                         void* stream);
+/* pht_step_graph_create whose captured backward is pht_backward_densify. */
+int pht_step_graph_create_densify(pht_workspace* ws, const pht_gaussians* g, const pht_camera* cams,
+                                  int n_views, const pht_settings* s, pht_view_outputs* out,
+                                  const float* const* targets, float* const* grad_color_bufs,
+                                  float* loss_sum, const pht_loss_config* loss, const pht_adam* adam,
+                                  const pht_densify_step* densify, void* stream,
+                                  pht_step_graph** out_graph);
+
 /* Host only: count draws of a fresh std::normal_distribution<float>(0, 1) over the std::mt19937
  * whose state (624 words, position in [0, 624]) the caller holds and which is advanced in place:
  * the split offsets of grow_igs_random_gpu (densification_adc_plus.cpp:113-120). */

@@ -138,6 +138,15 @@ class pht_densify_observation(C.Structure):
     ]
 
 
+class pht_densify_step(C.Structure):
+    _fields_ = [
+        ("stats", pht_densify_stats), ("view_index", C.c_int32 * 32),
+        ("use_maximum", C.c_int32), ("require_contribution_visibility", C.c_int32),
+        ("step_score_power", C.c_float), ("oversize_screen_threshold", C.c_float),
+        ("geometry_from_refine", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+
 class pht_row_buffer(C.Structure):
     _fields_ = [("src", C.c_void_p), ("dst", C.c_void_p), ("row_bytes", C.c_int64)]
 
@@ -203,6 +212,14 @@ SIGNATURES = {
     "pht_gather_rows": (C.c_int, [C.POINTER(pht_row_buffer), C.c_int, C.c_void_p, C.c_int64, C.c_void_p]),
     "pht_densify_split": (C.c_int, [C.POINTER(pht_split_params), C.c_void_p, C.c_int64, C.c_int64, C.c_void_p,
                                     C.c_void_p, C.c_void_p]),
+    "pht_backward_densify": (C.c_int, [C.c_void_p, C.POINTER(pht_gaussians), C.POINTER(pht_view_grads),
+                                       C.POINTER(pht_model_grads), C.POINTER(pht_adam), C.POINTER(pht_densify_step),
+                                       C.c_void_p]),
+    "pht_step_graph_create_densify": (C.c_int, [C.c_void_p, C.POINTER(pht_gaussians), C.POINTER(pht_camera), C.c_int,
+                                                C.POINTER(pht_settings), C.POINTER(pht_view_outputs),
+                                                C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_void_p,
+                                                C.POINTER(pht_loss_config), C.POINTER(pht_adam),
+                                                C.POINTER(pht_densify_step), C.c_void_p, C.POINTER(C.c_void_p)]),
     "pht_host_normal_floats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.c_void_p, C.c_int64]),
     "pht_adc_decay": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int64, C.c_float,
                                 C.c_float, C.c_void_p]),

@@ -335,6 +335,11 @@ int pht_forward(pht_workspace* ws, const pht_gaussians* g, const pht_camera* cam
 
 int pht_backward(pht_workspace* ws, const pht_gaussians* g, const pht_view_grads* dl, pht_model_grads* grads,
                  const pht_adam* adam, void* stream) {
+    return pht_backward_densify(ws, g, dl, grads, adam, nullptr, stream);
+}
+
+int pht_backward_densify(pht_workspace* ws, const pht_gaussians* g, const pht_view_grads* dl, pht_model_grads* grads,
+                         const pht_adam* adam, const pht_densify_step* densify, void* stream) {
     if (!ws || !g || !dl) return fail(PHT_EINVAL, "null argument");
     if (!ws->fwd_valid) return fail(PHT_ESTATE, "backward requires a completed forward on this workspace");
     if (g->count != ws->gaussians) return fail(PHT_EINVAL, "model size differs from the forward's");
@@ -349,6 +354,31 @@ int pht_backward(pht_workspace* ws, const pht_gaussians* g, const pht_view_grads
         if (adam->sh && adam->sh_first && (!adam->sh_packed || !adam->sh_bounds))
             return fail(PHT_EINVAL, "quantised SH Adam requires sh_first, sh_packed and sh_bounds");
         if (adam->lp_format != g->lp_format) return fail(PHT_EINVAL, "Adam lp_format differs from the model's");
+    }
+    DensifyStepArgs da{};
+    if (densify) {
+        const pht_densify_stats& d = densify->stats;
+        if (!d.gradient || !d.count || !d.max_screen_radius || !d.priority || !d.max_geometry_gradient ||
+            !d.first_view || !d.view_support || !d.accumulated_error)
+            return fail(PHT_EINVAL, "densify: null statistics array");
+        for (int v = 0; v < ws->n_views; ++v) {
+            const DevView& hv = ws->host_views[size_t(v)];
+            if (!hv.radii || !hv.visibility)
+                return fail(PHT_EINVAL, "densify: the forward must write radii and visibility for every view");
+            const int32_t side = std::max<int32_t>(1, std::min(hv.vi.cam.width, hv.vi.cam.height));
+            da.inverse_resolution[v] = 1.0f / float(side);
+            da.view_index[v] = densify->view_index[v];
+        }
+        da.stats = d;
+        da.rules.use_maximum = densify->use_maximum;
+        da.rules.require_contribution_visibility = densify->require_contribution_visibility;
+        da.rules.step_score_power = densify->step_score_power;
+        da.rules.oversize_screen_threshold = densify->oversize_screen_threshold;
+        da.geometry_from_refine = densify->geometry_from_refine;
+        da.views = ws->views;
+        da.n_views = ws->n_views;
+        da.sgrad = ws->sgrad;
+        da.n = ws->gaussians;
     }
     hipStream_t st = static_cast<hipStream_t>(stream);
     std::vector<DevView> hv = ws->host_views;
@@ -372,7 +402,7 @@ int pht_backward(pht_workspace* ws, const pht_gaussians* g, const pht_view_grads
     ba.counters = ws->counters;
     ba.slot_cap = ws->slot_cap;
     ba.n_gauss = ws->gaussians;
-    ba.want_refine = grads && grads->refine_weight ? 1 : 0;
+    ba.want_refine = (grads && grads->refine_weight) || densify ? 1 : 0;
     ws->mark(5, 0, st);
     launch_blend_backward(ba, int(ws->tiles_total), ws->settings.tile_size, ws->settings.batch, st);
     ws->mark(5, 1, st);
@@ -392,6 +422,7 @@ int pht_backward(pht_workspace* ws, const pht_gaussians* g, const pht_view_grads
     if (adam) ga.adam = *adam;
     ws->mark(6, 0, st);
     launch_gaussian_backward(ga, adam != nullptr, st);
+    if (densify) launch_densify_step(da, st);
     ws->mark(6, 1, st);
     return check_hip(hipGetLastError(), "backward launch");
 }
@@ -684,6 +715,15 @@ int pht_step_graph_create(pht_workspace* ws, const pht_gaussians* g, const pht_c
                           const pht_settings* s, pht_view_outputs* out, const float* const* targets,
                           float* const* grad_color_bufs, float* loss_sum, const pht_loss_config* loss,
                           const pht_adam* adam, void* stream, pht_step_graph** out_graph) {
+    return pht_step_graph_create_densify(ws, g, cams, n_views, s, out, targets, grad_color_bufs, loss_sum, loss, adam,
+                                         nullptr, stream, out_graph);
+}
+
+int pht_step_graph_create_densify(pht_workspace* ws, const pht_gaussians* g, const pht_camera* cams, int n_views,
+                                  const pht_settings* s, pht_view_outputs* out, const float* const* targets,
+                                  float* const* grad_color_bufs, float* loss_sum, const pht_loss_config* loss,
+                                  const pht_adam* adam, const pht_densify_step* densify, void* stream,
+                                  pht_step_graph** out_graph) {
     if (!out_graph || !targets || !grad_color_bufs || !adam) return fail(PHT_EINVAL, "null argument");
     pht_loss_config lc{};
     lc.kind = PHT_LOSS_L1;
@@ -726,7 +766,7 @@ int pht_step_graph_create(pht_workspace* ws, const pht_gaussians* g, const pht_c
             launch_l1_views(l1, loss_sum, lc.photometric_weight, cap_stream);
             rc = check_hip(hipGetLastError(), "l1 launch");
         }
-        if (rc == PHT_OK) rc = pht_backward(ws, g, dl.data(), nullptr, adam, cap_stream);
+        if (rc == PHT_OK) rc = pht_backward_densify(ws, g, dl.data(), nullptr, adam, densify, cap_stream);
         hipGraph_t graph = nullptr;
         const int rc_end = check_hip(hipStreamEndCapture(cap_stream, &graph), "end capture");
         if (rc == PHT_OK) rc = rc_end;

@@ -54,6 +54,35 @@ void launch_densify_accumulate(const DensifyObsSet& set, int64_t n, hipStream_t 
     k_densify_accumulate<<<blocks_for(n), kThreads, 0, st>>>(set, n);
 }
 
+// Fused into a batched backward: view v's observation of Gaussian i is its radius and visibility
+// in that view's render and the refine statistic the blend backward left in record (i, v).
+__global__ __launch_bounds__(kThreads) void k_densify_step(DensifyStepArgs a) {
+    pht_densify_observation o = a.rules;
+    for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < a.n; i += int64_t(gridDim.x) * kThreads) {
+        DensifyRow r;
+        bool loaded = false;
+        for (int v = 0; v < a.n_views; ++v) {
+            const DevView& dv = a.views[v];
+            const int32_t radius = dv.radii[i];
+            if (radius <= 0) continue;
+            if (!loaded) {
+                r = densify_load_row(a.stats, i);
+                loaded = true;
+            }
+            const float rw = a.sgrad[(int64_t(v) * a.n + i) * kGradStride + 9];
+            o.view_index = a.view_index[v];
+            densify_observe(r, o, a.inverse_resolution[v], radius, dv.visibility[i], rw, a.geometry_from_refine != 0, rw,
+                            false, 0.0f, false, 0.0f);
+        }
+        if (loaded) densify_store_row(a.stats, i, r);
+    }
+}
+
+void launch_densify_step(const DensifyStepArgs& a, hipStream_t st) {
+    if (a.n <= 0 || a.n_views <= 0) return;
+    k_densify_step<<<blocks_for(a.n), kThreads, 0, st>>>(a);
+}
+
 // ---------------------------------------------------------------- prune masks
 __global__ __launch_bounds__(kThreads) void k_prune_masks(pht_gaussians g, int sh_stride, float min_opacity,
                                                           float max_bounds, float cx, float cy, float cz,

@@ -125,6 +125,19 @@ struct DensifyObsSet {
     int n;
 };
 void launch_densify_accumulate(const DensifyObsSet& set, int64_t n, hipStream_t st);
+// The statistics of a batched backward's views, from its per-(gaussian, view) refine statistic.
+struct DensifyStepArgs {
+    pht_densify_stats stats;
+    pht_densify_observation rules; // use_maximum, visibility rule, power, oversize threshold
+    int32_t view_index[PHT_MAX_VIEWS];
+    float inverse_resolution[PHT_MAX_VIEWS];
+    int geometry_from_refine;
+    const DevView* views;
+    int n_views;
+    const float* sgrad;
+    int64_t n;
+};
+void launch_densify_step(const DensifyStepArgs& a, hipStream_t st);
 void launch_prune_masks(const pht_gaussians& g, int sh_stride, float min_opacity, float max_bounds,
                         const float center[3], uint8_t* keep, uint8_t* hard, float* opacities, hipStream_t st);
 // Device state of one radix select (in the caller's workspace, after the keys).

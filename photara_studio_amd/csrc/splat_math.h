@@ -1070,23 +1070,18 @@ PHT_HD void densify_store_row(const pht_densify_stats& s, int64_t i, const Densi
     s.accumulated_error[i] = r.accumulated_error;
     s.first_view[i] = r.first_view;
 }
-PHT_HD void densify_accumulate_row(DensifyRow& s, const pht_densify_observation& o, float inverse_resolution,
-                                   int64_t i) {
-    const int32_t radius = o.radii[i];
-    if (radius <= 0 || (o.require_contribution_visibility && o.visibility[i] <= 0.0f)) return;
-    const float rw = o.refine_weight[i];
+// One observation of one row: radius, visibility and refine weight of the view, the optional
+// geometry gradient / image error / normalised size, under the observation's rules.
+PHT_HD void densify_observe(DensifyRow& s, const pht_densify_observation& o, float inverse_resolution,
+                            int32_t radius, float visibility, float rw, bool has_geom, float geom, bool has_err,
+                            float err, bool has_size, float size) {
+    if (radius <= 0 || (o.require_contribution_visibility && visibility <= 0.0f)) return;
     const float raw = finite_(rw) ? fmaxf(rw, 0.0f) : 0.0f;
-    if (o.geometry_gradient != nullptr) {
-        const float gg = o.geometry_gradient[i];
-        if (finite_(gg)) s.max_geometry_gradient = fmaxf(s.max_geometry_gradient, gg);
-    }
+    if (has_geom && finite_(geom)) s.max_geometry_gradient = fmaxf(s.max_geometry_gradient, geom);
     const float w = o.step_score_power == 1.0f ? raw : ppowf(raw, o.step_score_power);
     s.gradient = o.use_maximum ? fmaxf(s.gradient, w) : s.gradient + w;
     s.count += 1.0f;
-    if (o.image_error != nullptr) {
-        const float e = o.image_error[i];
-        if (finite_(e)) s.accumulated_error += fmaxf(e, 0.0f);
-    }
+    if (has_err && finite_(err)) s.accumulated_error += fmaxf(err, 0.0f);
     if (o.view_index >= 0) {
         if (s.first_view == 0) {
             s.first_view = o.view_index + 1;
@@ -1095,10 +1090,20 @@ PHT_HD void densify_accumulate_row(DensifyRow& s, const pht_densify_observation&
             s.view_support = 2.0f;
         }
     }
-    const float screen = o.normalized_size != nullptr ? o.normalized_size[i] : float(radius) * inverse_resolution;
+    const float screen = has_size ? size : float(radius) * inverse_resolution;
     s.max_screen_radius = fmaxf(s.max_screen_radius, screen);
     const float thr = o.oversize_screen_threshold;
     s.priority += thr > 0.0f ? fmaxf(plog2f(screen / thr), 0.0f) : w * (1.0f + screen);
+}
+PHT_HD void densify_accumulate_row(DensifyRow& s, const pht_densify_observation& o, float inverse_resolution,
+                                   int64_t i) {
+    const int32_t radius = o.radii[i];
+    if (radius <= 0) return;
+    const bool has_geom = o.geometry_gradient != nullptr, has_err = o.image_error != nullptr;
+    const bool has_size = o.normalized_size != nullptr;
+    densify_observe(s, o, inverse_resolution, radius, o.visibility[i], o.refine_weight[i], has_geom,
+                    has_geom ? o.geometry_gradient[i] : 0.0f, has_err, has_err ? o.image_error[i] : 0.0f, has_size,
+                    has_size ? o.normalized_size[i] : 0.0f);
 }
 
 // adc_plus_prune_kernel (photara/src/splat/cuda_ops.cu:263-302): row i. Returns opacity.

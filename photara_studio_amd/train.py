@@ -49,14 +49,19 @@ class TrainStep:
                  options: RasterizeOptions, lrs: Optional[LearningRates] = None,
                  instance_capacity: Optional[int] = None, loss: str = "l1", ssim_weight: float = 0.2,
                  photometric_weight: float = 1.0, sh_optimizer: str = "quant",
-                 sh_regularization_weight: float = 0.0):
+                 sh_regularization_weight: float = 0.0, densify: bool = False,
+                 view_indices: Optional[Sequence[int]] = None, densify_screen_threshold: float = 0.5):
         """loss: "l1" (loss_kernel's L1, cuda_ops.cu:886-898) or "l1_ssim" (the trainer's default
         (1 - w) L1 + w (1 - SSIM), fused_ssim.cu:473-560, w = ssim_weight).
         sh_optimizer: "quant", the trainer's SH Adam with fp16 update + 8-bit log-second state
         (trainer.cpp:1569-1577, sh_adam_quant.cuh), or "fp32" (FP32 moments, the active-prefix Adam
         of cuda_ops.cu:1506-1546). sh_regularization_weight: the L2 weight on the non-DC SH
         coefficients applied inside the quantised step (TrainingOptions::sh_regularization_weight,
-        0.001 in the reference's defaults)."""
+        0.001 in the reference's defaults).
+        densify: accumulate the ADC-IGS densification statistics inside the step (every view of the
+        batch observed as the trainer observes its render, trainer.cpp:2441-2449, with view ids
+        ``view_indices`` and oversize threshold ``densify_screen_threshold``); ``refine()`` then
+        runs a refinement and re-binds the step to the new model."""
         if loss not in ("l1", "l1_ssim"):
             raise ValueError(f"unknown loss {loss!r}")
         if sh_optimizer not in ("quant", "fp32"):
@@ -119,6 +124,67 @@ class TrainStep:
         self.step_index = 0
         self.graph = None
         self.lib = _abi.load()
+        self.densify = bool(densify)
+        self.dstats = None
+        self.view_indices = list(view_indices) if view_indices is not None else list(range(nv))
+        if len(self.view_indices) != nv:
+            raise ValueError("one view index per camera")
+        self.densify_screen_threshold = float(densify_screen_threshold)
+        if self.densify:
+            from .densify import make_densification_stats
+            self.dstats = make_densification_stats(n, dev)
+            self._bind_densify()
+
+    def _bind_densify(self) -> None:
+        d = _abi.pht_densify_step()
+        d.stats = self.dstats.abi()
+        for k, v in enumerate(self.view_indices):
+            d.view_index[k] = int(v)
+        d.use_maximum, d.require_contribution_visibility = 1, 1
+        d.step_score_power, d.oversize_screen_threshold = 1.0, self.densify_screen_threshold
+        d.geometry_from_refine = 1
+        self._dstep = d
+
+    def _backward(self, adam, stream) -> None:
+        if self.densify:
+            check(self.lib.pht_backward_densify(self.ws.handle, C.byref(self.gauss), self.dl, None, C.byref(adam),
+                                                C.byref(self._dstep), stream))
+        else:
+            check(self.lib.pht_backward(self.ws.handle, C.byref(self.gauss), self.dl, None, C.byref(adam), stream))
+
+    def refine(self, iteration: int, options, scene_extent: float, scene_center, random):
+        """One ADC-IGS refinement of the model with this step's statistics and optimizer state
+        (densification_igs.cpp:291-515), then the step is re-bound to the new model: parameter
+        and per-view buffers, workspace capacity, statistics, and (at the next graph step) the
+        captured graph. Returns the refinement counts."""
+        from .densify import AdamStates, IgsStrategy, make_densification_stats
+        if not self.densify:
+            raise RuntimeError("refine() needs a step built with densify=True")
+        if self.graph is not None:
+            self.lib.pht_step_graph_destroy(self.graph)
+            self.graph = None
+        counts, _ = IgsStrategy().refine(self.model, self.dstats, iteration, scene_extent, scene_center, options,
+                                         random, AdamStates(self.adam, self.sh_state))
+        if self.sh_adam is not None:
+            st = self.sh_state
+            self.sh_adam.first, self.sh_adam.packed, self.sh_adam.bounds = st.first, st.packed, st.bounds
+        dev = self.model.means.device
+        n = self.model.size()
+        self.gauss = make_gaussians(self.model, self.options)
+        self.radii = [torch.empty((n,), dtype=torch.int32, device=dev) for _ in self.cameras]
+        self.visibility = [torch.empty((n,), device=dev) for _ in self.cameras]
+        for k in range(len(self.cameras)):
+            self.outs[k].radii, self.outs[k].visibility = self.radii[k].data_ptr(), self.visibility[k].data_ptr()
+        self.dstats = make_densification_stats(n, dev)
+        self._bind_densify()
+        if not self.ws.fits(n, len(self.cameras), int(self.options.tile_size), self.tiles, self.pixels,
+                            self.capacity):
+            self.ws = None
+            torch.cuda.empty_cache()
+            self.ws = Workspace(dev, int(n * 1.25), len(self.cameras), int(self.options.tile_size), self.tiles,
+                                self.pixels, self.capacity)
+        self.ensure_capacity()  # the instance list grows with the model
+        return counts
 
     def _corrections(self, step: int):
         b1, b2 = np.float32(self.adam.beta1), np.float32(self.adam.beta2)
@@ -150,7 +216,7 @@ class TrainStep:
                 check(self.lib.pht_l1_loss(color, target, grad, C.c_void_p(self.loss_sum.data_ptr()),
                                            self.colors[k][0].numel(), self.photometric_weight, stream))
         adam = make_adam(self.model, self.adam, self.sh_adam)
-        check(self.lib.pht_backward(self.ws.handle, C.byref(self.gauss), self.dl, None, C.byref(adam), stream))
+        self._backward(adam, stream)
 
     def capture(self) -> None:
         """Capture forward + L1 + backward + Adam as one hipGraph (hyper-parameters on device)."""
@@ -161,10 +227,12 @@ class TrainStep:
         grd = (C.c_void_p * nv)(*[g.data_ptr() for g in self.grad_colors])
         handle = C.c_void_p()
         stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
-        check(self.lib.pht_step_graph_create(self.ws.handle, C.byref(self.gauss), self.cams, nv,
-                                             C.byref(self.settings), self.outs, tgt, grd,
-                                             C.c_void_p(self.loss_sum.data_ptr()), C.byref(self.loss_cfg),
-                                             C.byref(self._adam_graph), stream, C.byref(handle)))
+        check(self.lib.pht_step_graph_create_densify(self.ws.handle, C.byref(self.gauss), self.cams, nv,
+                                                     C.byref(self.settings), self.outs, tgt, grd,
+                                                     C.c_void_p(self.loss_sum.data_ptr()), C.byref(self.loss_cfg),
+                                                     C.byref(self._adam_graph),
+                                                     C.byref(self._dstep) if self.densify else None, stream,
+                                                     C.byref(handle)))
         self.graph = handle
 
     def step_graph(self) -> None:
@@ -194,10 +262,11 @@ class TrainStep:
                 self.graph = None
             self.capacity = int(c.instances * headroom) + 4096
             dev = self.model.means.device
+            rows = max(self.model.size(), int(self.ws.cap.gaussians))
             self.ws = None
             torch.cuda.empty_cache()
-            self.ws = Workspace(dev, self.model.size(), len(self.cameras), int(self.options.tile_size), self.tiles,
-                                self.pixels, self.capacity)
+            self.ws = Workspace(dev, rows, len(self.cameras), int(self.options.tile_size), self.tiles, self.pixels,
+                                self.capacity)
 
     def loss(self) -> float:
         return float(self.loss_sum.item())

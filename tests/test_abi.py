@@ -34,7 +34,7 @@ def test_library_exports_every_declared_symbol():
 
 STRUCTS = ["pht_camera", "pht_settings", "pht_gaussians", "pht_view_outputs", "pht_view_grads", "pht_model_grads",
            "pht_adam", "pht_capacity", "pht_counts", "pht_loss_config", "pht_densify_stats",
-           "pht_densify_observation", "pht_row_buffer", "pht_split_params"]
+           "pht_densify_observation", "pht_row_buffer", "pht_split_params", "pht_densify_step"]
 
 
 def test_struct_layouts_match_header(tmp_path):
