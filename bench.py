@@ -144,28 +144,26 @@ def cpu_baseline(arrays, cams, opts, seconds):
                       f"in {el:.1f}s"}
 
 
-def densify_bench(step, refines=3, seed=3):
-    """Densification costs on the benchmark model: the statistics of all views of one step (one
-    launch), and ADC-IGS refinements (prune / recycle, Gumbel and ranked parent selection, row
-    remap of the model, bf16 copies, FP32 moments and quantised SH state, split, decay), wall
-    clock including the host readbacks the refinement branches on."""
+def densify_bench(model, cams, targets, opts, refines=3, seed=3):
+    """Densification costs on the benchmark model: the batched graph step with the statistics
+    accumulated inside it, the standalone statistics launch for all views, and ADC-IGS
+    refinements on the statistics the step gathered (prune / recycle, Gumbel and ranked parent
+    selection, row remap of the model, bf16 copies, FP32 moments and quantised SH state, split,
+    decay, re-binding the step), wall clock including the host readbacks the refinement branches
+    on, each followed by graph steps on the grown model (re-capture in their warm-up)."""
     from photara_studio_amd import densify as D
-    dev = step.model.means.device
+    from photara_studio_amd.train import LearningRates, TrainStep
+    dev = model.means.device
+    step = TrainStep(model, cams, targets, opts, LearningRates(), densify=True)
+    step.ensure_capacity()
+    step.capture()
+    step_s = timed(step.step_graph, 10, 2)
     gen = torch.Generator(device=dev)
     gen.manual_seed(seed)
-
-    def random_stats(n):
-        st = D.make_densification_stats(n, dev)
-        st.gradient.copy_(torch.rand((n,), generator=gen, device=dev) * 0.01)
-        st.count.copy_(torch.randint(0, 9, (n,), generator=gen, device=dev).float())
-        st.max_screen_radius.copy_(torch.rand((n,), generator=gen, device=dev) * 0.6)
-        st.priority.copy_(torch.rand((n,), generator=gen, device=dev) * 4.0)
-        return st
-
-    n = step.model.size()
+    n = model.size()
     stats = D.make_densification_stats(n, dev)
     obs = []
-    for v, cam in enumerate(step.cameras):
+    for v, cam in enumerate(cams):
         rw = torch.rand((n,), generator=gen, device=dev) * 0.01
         obs.append(D.Observation(rw, step.visibility[v], step.radii[v], cam.width, cam.height, True, True, 1.0, 0.5,
                                  rw, v))
@@ -178,20 +176,21 @@ def densify_bench(step, refines=3, seed=3):
     e1.record()
     torch.cuda.synchronize()
     acc_ms = e0.elapsed_time(e1) / reps
-    opts = D.apply_strategy_defaults(D.DensifyOptions(densification_cap=n + 400_000 * refines))
-    states = D.AdamStates(step.adam, step.sh_state)
+    del stats, obs
+    dopts = D.apply_strategy_defaults(D.DensifyOptions(densification_cap=n + 400_000 * refines))
     rows = []
     for r in range(refines):
-        stats = random_stats(step.model.size())
         before = step.model.size()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        counts, stats = D.refine_gaussians(step.model, stats, 200 * (r + 2), 1.0, [0.0, 0.0, 0.0], opts,
-                                           D.Mt19937(42 + r), states)
+        counts = step.refine(200 * (r + 2), dopts, 1.0, [0.0, 0.0, 0.0], D.Mt19937(42 + r))
         torch.cuda.synchronize()
-        rows.append({"ms": round((time.perf_counter() - t0) * 1e3, 2), "rows_in": before, "pruned": counts.pruned,
-                     "grown": counts.grown})
-    return {"accumulate_views_ms": round(acc_ms, 4), "views": len(obs), "refines": rows,
+        ms = (time.perf_counter() - t0) * 1e3
+        after_s = timed(step.step_graph, 5, 1)
+        rows.append({"ms": round(ms, 2), "rows_in": before, "pruned": counts.pruned, "grown": counts.grown,
+                     "step_ms_after": round(after_s * 1e3, 3)})
+    return {"step_ms_with_statistics": round(step_s * 1e3, 4), "accumulate_views_ms": round(acc_ms, 4),
+            "views": len(cams), "refines": rows,
             "refine_ms_median": sorted(r["ms"] for r in rows)[len(rows) // 2]}
 
 
@@ -297,11 +296,17 @@ def main():
     dens = None
     if not args.no_densify:
         # runs last on the batched step's model: a refinement replaces its arrays
-        dens = densify_bench(batched)
-        print(f"[bench] densify: statistics of {dens['views']} views {dens['accumulate_views_ms']:.3f} ms; refinements "
-              + ", ".join(f"{r['ms']:.1f} ms ({r['rows_in']} rows, -{r['pruned']} +{r['grown']})"
-                          for r in dens["refines"]), flush=True)
-    del batched
+        model = batched.model
+        del batched
+        torch.cuda.empty_cache()
+        dens = densify_bench(model, cams, targets, bopts)
+        print(f"[bench] densify: step with statistics {dens['step_ms_with_statistics']:.3f} ms; standalone statistics "
+              f"of {dens['views']} views {dens['accumulate_views_ms']:.3f} ms; refinements "
+              + ", ".join(f"{r['ms']:.1f} ms ({r['rows_in']} rows, -{r['pruned']} +{r['grown']}; then "
+                          f"{r['step_ms_after']:.2f} ms/step)" for r in dens["refines"]), flush=True)
+        del model
+    else:
+        del batched
     torch.cuda.empty_cache()
 
     cpu = None

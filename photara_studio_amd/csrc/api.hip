@@ -420,9 +420,11 @@ int pht_backward_densify(pht_workspace* ws, const pht_gaussians* g, const pht_vi
     if (grads) ga.out = *grads;
     std::memset(&ga.adam, 0, sizeof(ga.adam));
     if (adam) ga.adam = *adam;
+    ga.dens = da;
+    ga.has_dens = densify && adam ? 1 : 0; // folded into the fused-Adam kernel
     ws->mark(6, 0, st);
     launch_gaussian_backward(ga, adam != nullptr, st);
-    if (densify) launch_densify_step(da, st);
+    if (densify && !adam) launch_densify_step(da, st);
     ws->mark(6, 1, st);
     return check_hip(hipGetLastError(), "backward launch");
 }

@@ -57,7 +57,6 @@ void launch_densify_accumulate(const DensifyObsSet& set, int64_t n, hipStream_t 
 // Fused into a batched backward: view v's observation of Gaussian i is its radius and visibility
 // in that view's render and the refine statistic the blend backward left in record (i, v).
 __global__ __launch_bounds__(kThreads) void k_densify_step(DensifyStepArgs a) {
-    pht_densify_observation o = a.rules;
     for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < a.n; i += int64_t(gridDim.x) * kThreads) {
         DensifyRow r;
         bool loaded = false;
@@ -70,9 +69,8 @@ __global__ __launch_bounds__(kThreads) void k_densify_step(DensifyStepArgs a) {
                 loaded = true;
             }
             const float rw = a.sgrad[(int64_t(v) * a.n + i) * kGradStride + 9];
-            o.view_index = a.view_index[v];
-            densify_observe(r, o, a.inverse_resolution[v], radius, dv.visibility[i], rw, a.geometry_from_refine != 0, rw,
-                            false, 0.0f, false, 0.0f);
+            densify_observe(r, a.rules, a.view_index[v], a.inverse_resolution[v], radius, dv.visibility[i], rw,
+                            a.geometry_from_refine != 0, rw, false, 0.0f, false, 0.0f);
         }
         if (loaded) densify_store_row(a.stats, i, r);
     }

@@ -197,7 +197,9 @@ __device__ void shq_block(const ShqArgs& q, int64_t row_begin, int rows, int str
     }
 }
 
-template <int DEG, int LP, bool ADAM>
+// DENS: also fold each view's densification observation of the Gaussian into its statistics
+// (the refine statistic is the record's slot 9, already loaded for the geometry pass).
+template <int DEG, int LP, bool ADAM, bool DENS = false>
 __global__ __launch_bounds__(256) void k_gaussian_bwd(GaussArgs a) {
     constexpr int NB = (DEG + 1) * (DEG + 1);
     __shared__ float s_sh[kScanThreads * kShPitch];
@@ -285,6 +287,19 @@ __global__ __launch_bounds__(256) void k_gaussian_bwd(GaussArgs a) {
             chain_raw(act, hf, filt, sg, gls, gq, glogit);
         }
         if (a.out.refine_weight) a.out.refine_weight[i] = refine;
+        if constexpr (DENS) {
+            // its own pass over the views (the records are in cache), so the statistics' registers
+            // do not add to the geometry pass's
+            DensifyRow dr = densify_load_row(a.dens.stats, i);
+            for (int v = 0; v < a.n_views; ++v) {
+                if (!((mask >> v) & 1u)) continue;
+                const float rw = a.sgrad[(v * n + i) * kGradStride + 9];
+                const DevView& dv = a.views[v];
+                densify_observe(dr, a.dens.rules, a.dens.view_index[v], a.dens.inverse_resolution[v], dv.radii[i],
+                                dv.visibility[i], rw, a.dens.geometry_from_refine != 0, rw, false, 0.0f, false, 0.0f);
+            }
+            densify_store_row(a.dens.stats, i, dr);
+        }
         if (!precomp) {
             for (int v = 0; v < a.n_views; ++v) {
                 if (!((mask >> v) & 1u)) continue;
@@ -414,7 +429,8 @@ __global__ __launch_bounds__(256) void k_gaussian_bwd(GaussArgs a) {
 
 template <int DEG, int LP> static void gb_launch(const GaussArgs& a, bool adam, hipStream_t st) {
     const unsigned blocks = unsigned((a.g.count + 255) / 256);
-    if (adam) hipLaunchKernelGGL((k_gaussian_bwd<DEG, LP, true>), dim3(blocks), dim3(256), 0, st, a);
+    if (adam && a.has_dens) hipLaunchKernelGGL((k_gaussian_bwd<DEG, LP, true, true>), dim3(blocks), dim3(256), 0, st, a);
+    else if (adam) hipLaunchKernelGGL((k_gaussian_bwd<DEG, LP, true>), dim3(blocks), dim3(256), 0, st, a);
     else hipLaunchKernelGGL((k_gaussian_bwd<DEG, LP, false>), dim3(blocks), dim3(256), 0, st, a);
 }
 template <int LP> static void gb_deg(const GaussArgs& a, bool adam, hipStream_t st) {

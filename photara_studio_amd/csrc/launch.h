@@ -36,6 +36,18 @@ struct BlendArgs {
     int want_refine; // backward: accumulate the refine statistic (only when a caller reads it)
 };
 
+// The statistics of a batched backward's views, from its per-(gaussian, view) refine statistic.
+struct DensifyStepArgs {
+    pht_densify_stats stats;
+    pht_densify_observation rules; // use_maximum, visibility rule, power, oversize threshold
+    int32_t view_index[PHT_MAX_VIEWS];
+    float inverse_resolution[PHT_MAX_VIEWS];
+    int geometry_from_refine;
+    const DevView* views;
+    int n_views;
+    const float* sgrad;
+    int64_t n;
+};
 struct GaussArgs {
     pht_gaussians g;
     pht_settings s;
@@ -48,6 +60,8 @@ struct GaussArgs {
     const Counters* counters;
     pht_model_grads out;
     pht_adam adam;
+    DensifyStepArgs dens; // with the fused Adam: the step's densification statistics (has_dens)
+    int has_dens;
 };
 
 // k_gaussian.hip: standalone quantised SH Adam over rows x stride (stride <= kShqMaxStride).
@@ -125,18 +139,6 @@ struct DensifyObsSet {
     int n;
 };
 void launch_densify_accumulate(const DensifyObsSet& set, int64_t n, hipStream_t st);
-// The statistics of a batched backward's views, from its per-(gaussian, view) refine statistic.
-struct DensifyStepArgs {
-    pht_densify_stats stats;
-    pht_densify_observation rules; // use_maximum, visibility rule, power, oversize threshold
-    int32_t view_index[PHT_MAX_VIEWS];
-    float inverse_resolution[PHT_MAX_VIEWS];
-    int geometry_from_refine;
-    const DevView* views;
-    int n_views;
-    const float* sgrad;
-    int64_t n;
-};
 void launch_densify_step(const DensifyStepArgs& a, hipStream_t st);
 void launch_prune_masks(const pht_gaussians& g, int sh_stride, float min_opacity, float max_bounds,
                         const float center[3], uint8_t* keep, uint8_t* hard, float* opacities, hipStream_t st);

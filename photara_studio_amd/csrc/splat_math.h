@@ -1072,9 +1072,9 @@ PHT_HD void densify_store_row(const pht_densify_stats& s, int64_t i, const Densi
 }
 // One observation of one row: radius, visibility and refine weight of the view, the optional
 // geometry gradient / image error / normalised size, under the observation's rules.
-PHT_HD void densify_observe(DensifyRow& s, const pht_densify_observation& o, float inverse_resolution,
-                            int32_t radius, float visibility, float rw, bool has_geom, float geom, bool has_err,
-                            float err, bool has_size, float size) {
+PHT_HD void densify_observe(DensifyRow& s, const pht_densify_observation& o, int32_t view_index,
+                            float inverse_resolution, int32_t radius, float visibility, float rw, bool has_geom,
+                            float geom, bool has_err, float err, bool has_size, float size) {
     if (radius <= 0 || (o.require_contribution_visibility && visibility <= 0.0f)) return;
     const float raw = finite_(rw) ? fmaxf(rw, 0.0f) : 0.0f;
     if (has_geom && finite_(geom)) s.max_geometry_gradient = fmaxf(s.max_geometry_gradient, geom);
@@ -1082,11 +1082,11 @@ PHT_HD void densify_observe(DensifyRow& s, const pht_densify_observation& o, flo
     s.gradient = o.use_maximum ? fmaxf(s.gradient, w) : s.gradient + w;
     s.count += 1.0f;
     if (has_err && finite_(err)) s.accumulated_error += fmaxf(err, 0.0f);
-    if (o.view_index >= 0) {
+    if (view_index >= 0) {
         if (s.first_view == 0) {
-            s.first_view = o.view_index + 1;
+            s.first_view = view_index + 1;
             s.view_support = 1.0f;
-        } else if (s.first_view != o.view_index + 1) {
+        } else if (s.first_view != view_index + 1) {
             s.view_support = 2.0f;
         }
     }
@@ -1101,7 +1101,7 @@ PHT_HD void densify_accumulate_row(DensifyRow& s, const pht_densify_observation&
     if (radius <= 0) return;
     const bool has_geom = o.geometry_gradient != nullptr, has_err = o.image_error != nullptr;
     const bool has_size = o.normalized_size != nullptr;
-    densify_observe(s, o, inverse_resolution, radius, o.visibility[i], o.refine_weight[i], has_geom,
+    densify_observe(s, o, o.view_index, inverse_resolution, radius, o.visibility[i], o.refine_weight[i], has_geom,
                     has_geom ? o.geometry_gradient[i] : 0.0f, has_err, has_err ? o.image_error[i] : 0.0f, has_size,
                     has_size ? o.normalized_size[i] : 0.0f);
 }
