@@ -1,0 +1,147 @@
+"""The training step at the workloads' full sizes.
+
+- c1 (1M Gaussians, one 1920x1080 view), c4 (500k, 1024x1024) and c2 (3M, eight views of mixed
+  resolutions in one batch): against the oracle at full size. Keys, order, ranges, contributor
+  counts, final T, radii, visibility and images bit-exact. Gradients within 1e-4 relative
+  (max-abs error over max-abs value) of the float oracle for c1 and c4. At c2 (~97M instances)
+  the float oracle itself sits up to ~7e-5 from the FP64 decision-matched oracle and the GPU's
+  atomic sums differ from it in order (run to run), so the two float results can differ by the
+  sum of their errors. There the FP64 oracle is the arbiter: the GPU's worst element within 2e-4
+  of it and the bulk (99.9th percentile of the error over the max-abs value) within 1e-5; the
+  float oracle's own error and the GPU-float difference are reported. Measured over runs: the
+  worst log-scale element (a Gaussian covering much of a view) is off by 6.5e-5 in the float
+  oracle and by 0.8-1.3e-4 on the GPU, in the same direction; every other gradient is within
+  7e-5 on both.
+- The benchmark workload (3M Gaussians, eight 1080p views, ~144M instances) and c3 (6M Gaussians,
+  four 1080p views, bf16 storage): size-independent invariants of the sorted lists and images,
+  and whole training steps (graph and eager) that stay finite.
+"""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import training_options
+from test_gpu_parity import GRAD_TOL, _assert_forward_equal, _grad_images, _gpu_render, _rel
+
+pytestmark = pytest.mark.gpu
+
+THREADS = min(16, os.cpu_count() or 1)
+
+
+def _arrays(name):
+    from photara_studio_amd.scenes import WORKLOADS, synthetic_arrays
+    n, box = WORKLOADS[name]
+    return synthetic_arrays(n, box, 0)
+
+
+@pytest.mark.parametrize("name", ["c1", "c4", "c2"])
+def test_full_size_matches_oracle(cuda, oracle, name):
+    from photara_studio_amd.scenes import workload_cameras
+    arrays = _arrays(name)
+    cams = workload_cameras(name)
+    opts = training_options()
+    print(f"[fullsize] {name}: {len(arrays['means'])} gaussians, {len(cams)} views", flush=True)
+    r, model, res, lists = _gpu_render(arrays, cams, opts, cuda)
+    orc = oracle.OracleRun(oracle.Scene(arrays), cams, opts, 32, threads=THREADS, key_cap=1 << 27)
+    print(f"[fullsize] {name}: {int(orc.counts.instances)} instances", flush=True)
+    _assert_forward_equal(res, lists, orc, name)
+    gimg = _grad_images(res, 5)
+    grads = r.backward_views(model, res, [torch.from_numpy(g).to(cuda) for g in gimg])
+    og = orc.backward(gimg)
+    del orc
+    names = ("means", "log_scales", "quaternions", "opacity_logits", "sh", "refine_weight")
+    got = {k: getattr(grads, k).cpu().numpy().reshape(og[k].shape) for k in names}
+    if name != "c2":
+        for k in names:
+            err = _rel(got[k], og[k])
+            print(f"[fullsize] {name} grad {k}: {err:.2e}", flush=True)
+            assert err <= GRAD_TOL, (name, k, err)
+        return
+    o64 = oracle.OracleRun(oracle.Scene(arrays), cams, opts, 65, threads=THREADS).backward(gimg)
+    for k in names:
+        e_gpu, e_o32, e_pair = _rel(got[k], o64[k]), _rel(og[k], o64[k]), _rel(got[k], og[k])
+        print(f"[fullsize] c2 grad {k}: gpu vs fp64 {e_gpu:.2e}, float oracle vs fp64 {e_o32:.2e}, "
+              f"gpu vs float oracle {e_pair:.2e}", flush=True)
+        bulk = float(np.quantile(np.abs(got[k].astype(np.float64) - o64[k]), 0.999) / np.abs(o64[k]).max())
+        print(f"[fullsize] c2 grad {k}: 99.9th percentile vs fp64 {bulk:.2e}", flush=True)
+        assert e_gpu <= 2 * GRAD_TOL, (k, e_gpu, e_o32)
+        assert bulk <= 1e-5, (k, bulk)
+
+
+def _check_invariants(res, lists, cams, ts):
+    keys, vals, ranges = lists["keys"], lists["values"], lists["ranges"]
+    ni = int(lists["counts"].instances)
+    assert len(keys) == ni and ni > 0
+    assert np.all(keys[1:] >= keys[:-1]), "keys not sorted"
+    tiles = len(ranges) // 2
+    start, end = ranges[0::2].astype(np.int64), ranges[1::2].astype(np.int64)
+    nonempty = end > start
+    assert np.all(end >= start)
+    assert int((end - start).sum()) == ni                       # the ranges cover every instance once
+    s, e = start[nonempty], end[nonempty]
+    assert s[0] == 0 and e[-1] == ni and np.all(s[1:] == e[:-1])  # in tile order, no gaps
+    tile_of = (keys >> np.uint64(32)).astype(np.int64)
+    t_ids = np.flatnonzero(nonempty)
+    assert np.all(tile_of[s] == t_ids) and np.all(tile_of[e - 1] == t_ids)
+    assert int(vals.max()) < len(res[0].radii)
+    # per pixel: contributors within the tile's list, alpha = 1 - T
+    off = 0
+    tile_base = 0
+    for k, (r, c) in enumerate(zip(res, cams)):
+        h, w = c.height, c.width
+        nc = lists["n_contrib"][off:off + h * w].reshape(h, w).astype(np.int64)
+        ft = lists["final_T"][off:off + h * w].reshape(h, w)
+        gx, gy = (w + ts - 1) // ts, (h + ts - 1) // ts
+        tidx = tile_base + (np.arange(h)[:, None] // ts) * gx + (np.arange(w)[None, :] // ts)
+        assert np.all(nc <= (end - start)[tidx])
+        assert np.all((ft >= 0) & (ft <= 1))
+        np.testing.assert_array_equal(r.alpha.cpu().numpy(), (1.0 - ft).astype(np.float32))
+        col = r.color.cpu().numpy()
+        assert np.all(np.isfinite(col)) and col.min() >= 0.0
+        visible = int((r.radii > 0).sum())
+        assert visible > 0 and int((r.visibility > 0).sum()) <= visible
+        off += h * w
+        tile_base += gx * gy
+    assert tile_base == tiles
+
+
+@pytest.mark.parametrize("name,lp", [("bench8", False), ("c3", True)])
+def test_full_size_invariants_and_steps(cuda, name, lp):
+    from photara_studio_amd.rasterizer import debug_lists
+    from photara_studio_amd.scenes import model_from, synthetic_torch, workload_cameras
+    from photara_studio_amd.train import LearningRates, TrainStep, to_low_precision
+    from photara_studio_amd.rasterizer import Rasterizer
+    n = 6_000_000 if name == "c3" else 3_000_000
+    cams = workload_cameras(name)
+    opts = training_options()
+    arrays = synthetic_torch(n, 3.0, 0, cuda)
+    model = model_from(arrays, cuda)
+    if lp:
+        to_low_precision(model, torch.bfloat16)
+    print(f"[fullsize] {name}: forward of {n} gaussians x {len(cams)} views", flush=True)
+    r = Rasterizer()
+    res = r.forward_views(model, cams, opts)
+    lists = debug_lists(res[0])
+    print(f"[fullsize] {name}: {int(lists['counts'].instances)} instances", flush=True)
+    _check_invariants(res, lists, cams, int(opts.tile_size))
+    # per-view slice of the batch equals the single-view render
+    one = r.forward_views(model, [cams[1]], opts)[0]
+    np.testing.assert_array_equal(one.color.cpu().numpy(), res[1].color.cpu().numpy())
+    del res, lists, one, r
+    torch.cuda.empty_cache()
+    rng = np.random.default_rng(2)
+    targets = [torch.from_numpy(rng.random((3, c.height, c.width)).astype(np.float32)).to(cuda) for c in cams]
+    st = TrainStep(model, cams, targets, opts, LearningRates())
+    st.ensure_capacity()
+    losses = []
+    for k in range(3):
+        print(f"[fullsize] {name}: step {k}", flush=True)
+        st.step_graph() if k else st.step_eager()
+        losses.append(st.loss())
+    torch.cuda.synchronize()
+    assert all(np.isfinite(losses)) and not st.counts().overflow
+    assert bool(torch.isfinite(st.model.means).all()) and bool(torch.isfinite(st.model.sh).all())
