@@ -144,7 +144,7 @@ def cpu_baseline(arrays, cams, opts, seconds):
                       f"in {el:.1f}s"}
 
 
-def densify_bench(model, cams, targets, opts, refines=3, seed=3):
+def densify_bench(model, cams, targets, opts, refines=3, seed=3, sh_optimizer="fp32"):
     """Densification costs on the benchmark model: the batched graph step with the statistics
     accumulated inside it, the standalone statistics launch for all views, and ADC-IGS
     refinements on the statistics the step gathered (prune / recycle, Gumbel and ranked parent
@@ -154,7 +154,7 @@ def densify_bench(model, cams, targets, opts, refines=3, seed=3):
     from photara_studio_amd import densify as D
     from photara_studio_amd.train import LearningRates, TrainStep
     dev = model.means.device
-    step = TrainStep(model, cams, targets, opts, LearningRates(), densify=True)
+    step = TrainStep(model, cams, targets, opts, LearningRates(), densify=True, sh_optimizer=sh_optimizer)
     step.ensure_capacity()
     step.capture()
     step_s = timed(step.step_graph, 10, 2)
@@ -205,6 +205,9 @@ def main():
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-densify", action="store_true", help="skip the densification timings")
+    p.add_argument("--sh-optimizer", choices=("fp32", "quant"), default="fp32",
+                   help="SH optimizer of the batched step: FP32 moments, the plain step's (default), or the "
+                        "reference trainer's quantised state (also timed beside it as batched_quant_sh)")
     p.add_argument("--workload", choices=("ring", "c2"), default="ring",
                    help="ring: --views 1920x1080 views on a radius-8 ring; c2: 2x1080p + 3x720p + 3x540p")
     p.add_argument("--tile", type=int, default=16)
@@ -264,7 +267,7 @@ def main():
         bopts, tune_rows = autotune(model, cams, targets, opts, LearningRates(), steps=3, warmup=1,
                                     log=lambda m: print(m, flush=True))
         print(f"[bench] autotuned: tile {bopts.tile_size}, batch {bopts.batch}", flush=True)
-    batched = TrainStep(model, cams, targets, bopts, LearningRates())
+    batched = TrainStep(model, cams, targets, bopts, LearningRates(), sh_optimizer=args.sh_optimizer)
     batched.ensure_capacity()
     batched.capture()
     step_s = timed(batched.step_graph, args.steps, args.warmup)
@@ -284,7 +287,8 @@ def main():
     pixels = sum(c.width * c.height for c in cams)
     bt = int(bopts.tile_size)
     tiles = sum(((c.width + bt - 1) // bt) * ((c.height + bt - 1) // bt) for c in cams)
-    ab = algorithmic_bytes(n, nv, int(bc.visible), int(bc.instances), pixels, tiles, 2, True, sh_quant=True)
+    ab = algorithmic_bytes(n, nv, int(bc.visible), int(bc.instances), pixels, tiles, 2, True,
+                           sh_quant=args.sh_optimizer == "quant")
     dominant = max(acc, key=acc.get)
     dom_gbps = ab[dominant] / (acc[dominant] * 1e-3) / 1e9
     step_bytes = sum(ab.values())
@@ -293,20 +297,29 @@ def main():
     traffic, traffic_src = profiled_traffic(dominant, workload_key)
     print(f"[bench] stages ms: " + ", ".join(f"{k}={v:.3f}" for k, v in acc.items()), flush=True)
 
+    # the same batched step with the other SH optimizer, on a fresh model
+    other_sh = "quant" if args.sh_optimizer == "fp32" else "fp32"
+    del batched
+    torch.cuda.empty_cache()
+    alt = TrainStep(to_low_precision(model_from(arrays, dev), torch.bfloat16), cams, targets, bopts, LearningRates(),
+                    sh_optimizer=other_sh)
+    alt.ensure_capacity()
+    alt.capture()
+    alt_s = timed(alt.step_graph, args.steps, args.warmup)
+    print(f"[bench] batched bf16 graph, {other_sh} SH optimizer: {alt_s * 1e3:.3f} ms/step, "
+          f"{nv / alt_s:.2f} view-steps/s", flush=True)
+    del alt
+    torch.cuda.empty_cache()
+
     dens = None
     if not args.no_densify:
-        # runs last on the batched step's model: a refinement replaces its arrays
-        model = batched.model
-        del batched
-        torch.cuda.empty_cache()
-        dens = densify_bench(model, cams, targets, bopts)
+        # runs last: a refinement replaces the model's arrays
+        dens = densify_bench(model, cams, targets, bopts, sh_optimizer=args.sh_optimizer)
         print(f"[bench] densify: step with statistics {dens['step_ms_with_statistics']:.3f} ms; standalone statistics "
               f"of {dens['views']} views {dens['accumulate_views_ms']:.3f} ms; refinements "
               + ", ".join(f"{r['ms']:.1f} ms ({r['rows_in']} rows, -{r['pruned']} +{r['grown']}; then "
                           f"{r['step_ms_after']:.2f} ms/step)" for r in dens["refines"]), flush=True)
-        del model
-    else:
-        del batched
+    del model
     torch.cuda.empty_cache()
 
     cpu = None
@@ -332,12 +345,15 @@ def main():
         "config": {"workload": f"3dgs-train-step {n} gaussians x {nv} views {res_label}, bf16 storage, hipGraph",
                    "gaussians": n, "views": nv, "resolution": res_label, "tile": int(bopts.tile_size),
                    "batch": int(bopts.batch), "storage": "bf16 sh/log_scales/opacity, fp32 masters",
-                   "sh_optimizer": "quantised (fp16 update + 8-bit log-second), fp32 structure moments",
+                   "sh_optimizer": ("FP32 moments (as the plain step)" if args.sh_optimizer == "fp32" else
+                                    "quantised (fp16 update + 8-bit log-second)") + ", fp32 structure moments",
                    "parallelism": "none (1 GPU)"},
         "autotune": [[r.tile_size, r.batch, round(r.ms_per_step, 3)] for r in tune_rows] if tune_rows else None,
         "plain_fp32": {"ms_per_step": round(plain_s * 1e3, 4), "steps_per_s": round(1.0 / plain_s, 3),
                        "instances": int(pc.instances)},
         "speedup_vs_plain_fp32": round((nv / step_s) * plain_s, 3),
+        f"batched_{other_sh}_sh": {"ms_per_step": round(alt_s * 1e3, 4), "view_steps_per_s": round(nv / alt_s, 3),
+                                   "speedup_vs_plain_fp32": round((nv / alt_s) * plain_s, 3)},
         "step_algorithmic_GBps": round(step_bytes / step_s / 1e9, 1),
         "counts": {"visible": int(bc.visible), "instances": int(bc.instances), "fp64_conics": int(bc.fp64_conics)},
         "stage_ms": {k: round(v, 4) for k, v in acc.items()},
