@@ -74,6 +74,8 @@ struct pht_workspace {
     uint32_t* tkeys[2] = {nullptr, nullptr};
     uint32_t* tvals[2] = {nullptr, nullptr};
     uint2* ranges = nullptr;
+    uint32_t* tile_order = nullptr;
+    uint32_t* order_hist = nullptr;
     float* final_T = nullptr;
     uint32_t* n_contrib = nullptr;
     uint32_t* scan_scratch = nullptr;
@@ -125,6 +127,8 @@ static void carve_workspace(pht_workspace& w, char* base, const pht_capacity& ca
         w.tvals[k] = c.take<uint32_t>(inst);
     }
     w.ranges = c.take<uint2>(size_t(cap.tiles));
+    w.tile_order = c.take<uint32_t>(size_t(cap.tiles));
+    w.order_hist = c.take<uint32_t>(512);
     w.final_T = c.take<float>(size_t(cap.pixels));
     w.n_contrib = c.take<uint32_t>(size_t(cap.pixels));
     w.scan_scratch = c.take<uint32_t>(kScanBlocks);
@@ -290,9 +294,11 @@ int pht_forward(pht_workspace* ws, const pht_gaussians* g, const pht_camera* cam
     const int tsel = launch_radix_sort(ws->tkeys, ws->tvals, nullptr, &ws->counters->instances, ws->inst_cap, tbits,
                                        ws->hist, ws->digit_total, st);
     launch_ranges(ws->tkeys[tsel], ws->counters, ws->inst_cap, uint32_t(tiles), ws->ranges, st);
+    launch_tile_order(ws->ranges, uint32_t(tiles), ws->tile_order, ws->order_hist, st);
     ws->mark(3, 1, st);
     BlendArgs ba;
     ba.ranges = ws->ranges;
+    ba.tile_order = ws->tile_order;
     ba.inst_vals = ws->tvals[tsel];
     ba.rec = ws->rec;
     ba.views = ws->views;
@@ -390,6 +396,7 @@ int pht_backward_densify(pht_workspace* ws, const pht_gaussians* g, const pht_vi
     launch_upload_views(ws->views, hv.data(), ws->n_views, st);
     BlendArgs ba;
     ba.ranges = ws->ranges;
+    ba.tile_order = ws->tile_order;
     ba.inst_vals = ws->tvals[ws->tile_sel];
     ba.rec = ws->rec;
     ba.views = ws->views;

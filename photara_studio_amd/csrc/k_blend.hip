@@ -14,6 +14,8 @@
 //            count (T_before = T_after / (1 - alpha)), with no per-bucket snapshots. Each lane
 //            sums its pixels' terms, the wave reduce-scatters the ten values (wave_scatter10),
 //            and ten lanes commit them with one global atomic each per (instance, tile).
+#include <algorithm>
+
 #include "launch.h"
 
 namespace pht {
@@ -110,7 +112,7 @@ struct TileCoord {
 
 __device__ __forceinline__ TileCoord tile_of_block(const BlendArgs& a, const DevView*& dv) {
     TileCoord c;
-    c.tile = xcd_contiguous(int(blockIdx.x), int(gridDim.x));
+    c.tile = a.tile_order ? int(a.tile_order[blockIdx.x]) : xcd_contiguous(int(blockIdx.x), int(gridDim.x));
     dv = &view_of_tile(a.views, a.n_views, c.tile, c.v);
     const int local = c.tile - dv->vi.tile_base;
     c.tx = local % dv->vi.grid_x;
@@ -393,6 +395,67 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendArgs a) {
 template <int TS, int BATCH> static void fwd_launch(const BlendArgs& a, int n_tiles, hipStream_t st) {
     hipLaunchKernelGGL((k_blend_fwd<TS, BATCH>), dim3(n_tiles), dim3(64), 0, st, a);
 }
+// ---------------------------------------------------------------- dispatch order of the tiles
+// Blocks start in blockIdx order. A heavy tile that starts last runs as a lone wave after the rest
+// of the grid has drained, so the blend launches take tiles longest first: a counting sort of the
+// tiles by instance count (buckets of 128 instances, the last bucket open-ended), heaviest bucket
+// first. Any order gives the same pixels; only the schedule changes.
+constexpr int kOrderBuckets = 256;
+__device__ __forceinline__ uint32_t order_bucket(const uint2* ranges, uint32_t t) {
+    const uint32_t b = (ranges[t].y - ranges[t].x) >> 7;
+    return uint32_t(kOrderBuckets - 1) - (b < uint32_t(kOrderBuckets - 1) ? b : uint32_t(kOrderBuckets - 1));
+}
+__global__ void k_order_clear(uint32_t* hist) {
+    for (int i = threadIdx.x; i < 2 * kOrderBuckets; i += blockDim.x) hist[i] = 0;
+}
+__global__ __launch_bounds__(256) void k_order_hist(const uint2* ranges, uint32_t n_tiles, uint32_t* hist) {
+    __shared__ uint32_t h[kOrderBuckets];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    for (uint32_t t = blockIdx.x * 256 + threadIdx.x; t < n_tiles; t += gridDim.x * 256)
+        atomicAdd(&h[order_bucket(ranges, t)], 1u);
+    __syncthreads();
+    if (h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+}
+__global__ __launch_bounds__(256) void k_order_scan(uint32_t* hist) {
+    __shared__ uint32_t wave_tot[4];
+    const uint32_t c = hist[threadIdx.x];
+    const uint32_t incl_w = wave_inclusive_scan(c);
+    if (lane_id() == 63) wave_tot[threadIdx.x >> 6] = incl_w;
+    __syncthreads();
+    uint32_t before = 0;
+    for (int w = 0; w < int(threadIdx.x >> 6); ++w) before += wave_tot[w];
+    hist[kOrderBuckets + threadIdx.x] = before + incl_w - c; // exclusive cursor of the bucket
+}
+// Ranks within a block come from LDS atomics; each (block, bucket) takes its global range with
+// one atomic, so the few buckets that hold most tiles are not a serial hot spot.
+__global__ __launch_bounds__(256) void k_order_scatter(const uint2* ranges, uint32_t n_tiles, uint32_t* hist,
+                                                       uint32_t* order) {
+    __shared__ uint32_t h[kOrderBuckets], base[kOrderBuckets];
+    for (uint32_t t0 = blockIdx.x * 256; t0 < n_tiles; t0 += gridDim.x * 256) {
+        h[threadIdx.x] = 0;
+        __syncthreads();
+        const uint32_t t = t0 + threadIdx.x;
+        const bool own = t < n_tiles;
+        const uint32_t b = own ? order_bucket(ranges, t) : 0u;
+        const uint32_t local = own ? atomicAdd(&h[b], 1u) : 0u;
+        __syncthreads();
+        const uint32_t c = h[threadIdx.x];
+        if (c) base[threadIdx.x] = atomicAdd(&hist[kOrderBuckets + threadIdx.x], c);
+        __syncthreads();
+        if (own) order[base[b] + local] = t;
+        __syncthreads();
+    }
+}
+
+void launch_tile_order(const uint2* ranges, uint32_t n_tiles, uint32_t* order, uint32_t* hist, hipStream_t st) {
+    const unsigned blocks = unsigned(std::min<uint32_t>((n_tiles + 255) / 256, 256u));
+    k_order_clear<<<1, 256, 0, st>>>(hist);
+    k_order_hist<<<blocks, 256, 0, st>>>(ranges, n_tiles, hist);
+    k_order_scan<<<1, 256, 0, st>>>(hist);
+    k_order_scatter<<<blocks, 256, 0, st>>>(ranges, n_tiles, hist, order);
+}
+
 template <int TS, int BATCH> static void bwd_launch(const BlendArgs& a, int n_tiles, hipStream_t st) {
     if (a.want_refine) hipLaunchKernelGGL((k_blend_bwd<TS, BATCH, true>), dim3(n_tiles), dim3(64), 0, st, a);
     else hipLaunchKernelGGL((k_blend_bwd<TS, BATCH, false>), dim3(n_tiles), dim3(64), 0, st, a);

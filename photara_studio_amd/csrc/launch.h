@@ -34,6 +34,7 @@ struct BlendArgs {
     uint32_t slot_cap;
     int64_t n_gauss;
     int want_refine; // backward: accumulate the refine statistic (only when a caller reads it)
+    const uint32_t* tile_order; // tile of each block (heaviest first), or null for the XCD-contiguous map
 };
 
 // The statistics of a batched backward's views, from its per-(gaussian, view) refine statistic.
@@ -104,6 +105,9 @@ void launch_debug_splat_grads(const float* sgrad, const uint32_t* view_mask, int
                               float* out, hipStream_t st);
 
 // k_blend.hip
+// order: n_tiles entries; hist: 512 words of scratch.
+// The tiles heaviest first, by instance count.
+void launch_tile_order(const uint2* ranges, uint32_t n_tiles, uint32_t* order, uint32_t* hist, hipStream_t st);
 void launch_blend_forward(const BlendArgs& a, int n_tiles, int tile_size, int batch, hipStream_t st);
 void launch_blend_backward(const BlendArgs& a, int n_tiles, int tile_size, int batch, hipStream_t st);
 
