@@ -197,6 +197,28 @@ __device__ void shq_block(const ShqArgs& q, int64_t row_begin, int rows, int str
     }
 }
 
+// Adam over one structure array of the block's rows (W floats per row), flat and coalesced: element
+// e of the block's range is row e / W, component e % W, with its gradient at
+// s_g[row * kStructGrads + first + component]. The reduced-precision mirror is rewritten when given.
+constexpr int kStructGrads = 11;
+template <int W, int LP>
+__device__ __forceinline__ void struct_adam_pass(float* P, float* M, float* V, uint16_t* L, int rows, const float* s_g,
+                                                 int first, float lr, const pht_adam& A, float c1, float c2, float lo,
+                                                 float hi) {
+    for (int e = int(threadIdx.x); e < rows * W; e += kScanThreads) {
+        const int row = e / W, k = e - row * W;
+        float p = P[e], m = M[e], v = V[e];
+        adam_update(p, m, v, s_g[row * kStructGrads + first + k], lr, A.beta1, A.beta2, c1, c2, A.epsilon, lo, hi,
+                    false);
+        P[e] = p;
+        M[e] = m;
+        V[e] = v;
+        if constexpr (LP != PHT_LP_NONE) {
+            if (L) L[e] = f32_to_lp(p, LP);
+        }
+    }
+}
+
 // DENS: also fold each view's densification observation of the Gaussian into its statistics
 // (the refine statistic is the record's slot 9, already loaded for the geometry pass).
 template <int DEG, int LP, bool ADAM, bool DENS = false>
@@ -345,27 +367,6 @@ __global__ __launch_bounds__(256) void k_gaussian_bwd(GaussArgs a) {
             c2 = A.device_hparams[1];
             lr_means = A.device_hparams[2];
         }
-        if (live) {
-            for (int k = 0; k < 3; ++k) {
-                const int64_t j = 3 * i + k;
-                adam_update(A.means[j], A.m_means[j], A.v_means[j], sg.mean[k], lr_means, A.beta1, A.beta2, c1, c2,
-                            A.epsilon, -inf, inf, false);
-                adam_update(A.log_scales[j], A.m_log_scales[j], A.v_log_scales[j], gls[k], A.lr_scales, A.beta1,
-                            A.beta2, c1, c2, A.epsilon, A.min_log_scale, A.max_log_scale, false);
-            }
-            for (int k = 0; k < 4; ++k) {
-                const int64_t j = 4 * i + k;
-                adam_update(A.quaternions[j], A.m_quaternions[j], A.v_quaternions[j], gq[k], A.lr_quaternions,
-                            A.beta1, A.beta2, c1, c2, A.epsilon, -inf, inf, false);
-            }
-            adam_update(A.opacity_logits[i], A.m_opacity_logits[i], A.v_opacity_logits[i], glogit, A.lr_opacity,
-                        A.beta1, A.beta2, c1, c2, A.epsilon, -12.0f, 12.0f, false);
-            if constexpr (LP != PHT_LP_NONE) {
-                for (int k = 0; k < 3; ++k)
-                    if (A.log_scales_lp) A.log_scales_lp[3 * i + k] = f32_to_lp(A.log_scales[3 * i + k], LP);
-                if (A.opacity_logits_lp) A.opacity_logits_lp[i] = f32_to_lp(A.opacity_logits[i], LP);
-            }
-        }
         if (A.sh && use_sh && A.sh_first) {
             ShqArgs qa;
             qa.param = A.sh;
@@ -424,6 +425,32 @@ __global__ __launch_bounds__(256) void k_gaussian_bwd(GaussArgs a) {
                 }
             });
         }
+        // Structure parameters: the rows' 11 gradients go through LDS (the SH rows are done with
+        // it), then each parameter array of the block's contiguous rows is updated in one
+        // coalesced pass, as the SH values are; per-lane rows of 3, 4 and 1 floats made every
+        // load and store a strided access.
+        __syncthreads();
+        float* s_g = s_sh; // [row][kStructGrads]: mean 0-2, log-scale 3-5, quaternion 6-9, logit 10
+        if (live) {
+            float* d = s_g + threadIdx.x * kStructGrads;
+            for (int k = 0; k < 3; ++k) {
+                d[k] = sg.mean[k];
+                d[3 + k] = gls[k];
+            }
+            for (int k = 0; k < 4; ++k) d[6 + k] = gq[k];
+            d[10] = glogit;
+        }
+        __syncthreads();
+        struct_adam_pass<3, LP>(A.means + 3 * i0, A.m_means + 3 * i0, A.v_means + 3 * i0, nullptr, rows, s_g, 0,
+                                lr_means, A, c1, c2, -inf, inf);
+        struct_adam_pass<3, LP>(A.log_scales + 3 * i0, A.m_log_scales + 3 * i0, A.v_log_scales + 3 * i0,
+                                A.log_scales_lp ? A.log_scales_lp + 3 * i0 : nullptr, rows, s_g, 3, A.lr_scales, A,
+                                c1, c2, A.min_log_scale, A.max_log_scale);
+        struct_adam_pass<4, LP>(A.quaternions + 4 * i0, A.m_quaternions + 4 * i0, A.v_quaternions + 4 * i0, nullptr,
+                                rows, s_g, 6, A.lr_quaternions, A, c1, c2, -inf, inf);
+        struct_adam_pass<1, LP>(A.opacity_logits + i0, A.m_opacity_logits + i0, A.v_opacity_logits + i0,
+                                A.opacity_logits_lp ? A.opacity_logits_lp + i0 : nullptr, rows, s_g, 10, A.lr_opacity,
+                                A, c1, c2, -12.0f, 12.0f);
     }
 }
 
