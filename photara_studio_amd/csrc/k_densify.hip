@@ -82,24 +82,52 @@ void launch_densify_step(const DensifyStepArgs& a, hipStream_t st) {
 }
 
 // ---------------------------------------------------------------- prune masks
+// One row per thread, 256 contiguous rows per block. The rows' SH values (the bulk of the bytes)
+// are checked for finiteness by the whole block in one coalesced pass over its contiguous range,
+// the result kept per row in LDS; a lane walking its own row would touch 64 lines per load.
 __global__ __launch_bounds__(kThreads) void k_prune_masks(pht_gaussians g, int sh_stride, float min_opacity,
                                                           float max_bounds, float cx, float cy, float cz,
                                                           uint8_t* keep, uint8_t* hard, float* opacities) {
+    __shared__ uint32_t s_bad[kThreads];
     const float center[3] = {cx, cy, cz};
-    for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < g.count; i += int64_t(gridDim.x) * kThreads) {
-        bool k, h;
-        opacities[i] = densify_prune_row(g.means, g.log_scales, g.quaternions, g.opacity_logits, g.sh, sh_stride, i,
-                                         min_opacity, max_bounds, center, k, h);
-        keep[i] = k ? 1 : 0;
-        hard[i] = h ? 1 : 0;
+    const int64_t i0 = int64_t(blockIdx.x) * kThreads;
+    const int rows = int(g.count - i0 < kThreads ? g.count - i0 : kThreads);
+    s_bad[threadIdx.x] = 0;
+    __syncthreads();
+    if (sh_stride > 0) {
+        const float* sh = g.sh + i0 * sh_stride;
+        const int total = rows * sh_stride;
+        if ((sh_stride & 3) == 0 && (reinterpret_cast<uintptr_t>(sh) & 15u) == 0) {
+            const float4* sh4 = reinterpret_cast<const float4*>(sh);
+            for (int c = int(threadIdx.x); c < (total >> 2); c += kThreads) {
+                const float4 v = sh4[c];
+                if (!(finite_(v.x) && finite_(v.y) && finite_(v.z) && finite_(v.w))) s_bad[(4 * c) / sh_stride] = 1u;
+            }
+        } else {
+            for (int e = int(threadIdx.x); e < total; e += kThreads)
+                if (!finite_(sh[e])) s_bad[e / sh_stride] = 1u;
+        }
     }
+    __syncthreads();
+    const int64_t i = i0 + threadIdx.x;
+    if (i >= g.count) return;
+    bool k, h;
+    opacities[i] = densify_prune_row(g.means, g.log_scales, g.quaternions, g.opacity_logits, nullptr, 0, i,
+                                     min_opacity, max_bounds, center, k, h);
+    if (s_bad[threadIdx.x]) {
+        h = true;
+        k = false;
+    }
+    keep[i] = k ? 1 : 0;
+    hard[i] = h ? 1 : 0;
 }
 
 void launch_prune_masks(const pht_gaussians& g, int sh_stride, float min_opacity, float max_bounds,
                         const float center[3], uint8_t* keep, uint8_t* hard, float* opacities, hipStream_t st) {
     if (g.count <= 0) return;
-    k_prune_masks<<<blocks_for(g.count), kThreads, 0, st>>>(g, sh_stride, min_opacity, max_bounds, center[0],
-                                                            center[1], center[2], keep, hard, opacities);
+    const unsigned blocks = unsigned((g.count + kThreads - 1) / kThreads); // one row per thread
+    k_prune_masks<<<blocks, kThreads, 0, st>>>(g, sh_stride, min_opacity, max_bounds, center[0], center[1], center[2],
+                                               keep, hard, opacities);
 }
 
 // ---------------------------------------------------------------- radix select

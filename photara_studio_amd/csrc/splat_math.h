@@ -1120,7 +1120,8 @@ PHT_HD float densify_prune_row(const float* means, const float* log_scales, cons
         max_scale = fmaxf(max_scale, pexpf(ls));
     }
     for (int k = 0; k < 4; ++k) bad = bad || !finite_(quats[4 * i + k]);
-    for (int k = 0; k < sh_stride; ++k) bad = bad || !finite_(sh[i * sh_stride + k]);
+    if (sh != nullptr) // (a caller may check the SH values itself and pass nullptr)
+        for (int k = 0; k < sh_stride; ++k) bad = bad || !finite_(sh[i * sh_stride + k]);
     const bool outside = fabsf(means[3 * i] - center[0]) > maximum_bounds ||
                          fabsf(means[3 * i + 1] - center[1]) > maximum_bounds ||
                          fabsf(means[3 * i + 2] - center[2]) > maximum_bounds;
