@@ -205,6 +205,8 @@ def main():
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-densify", action="store_true", help="skip the densification timings")
+    p.add_argument("--no-alt", action="store_true",
+                   help="skip timing the batched step with the other SH optimizer (profiling runs)")
     p.add_argument("--sh-optimizer", choices=("fp32", "quant"), default="fp32",
                    help="SH optimizer of the batched step: FP32 moments, the plain step's (default), or the "
                         "reference trainer's quantised state (also timed beside it as batched_quant_sh)")
@@ -301,15 +303,17 @@ def main():
     other_sh = "quant" if args.sh_optimizer == "fp32" else "fp32"
     del batched
     torch.cuda.empty_cache()
-    alt = TrainStep(to_low_precision(model_from(arrays, dev), torch.bfloat16), cams, targets, bopts, LearningRates(),
-                    sh_optimizer=other_sh)
-    alt.ensure_capacity()
-    alt.capture()
-    alt_s = timed(alt.step_graph, args.steps, args.warmup)
-    print(f"[bench] batched bf16 graph, {other_sh} SH optimizer: {alt_s * 1e3:.3f} ms/step, "
-          f"{nv / alt_s:.2f} view-steps/s", flush=True)
-    del alt
-    torch.cuda.empty_cache()
+    alt_s = None
+    if not args.no_alt:
+        alt = TrainStep(to_low_precision(model_from(arrays, dev), torch.bfloat16), cams, targets, bopts,
+                        LearningRates(), sh_optimizer=other_sh)
+        alt.ensure_capacity()
+        alt.capture()
+        alt_s = timed(alt.step_graph, args.steps, args.warmup)
+        print(f"[bench] batched bf16 graph, {other_sh} SH optimizer: {alt_s * 1e3:.3f} ms/step, "
+              f"{nv / alt_s:.2f} view-steps/s", flush=True)
+        del alt
+        torch.cuda.empty_cache()
 
     dens = None
     if not args.no_densify:
@@ -352,8 +356,9 @@ def main():
         "plain_fp32": {"ms_per_step": round(plain_s * 1e3, 4), "steps_per_s": round(1.0 / plain_s, 3),
                        "instances": int(pc.instances)},
         "speedup_vs_plain_fp32": round((nv / step_s) * plain_s, 3),
-        f"batched_{other_sh}_sh": {"ms_per_step": round(alt_s * 1e3, 4), "view_steps_per_s": round(nv / alt_s, 3),
-                                   "speedup_vs_plain_fp32": round((nv / alt_s) * plain_s, 3)},
+        f"batched_{other_sh}_sh": None if alt_s is None else {
+            "ms_per_step": round(alt_s * 1e3, 4), "view_steps_per_s": round(nv / alt_s, 3),
+            "speedup_vs_plain_fp32": round((nv / alt_s) * plain_s, 3)},
         "step_algorithmic_GBps": round(step_bytes / step_s / 1e9, 1),
         "counts": {"visible": int(bc.visible), "instances": int(bc.instances), "fp64_conics": int(bc.fp64_conics)},
         "stage_ms": {k: round(v, 4) for k, v in acc.items()},
