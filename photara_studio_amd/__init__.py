@@ -6,10 +6,13 @@ reference rasterizer API and a torch.autograd.Function. The kernels live in
 to a CPU implementation.
 """
 from .rasterizer import (Camera, GaussianModel, ModelGradients, RasterizationPolicy, RasterizeOptions,
-                         Rasterizer, RenderResult, StructureAdamUpdate, configure_rasterization)
+                         Rasterizer, RenderResult, SHAdamUpdate, ShAdamQuant, StructureAdamUpdate,
+                         configure_rasterization)
 from .autograd import rasterize
+from . import densify
 
 __all__ = [
     "Camera", "GaussianModel", "ModelGradients", "RasterizationPolicy", "RasterizeOptions", "Rasterizer",
-    "RenderResult", "StructureAdamUpdate", "configure_rasterization", "rasterize",
+    "RenderResult", "SHAdamUpdate", "ShAdamQuant", "StructureAdamUpdate", "configure_rasterization", "densify",
+    "rasterize",
 ]
