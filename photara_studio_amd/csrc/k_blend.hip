@@ -93,6 +93,14 @@ __device__ __forceinline__ float wave_scatter10(const float (&r)[10], int lane) 
     return e + __shfl_xor(e, 32, 64);
 }
 
+// pexp2f_blend (splat_math.h) for an argument in [kSkipPower2, 0]: the same rounding and
+// polynomial, and the scaling by 2^k as one v_ldexp. p * 2^k is exact for k >= -8 (p is near 1),
+// so the result equals pexp2f_blend's product bit for bit.
+__device__ __forceinline__ float exp2_pair(float x) {
+    const float kf = rintf(x);
+    return __builtin_amdgcn_ldexpf(pexp2f_poly(x - kf), int(kf));
+}
+
 // pair_alpha2 (splat_math.h) without early returns, so a wave evaluates it with no exec-mask
 // branches: identical G, alpha and decision for every pair pair_alpha2 accepts (the exponential's
 // argument is the same exponent), including a NaN exponent; other lanes get a placeholder G.
@@ -101,7 +109,7 @@ __device__ __forceinline__ bool pair_alpha_flat(float a2, float b2, float c2, fl
                                                 float clip, float& alpha, float& G) {
     const float p2 = pair_power2(a2, b2, c2, dx, dy);
     const bool ok = !(p2 > 0.0f) && !(p2 < kSkipPower2);
-    G = pexp2f_blend(ok ? p2 : 0.0f);
+    G = exp2_pair(ok ? p2 : 0.0f);
     alpha = min_(clip, op * G);
     return ok && !(alpha < K<float>::alpha_floor());
 }
@@ -141,15 +149,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
     const DevView& dv = *dvp;
     const int lane = threadIdx.x;
     const int W = dv.vi.cam.width, H = dv.vi.cam.height;
-    float pxf[Q], pyf[Q];
-    bool done[Q];
+    // A pixel that has stopped (or lies outside the image) carries its transmittance negated: the
+    // saturation test T (1 - alpha) < floor then holds for it at every later pair, so it needs no
+    // flag of its own, and |T| is its final transmittance.
+    float pxf[Q], pyf[Q], T[Q];
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
         const int px = tc.tx * TS + (k & 1) * 8 + (lane & 7);
         const int py = tc.ty * TS + (k >> 1) * 8 + (lane >> 3);
         pxf[k] = float(px);
         pyf[k] = float(py);
-        done[k] = !(px < W && py < H);
+        T[k] = px < W && py < H ? 1.0f : -1.0f;
     }
     if (a.counters->overflow) return; // incomplete instance list: the host re-runs with more capacity
     const uint2 range = a.ranges[tc.tile];
@@ -160,18 +170,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
     __shared__ uint32_t sM[BATCH];
     __shared__ uint32_t sV[BATCH];
 
-    float T[Q], C0[Q], C1[Q], C2[Q];
+    float C0[Q], C1[Q], C2[Q];
     uint32_t last[Q];
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
-        T[k] = 1.0f;
         C0[k] = C1[k] = C2[k] = 0.0f;
         last[k] = 0;
     }
     uint32_t live = 0; // quadrants with a pixel still blending (wave-uniform)
 #pragma unroll
     for (int k = 0; k < Q; ++k)
-        if (__ballot(!done[k]) != 0) live |= 1u << k;
+        if (__ballot(T[k] > 0.0f) != 0) live |= 1u << k;
     const float clip = a.alpha_clip;
     const float t_floor = K<float>::transmittance_floor();
     uint32_t my_slot[BATCH / 64 > 0 ? BATCH / 64 : 1];
@@ -205,12 +214,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
             bool blended = false;
 #pragma unroll
             for (int k = 0; k < Q; ++k) {
-                if (!((m >> k) & 1u) || done[k]) continue;
+                if (!((m >> k) & 1u)) continue;
                 float alpha, G;
                 if (pair_alpha_flat(A.z, A.w, B.x, B.y, A.x - pxf[k], A.y - pyf[k], clip, alpha, G)) {
                     const float test_t = T[k] * (1.0f - alpha);
                     if (test_t < t_floor) {
-                        done[k] = true;
+                        T[k] = -fabsf(T[k]);
                     } else {
                         const float a_t = alpha * T[k];
                         C0[k] = fmaf(B.z, a_t, C0[k]);
@@ -226,7 +235,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
         }
 #pragma unroll
         for (int k = 0; k < Q; ++k)
-            if (__ballot(!done[k]) == 0) live &= ~(1u << k);
+            if (__ballot(T[k] > 0.0f) == 0) live &= ~(1u << k);
         __syncthreads();
         if (dv.visibility) {
 #pragma unroll
@@ -242,11 +251,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
         const int px = int(pxf[k]), py = int(pyf[k]);
         if (px < W && py < H) {
             const int64_t pid = int64_t(py) * W + px;
-            dv.color[pid] = C0[k] + T[k] * dv.vi.cam.background[0];
-            dv.color[P + pid] = C1[k] + T[k] * dv.vi.cam.background[1];
-            dv.color[2 * P + pid] = C2[k] + T[k] * dv.vi.cam.background[2];
-            dv.alpha[pid] = 1.0f - T[k];
-            a.final_T[dv.vi.pixel_base + pid] = T[k];
+            const float Tk = fabsf(T[k]);
+            dv.color[pid] = C0[k] + Tk * dv.vi.cam.background[0];
+            dv.color[P + pid] = C1[k] + Tk * dv.vi.cam.background[1];
+            dv.color[2 * P + pid] = C2[k] + Tk * dv.vi.cam.background[2];
+            dv.alpha[pid] = 1.0f - Tk;
+            a.final_T[dv.vi.pixel_base + pid] = Tk;
             a.n_contrib[dv.vi.pixel_base + pid] = last[k];
         }
     }

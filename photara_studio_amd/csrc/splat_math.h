@@ -628,16 +628,18 @@ constexpr float kSkipPower2 = -8.0f;
 // 2^x for x in [-8, 0] (or NaN): x = k + r, |r| <= 1/2, a degree-6 polynomial for 2^r (fitted
 // to minimise the relative error on [-1/2, 1/2]; <= 0.94 ulp in float Horner form), exact scaling
 // by 2^k. NaN propagates.
-PHT_HD float pexp2f_blend(float x) {
-    const float kf = rintf(x);
-    const float r = x - kf;
+PHT_HD float pexp2f_poly(float r) {
     float p = 0.00015345800784416497f;
     p = fma_(p, r, 0.0013399932067841291f);
     p = fma_(p, r, 0.009618489071726799f);
     p = fma_(p, r, 0.05550328642129898f);
     p = fma_(p, r, 0.24022646248340607f);
     p = fma_(p, r, 0.6931471824645996f);
-    p = fma_(p, r, 1.0f);
+    return fma_(p, r, 1.0f);
+}
+PHT_HD float pexp2f_blend(float x) {
+    const float kf = rintf(x);
+    const float p = pexp2f_poly(x - kf);
     const int k = x == x ? int(kf) : 0;
     return p * u2f(uint32_t(k + 127) << 23);
 }
