@@ -103,13 +103,15 @@ __device__ __forceinline__ float exp2_pair(float x) {
 
 // pair_alpha2 (splat_math.h) without early returns, so a wave evaluates it with no exec-mask
 // branches: identical G, alpha and decision for every pair pair_alpha2 accepts (the exponential's
-// argument is the same exponent), including a NaN exponent; other lanes get a placeholder G.
-// (a2, b2, c2) is the splat's base-2 conic.
+// argument is the same exponent), including a NaN exponent. With FINITE_G a rejected pair gets
+// G = 1 (so alpha is finite and below 1); without it, a rejected pair's G and alpha are
+// unspecified (possibly inf or NaN) and must not be used.
+template <bool FINITE_G>
 __device__ __forceinline__ bool pair_alpha_flat(float a2, float b2, float c2, float op, float dx, float dy,
                                                 float clip, float& alpha, float& G) {
     const float p2 = pair_power2(a2, b2, c2, dx, dy);
     const bool ok = !(p2 > 0.0f) && !(p2 < kSkipPower2);
-    G = exp2_pair(ok ? p2 : 0.0f);
+    G = exp2_pair(FINITE_G ? (ok ? p2 : 0.0f) : p2);
     alpha = min_(clip, op * G);
     return ok && !(alpha < K<float>::alpha_floor());
 }
@@ -216,7 +218,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
             for (int k = 0; k < Q; ++k) {
                 if (!((m >> k) & 1u)) continue;
                 float alpha, G;
-                if (pair_alpha_flat(A.z, A.w, B.x, B.y, A.x - pxf[k], A.y - pyf[k], clip, alpha, G)) {
+                if (pair_alpha_flat<false>(A.z, A.w, B.x, B.y, A.x - pxf[k], A.y - pyf[k], clip, alpha, G)) {
                     const float test_t = T[k] * (1.0f - alpha);
                     if (test_t < t_floor) {
                         T[k] = -fabsf(T[k]);
@@ -359,7 +361,7 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendArgs a) {
                 const float dx = A.x - pxf[k], dy = A.y - pyf[k];
                 float alpha = 0.0f, G = 0.0f;
                 const bool contrib =
-                    pair_alpha_flat(A.z, A.w, B.x, B.y, dx, dy, clip, alpha, G) && idx < last[k];
+                    pair_alpha_flat<true>(A.z, A.w, B.x, B.y, dx, dy, clip, alpha, G) && idx < last[k];
                 if (__ballot(contrib) == 0) continue;
                 any = true;
                 // Branch-free over the wave: lanes without a contributing pair zero the three
