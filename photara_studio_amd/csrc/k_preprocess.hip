@@ -287,8 +287,12 @@ __global__ __launch_bounds__(256) void k_ranges(const uint32_t* keys, const Coun
         } else {
             for (int e = 0; e < 4; ++e) k[1 + e] = i0 + uint32_t(e) < n ? keys[i0 + uint32_t(e)] : 0xffffffffu;
         }
-        k[0] = i0 > 0u ? keys[i0 - 1u] : 0xffffffffu;
-        k[5] = i0 + 4u < n ? keys[i0 + 4u] : 0xffffffffu;
+        // The neighbours come from the adjacent lanes, which hold the adjacent quads (lane l + 1
+        // holds quad q + 1 whenever key i0 + 4 exists); only the wave's end lanes load them.
+        const int lane = int(threadIdx.x & 63u);
+        const uint32_t up = __shfl_up(k[4], 1, 64), down = __shfl_down(k[1], 1, 64);
+        k[0] = i0 == 0u ? 0xffffffffu : (lane > 0 ? up : keys[i0 - 1u]);
+        k[5] = i0 + 4u < n ? (lane < 63 ? down : keys[i0 + 4u]) : 0xffffffffu;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const uint32_t i = i0 + uint32_t(e);

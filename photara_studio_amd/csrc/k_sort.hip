@@ -140,12 +140,16 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_write(Src src, Sink sink,
 // array in the write-out, and half the per-tile bookkeeping of 8192-key tiles); the depth sort,
 // with a third array in LDS, 1024 x 8.
 
-__global__ __launch_bounds__(kScanThreads) void k_radix_upsweep(const uint32_t* keys, const uint32_t* n_ptr,
-                                                                uint32_t n_const, uint32_t cap, int shift, int nb,
-                                                                uint32_t* hist) {
+// 1024 threads per chunk (16 waves; four LDS histograms, each shared by four waves): the
+// chunks are the downsweep's, one per CU, so the block's width is what keeps enough loads in
+// flight to stream its chunk at HBM rate.
+constexpr int kUpThreads = 1024;
+__global__ __launch_bounds__(kUpThreads) void k_radix_upsweep(const uint32_t* keys, const uint32_t* n_ptr,
+                                                              uint32_t n_const, uint32_t cap, int shift, int nb,
+                                                              uint32_t* hist) {
     __shared__ uint32_t s_h[4][256];
-    const int t = threadIdx.x, w = t >> 6;
-    for (int k = 0; k < 4; ++k) s_h[k][t] = 0;
+    const int t = threadIdx.x, w = t >> 8;
+    s_h[w][t & 255] = 0;
     __syncthreads();
     const uint32_t n = load_count(n_ptr, n_const, cap);
     uint32_t lo, hi;
@@ -154,7 +158,7 @@ __global__ __launch_bounds__(kScanThreads) void k_radix_upsweep(const uint32_t* 
     // uint4 loads per thread are issued before any of them is counted (bytes in flight)
     const uint32_t hi4 = lo + ((hi - lo) & ~3u);
     const uint4* k4 = reinterpret_cast<const uint4*>(keys);
-    constexpr uint32_t kStep = 4u * kScanThreads;
+    constexpr uint32_t kStep = 4u * kUpThreads;
     auto count4 = [&](const uint4& q) {
         atomicAdd(&s_h[w][(q.x >> shift) & 255u], 1u);
         atomicAdd(&s_h[w][(q.y >> shift) & 255u], 1u);
@@ -170,9 +174,9 @@ __global__ __launch_bounds__(kScanThreads) void k_radix_upsweep(const uint32_t* 
         for (int u = 0; u < 4; ++u) count4(q[u]);
     }
     for (; i < hi4; i += kStep) count4(k4[i >> 2]);
-    for (uint32_t i = hi4 + t; i < hi; i += kScanThreads) atomicAdd(&s_h[w][(keys[i] >> shift) & 255u], 1u);
+    for (uint32_t i = hi4 + t; i < hi; i += kUpThreads) atomicAdd(&s_h[w][(keys[i] >> shift) & 255u], 1u);
     __syncthreads();
-    hist[size_t(t) * nb + blockIdx.x] = (s_h[0][t] + s_h[1][t]) + (s_h[2][t] + s_h[3][t]);
+    if (t < 256) hist[size_t(t) * nb + blockIdx.x] = (s_h[0][t] + s_h[1][t]) + (s_h[2][t] + s_h[3][t]);
 }
 
 // Block d (256 blocks of 1024 threads): exclusive scan of digit d's counts over the nb <= 1024
@@ -359,7 +363,7 @@ int launch_radix_sort(uint32_t* keys[2], uint32_t* vals[2], uint32_t* aux[2], co
     const int nb = blocks_for(cap) < 256 ? blocks_for(cap) : 256;
     int cur = 0;
     for (int shift = 0; shift < bits; shift += 8) {
-        hipLaunchKernelGGL(k_radix_upsweep, dim3(nb), dim3(kScanThreads), 0, st, keys[cur], n_ptr, 0u, cap, shift,
+        hipLaunchKernelGGL(k_radix_upsweep, dim3(nb), dim3(kUpThreads), 0, st, keys[cur], n_ptr, 0u, cap, shift,
                            nb, hist);
         hipLaunchKernelGGL(k_radix_scan, dim3(256), dim3(1024), 0, st, hist, nb, digit_total);
         if (aux)
