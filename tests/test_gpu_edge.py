@@ -6,7 +6,9 @@
 - one Gaussian: a list of one splat and a handful of instances;
 - opaque layers: splats at the alpha clip, so about a third of the pixels reach the transmittance
   floor after a few splats and stop (the forward's stop test and the backward's walk from the
-  stored contributor count).
+  stored contributor count);
+- a camera inside the cloud, looking off-axis, on a 200x152 view (partial tiles on both axes):
+  most splats are behind the camera or outside the frustum and are culled.
 """
 from __future__ import annotations
 
@@ -22,9 +24,9 @@ pytestmark = pytest.mark.gpu
 GRADS = ("means", "log_scales", "quaternions", "opacity_logits", "sh", "refine_weight")
 
 
-def _parity(arrays, cuda, oracle, label):
+def _parity(arrays, cuda, oracle, label, cams=None):
     from photara_studio_amd.scenes import workload_cameras
-    cams = workload_cameras("c0")
+    cams = cams or workload_cameras("c0")
     opts = training_options()
     print(f"[edge] {label}: {len(arrays['means'])} gaussians", flush=True)
     r, model, res, lists = _gpu_render(arrays, cams, opts, cuda)
@@ -68,3 +70,12 @@ def test_opaque_layers_saturate(cuda, oracle):
     res, lists = _parity(arrays, cuda, oracle, "opaque layers")
     ft = lists["final_T"]
     assert (ft < 1e-3).mean() > 0.2  # a large share of the pixels stopped at the floor
+
+
+def test_camera_inside_cloud(cuda, oracle):
+    from photara_studio_amd.scenes import look_at
+    arrays = small_scene(4000, 1.0, 14)
+    cams = [look_at((0.1, 0.05, 0.3), target=(0.5, 0.2, -1.0), width=200, height=152)]
+    res, lists = _parity(arrays, cuda, oracle, "camera inside the cloud", cams)
+    visible = int(lists["counts"].visible)
+    assert 0 < visible < len(arrays["means"]) // 2
