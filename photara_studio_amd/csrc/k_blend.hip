@@ -93,12 +93,22 @@ __device__ __forceinline__ float wave_scatter10(const float (&r)[10], int lane) 
     return e + __shfl_xor(e, 32, 64);
 }
 
+// Round-to-integer float to int32 by v_cvt_i32_f32, which saturates out-of-range values and maps
+// NaN to 0. Written as asm so that an out-of-range argument is a defined value (a C++ conversion
+// of it would be undefined behaviour).
+__device__ __forceinline__ int cvt_i32_sat(float x) {
+    int k;
+    asm("v_cvt_i32_f32 %0, %1" : "=v"(k) : "v"(x));
+    return k;
+}
+
 // pexp2f_blend (splat_math.h) for an argument in [kSkipPower2, 0]: the same rounding and
 // polynomial, and the scaling by 2^k as one v_ldexp. p * 2^k is exact for k >= -8 (p is near 1),
-// so the result equals pexp2f_blend's product bit for bit.
+// so the result equals pexp2f_blend's product bit for bit. Any other argument (including inf and
+// NaN) gives some float, with no undefined behaviour.
 __device__ __forceinline__ float exp2_pair(float x) {
     const float kf = rintf(x);
-    return __builtin_amdgcn_ldexpf(pexp2f_poly(x - kf), int(kf));
+    return __builtin_amdgcn_ldexpf(pexp2f_poly(x - kf), cvt_i32_sat(kf));
 }
 
 // pair_alpha2 (splat_math.h) without early returns, so a wave evaluates it with no exec-mask
