@@ -141,8 +141,9 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_write(Src src, Sink sink,
 // with a third array in LDS, 1024 x 8.
 
 // 1024 threads per chunk (16 waves; four LDS histograms, each shared by four waves): the
-// chunks are the downsweep's, one per CU, so the block's width is what keeps enough loads in
-// flight to stream its chunk at HBM rate.
+// chunks are the downsweep's, one per CU, so the block's width sets how many loads a CU has in
+// flight (four times as many as 256-thread blocks; the six passes of the bench workload went
+// from 456 to 409 us, still short of HBM rate, with the LDS atomics the other limit).
 constexpr int kUpThreads = 1024;
 __global__ __launch_bounds__(kUpThreads) void k_radix_upsweep(const uint32_t* keys, const uint32_t* n_ptr,
                                                               uint32_t n_const, uint32_t cap, int shift, int nb,
