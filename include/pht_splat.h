@@ -246,6 +246,12 @@ int pht_sh_adam_quant_step(float* parameter, const float* gradient, uint16_t* fi
 
 /* ---- counters and debug views of the last forward ---- */
 int pht_get_counts(pht_workspace* ws, pht_counts* out, void* stream); /* synchronises */
+/* Enqueues on `stream`, without synchronising, a copy of the last forward's counters into dst[4]
+ * (visible, instances, overflow, fp64_conics as uint32; dst should be pinned host memory): the
+ * step driver reads it after an event to find a replayed step whose instance list overflowed
+ * (that step left the model unchanged) without stalling the replay loop. The reference reads its
+ * counts back synchronously instead (pipeline.cu:206-212). */
+int pht_get_counts_async(pht_workspace* ws, uint32_t* dst, void* stream);
 /* Copies the sorted instance list as 64-bit keys ((global_tile << 32) | depth_bits) and
  * values (gaussian index) plus per-tile [start,end) ranges and per-pixel contributor counts
  * into caller device buffers (any may be NULL). Sizes: keys/values >= instances, ranges

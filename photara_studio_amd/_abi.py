@@ -183,6 +183,7 @@ SIGNATURES = {
                                          C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_float, C.c_float,
                                          C.c_float, C.c_float, C.c_float, C.c_float, C.c_void_p]),
     "pht_get_counts": (C.c_int, [C.c_void_p, C.POINTER(pht_counts), C.c_void_p]),
+    "pht_get_counts_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "pht_debug_lists": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                   C.c_void_p]),
     "pht_debug_splat_grads": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -8,6 +8,7 @@
 // from one caller-owned allocation instead of grow-callbacks; every launch goes to the caller's
 // stream, so the step can be captured as one hipGraph.
 #include <algorithm>
+#include <cstddef>
 #include <cstring>
 #include <random>
 #include <string>
@@ -84,6 +85,10 @@ struct pht_workspace {
     Counters* host_counters = nullptr; // pinned
     // state of the last forward
     bool fwd_valid = false;
+    // a backward has accumulated into sgrad since the forward cleared it: the next backward of
+    // the same forward clears the records again first (pipeline.cu:680-688 zeroes the gradient
+    // state on every backward)
+    bool bwd_done = false;
     int n_views = 0;
     int tile_sel = 0;
     int64_t gaussians = 0;
@@ -318,6 +323,7 @@ int pht_forward(pht_workspace* ws, const pht_gaussians* g, const pht_camera* cam
     if (int r = check_hip(hipGetLastError(), "forward launch")) return r;
 
     ws->fwd_valid = true;
+    ws->bwd_done = false;
     ws->n_views = n_views;
     ws->tile_sel = tsel;
     ws->gaussians = g->count;
@@ -411,6 +417,9 @@ int pht_backward_densify(pht_workspace* ws, const pht_gaussians* g, const pht_vi
     ba.n_gauss = ws->gaussians;
     ba.want_refine = (grads && grads->refine_weight) || densify ? 1 : 0;
     ws->mark(5, 0, st);
+    if (ws->bwd_done)
+        launch_zero_f32(ws->sgrad, int64_t(ws->n_views) * ws->gaussians * kGradStride, st);
+    ws->bwd_done = true;
     launch_blend_backward(ba, int(ws->tiles_total), ws->settings.tile_size, ws->settings.batch, st);
     ws->mark(5, 1, st);
     GaussArgs ga;
@@ -681,6 +690,16 @@ int pht_get_counts(pht_workspace* ws, pht_counts* out, void* stream) {
     out->overflow = int32_t(ws->host_counters->overflow);
     out->reserved = 0;
     return PHT_OK;
+}
+
+int pht_get_counts_async(pht_workspace* ws, uint32_t* dst, void* stream) {
+    if (!ws || !dst) return fail(PHT_EINVAL, "null argument");
+    static_assert(offsetof(Counters, visible) == 0 && offsetof(Counters, instances) == 4 &&
+                      offsetof(Counters, overflow) == 8 && offsetof(Counters, fp64) == 12,
+                  "the first four counters are copied as one block");
+    return check_hip(hipMemcpyAsync(dst, ws->counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                    static_cast<hipStream_t>(stream)),
+                     "counter copy");
 }
 
 int pht_debug_lists(pht_workspace* ws, uint64_t* keys, uint32_t* values, uint32_t* ranges, uint32_t* n_contrib,

@@ -9,6 +9,8 @@
 // (adam_scalar :525-550 for structure, the active-prefix SH Adam of cuda_ops.cu:1506-1546),
 // including rows culled in every view, whose moments still decay. With reduced-precision storage
 // the bf16/fp16 mirrors read by the next forward are rewritten from the FP32 masters here.
+#include <algorithm>
+
 #include "launch.h"
 
 namespace pht {
@@ -622,13 +624,17 @@ void launch_l1_views(const L1Views& v, float* loss_sum, float weight, hipStream_
     else hipLaunchKernelGGL(k_l1_views<false>, grid, dim3(256), 0, st, v, loss_sum, weight);
 }
 
-__global__ void k_zero_f32(float* p, int n) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < n) p[i] = 0.0f;
+__global__ void k_zero_f32(float* p, int64_t n) {
+    const int64_t stride = int64_t(gridDim.x) * 256;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride) p[i] = 0.0f;
 }
 
-void launch_zero_f32(float* p, int n, hipStream_t st) {
-    hipLaunchKernelGGL(k_zero_f32, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, p, n);
+// Zero fill by a kernel (no memset node, so it may be recorded into a graph). Grid-stride over at
+// most 4096 blocks.
+void launch_zero_f32(float* p, int64_t n, hipStream_t st) {
+    if (n <= 0) return;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_zero_f32, dim3(unsigned(blocks)), dim3(256), 0, st, p, n);
 }
 
 } // namespace pht

@@ -133,7 +133,7 @@ struct L1Views {
 void launch_l1_views(const L1Views& v, float* loss_sum, float weight, hipStream_t st);
 void launch_l1_loss(const float* color, const float* target, float* grad, float* loss_sum, int64_t pixels,
                     float weight, hipStream_t st);
-void launch_zero_f32(float* p, int n, hipStream_t st);
+void launch_zero_f32(float* p, int64_t n, hipStream_t st);
 
 // k_densify.hip
 struct DensifyObsSet {

@@ -259,6 +259,26 @@ template <class T> struct Projected {
 // Relative determinant below which the float 2x2 inverse loses more than 8 bits to
 // cancellation; such splats are inverted in FP64 when fp64_conic is set.
 constexpr float kNearSingular = 1.0f / 256.0f;
+// Relative determinant below which the backward of the 2x2 inverse (conic -> covariance) is
+// evaluated in FP64 when fp64_conic is set. Its terms cancel by about a c / det, and the scale
+// gradient along an elongated splat's long axis cancels again when it is contracted with the
+// projection; in float the per-view gradient of such a splat's long log-scale loses up to 1e-3
+// relative (anisotropy ~150 in 2D). The FP64 evaluation takes the same float inputs and rounds
+// its three results to float.
+constexpr float kIllConditioned = 1.0f / 4.0f;
+
+// The conic backward (geometry.cuh:759-809 uncapped branch: dL/d(a, b, c) of the 2D covariance
+// from dL/d(conic)) in double precision, from and to float.
+template <class T>
+PHT_HD void conic_backward_f64(T a, T b, T c, const T dconic[3], T& ga, T& gb, T& gc) {
+    const double A = double(a), B = double(b), Cv = double(c);
+    const double d0 = double(dconic[0]), d1 = double(dconic[1]), d2 = double(dconic[2]);
+    const double denom = A * Cv - B * B;
+    const double inv = 1.0 / (denom * denom + 1.0e-7);
+    ga = T(inv * ((-Cv * Cv * d0 + 2.0 * B * Cv * d1) + (denom - A * Cv) * d2));
+    gc = T(inv * ((-A * A * d2 + 2.0 * A * B * d1) + (denom - A * Cv) * d0));
+    gb = T(inv * 2.0 * ((B * Cv * d0 - (denom + 2.0 * B * B) * d1) + A * B * d2));
+}
 
 template <class T>
 PHT_HD bool project_pinhole(const T mean[3], const Activated<T>& act, const pht_camera& cam,
@@ -729,9 +749,13 @@ PHT_HD void splat_backward_pinhole(const T mean[3], const Activated<T>& act, con
     T dcov[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
     const Cap<T> cp = cap_policy(a, b, c, cam.width, cam.height);
     if (denom2inv != T(0) || cp.q < T(1)) {
-        dL_da = denom2inv * ((-c * c * d_conic[0] + T(2) * b * c * d_conic[1]) + (denom - a * c) * d_conic[2]);
-        dL_dc = denom2inv * ((-a * a * d_conic[2] + T(2) * a * b * d_conic[1]) + (denom - a * c) * d_conic[0]);
-        dL_db = denom2inv * T(2) * ((b * c * d_conic[0] - (denom + T(2) * b * b) * d_conic[1]) + a * b * d_conic[2]);
+        if (s.fp64_conic && sizeof(T) == 4 && !(denom > T(kIllConditioned) * (a * c))) {
+            conic_backward_f64(a, b, c, d_conic, dL_da, dL_db, dL_dc);
+        } else {
+            dL_da = denom2inv * ((-c * c * d_conic[0] + T(2) * b * c * d_conic[1]) + (denom - a * c) * d_conic[2]);
+            dL_dc = denom2inv * ((-a * a * d_conic[2] + T(2) * a * b * d_conic[1]) + (denom - a * c) * d_conic[0]);
+            dL_db = denom2inv * T(2) * ((b * c * d_conic[0] - (denom + T(2) * b * b) * d_conic[1]) + a * b * d_conic[2]);
+        }
         if (cp.q < T(1)) {
             T g3[3];
             conic_backward_capped(a, b, c, cp, d_conic, g3);

@@ -7,6 +7,7 @@ one captured hipGraph (the whole step, no host synchronisation inside it).
 """
 from __future__ import annotations
 
+import collections
 import ctypes as C
 from dataclasses import dataclass
 from typing import List, Optional, Sequence
@@ -42,6 +43,9 @@ def to_low_precision(model: GaussianModel, dtype: torch.dtype) -> GaussianModel:
     model.log_scales_lp = model.log_scales.to(dtype).contiguous()
     model.opacity_logits_lp = model.opacity_logits.to(dtype).contiguous()
     return model
+
+
+_COUNT_RING = 8
 
 
 class TrainStep:
@@ -124,6 +128,13 @@ class TrainStep:
         self.step_index = 0
         self.graph = None
         self.lib = _abi.load()
+        # Counters of launched steps, copied back asynchronously (pinned ring, one row per step):
+        # a step whose instance list overflowed the workspace left the model unchanged and is
+        # re-run after the workspace grows (see _poll).
+        self._counts_host = torch.zeros((_COUNT_RING, 4), dtype=torch.int32, pin_memory=True)
+        self._pending = collections.deque()  # (step_index, ring row, event, "graph" | "eager")
+        self._ring_next = 0
+        self.overflow_recoveries = 0
         self.densify = bool(densify)
         self.dstats = None
         self.view_indices = list(view_indices) if view_indices is not None else list(range(nv))
@@ -160,6 +171,7 @@ class TrainStep:
         from .densify import AdamStates, IgsStrategy, make_densification_stats
         if not self.densify:
             raise RuntimeError("refine() needs a step built with densify=True")
+        self.sync()
         if self.graph is not None:
             self.lib.pht_step_graph_destroy(self.graph)
             self.graph = None
@@ -192,8 +204,55 @@ class TrainStep:
         c2 = float(np.float32(1.0) - np.power(b2, np.float32(step)))
         return c1, c2
 
+    def _track(self, mode: str) -> None:
+        """After a step's launches: enqueue the copy of its counters into the next ring row and
+        record an event behind it."""
+        while len(self._pending) >= _COUNT_RING:
+            self._poll(block=True, limit=1)
+        row = self._ring_next % _COUNT_RING
+        self._ring_next += 1
+        stream = torch.cuda.current_stream()
+        check(self.lib.pht_get_counts_async(self.ws.handle, C.c_void_p(self._counts_host[row].data_ptr()),
+                                            C.c_void_p(stream.cuda_stream)))
+        ev = torch.cuda.Event()
+        ev.record(stream)
+        self._pending.append((self.step_index, row, ev, mode))
+
+    def _poll(self, block: bool = False, limit: Optional[int] = None) -> None:
+        """Reads the counters of finished steps in launch order (waiting for them when `block`).
+        An overflowed step skipped its ranges, blends and Adam update, so the model is as before
+        it; every later step saw the same model and the same instance count and was skipped too.
+        The workspace then grows to the measured count and those steps run again, with their own
+        step indices (Adam's bias corrections) and launch modes."""
+        done = 0
+        while self._pending and (limit is None or done < limit):
+            s, row, ev, mode = self._pending[0]
+            if block:
+                ev.synchronize()
+            elif not ev.query():
+                return
+            self._pending.popleft()
+            done += 1
+            if int(self._counts_host[row, 2]) != 0:
+                modes = [mode] + [p[3] for p in self._pending]
+                self._pending.clear()
+                torch.cuda.synchronize()
+                if self.step_index - s + 1 != len(modes):
+                    raise RuntimeError("step counters out of order")
+                self.step_index = s - 1
+                self.overflow_recoveries += 1
+                self.ensure_capacity()
+                for m in modes:
+                    self.step_graph() if m == "graph" else self.step_eager()
+                return
+
+    def sync(self) -> None:
+        """Waits for every launched step and re-runs any that overflowed the instance capacity."""
+        self._poll(block=True)
+
     def step_eager(self) -> None:
         """One iteration launched kernel by kernel on the current stream (no host sync)."""
+        self._poll()
         self.step_index += 1
         c1, c2 = self._corrections(self.step_index)
         self.adam.correction1, self.adam.correction2 = c1, c2
@@ -217,6 +276,7 @@ class TrainStep:
                                            self.colors[k][0].numel(), self.photometric_weight, stream))
         adam = make_adam(self.model, self.adam, self.sh_adam)
         self._backward(adam, stream)
+        self._track("eager")
 
     def capture(self) -> None:
         """Capture forward + L1 + backward + Adam as one hipGraph (hyper-parameters on device)."""
@@ -236,12 +296,16 @@ class TrainStep:
         self.graph = handle
 
     def step_graph(self) -> None:
+        """One iteration as a replay of the captured step (no host sync; an instance-list overflow
+        is found from the step's counters at a later call and the step re-run)."""
+        self._poll()
         if self.graph is None:
             self.capture()
         self.step_index += 1
         c1, c2 = self._corrections(self.step_index)
         self.hparams.copy_(torch.tensor([c1, c2, self.adam.means_lr], dtype=torch.float32))
         check(self.lib.pht_step_graph_launch(self.graph, C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+        self._track("graph")
 
     def counts(self) -> _abi.pht_counts:
         return self.ws.counts()
@@ -269,6 +333,7 @@ class TrainStep:
                                 self.capacity)
 
     def loss(self) -> float:
+        self.sync()
         return float(self.loss_sum.item())
 
     def __del__(self):

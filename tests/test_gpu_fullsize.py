@@ -4,14 +4,11 @@
   resolutions in one batch): against the oracle at full size. Keys, order, ranges, contributor
   counts, final T, radii, visibility and images bit-exact. Gradients within 1e-4 relative
   (max-abs error over max-abs value) of the float oracle for c1 and c4. At c2 (~97M instances)
-  the float oracle itself sits up to ~7e-5 from the FP64 decision-matched oracle and the GPU's
-  atomic sums differ from it in order (run to run), so the two float results can differ by the
-  sum of their errors. There the FP64 oracle is the arbiter: the GPU's worst element within 2e-4
-  of it and the bulk (99.9th percentile of the error over the max-abs value) within 1e-5; the
-  float oracle's own error and the GPU-float difference are reported. Measured over runs: the
-  worst log-scale element (a Gaussian covering much of a view) is off by 6.5e-5 in the float
-  oracle and by 0.8-1.3e-4 on the GPU, in the same direction; every other gradient is within
-  7e-5 on both.
+  the GPU is also held to 1e-4 against the FP64 decision-matched oracle, with the bulk (99.9th
+  percentile of the error over the max-abs value) within 1e-5, and to 1e-4 against the float
+  oracle. The largest float errors come from elongated splats, whose conic backward cancels by
+  about a c / det: ill-conditioned 2x2 blocks take that step in FP64 (float oracle against FP64
+  at c2: log-scales 6.5e-5 -> 1.1e-5).
 - The benchmark workload (3M Gaussians, eight 1080p views, ~144M instances) and c3 (6M Gaussians,
   four 1080p views, bf16 storage): size-independent invariants of the sorted lists and images,
   and whole training steps (graph and eager) that stay finite.
@@ -68,7 +65,8 @@ def test_full_size_matches_oracle(cuda, oracle, name):
               f"gpu vs float oracle {e_pair:.2e}", flush=True)
         bulk = float(np.quantile(np.abs(got[k].astype(np.float64) - o64[k]), 0.999) / np.abs(o64[k]).max())
         print(f"[fullsize] c2 grad {k}: 99.9th percentile vs fp64 {bulk:.2e}", flush=True)
-        assert e_gpu <= 2 * GRAD_TOL, (k, e_gpu, e_o32)
+        assert e_gpu <= GRAD_TOL, (k, e_gpu, e_o32)
+        assert e_pair <= GRAD_TOL, (k, e_pair)
         assert bulk <= 1e-5, (k, bulk)
 
 

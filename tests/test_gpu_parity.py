@@ -249,3 +249,26 @@ def test_backward_repeatable_no_stale_state(cuda):
         gimg = torch.full_like(res[0].color, 1.0 / res[0].color[0].numel())
         out.append(r.backward_views(model, res, [gimg]).opacity_logits.cpu().numpy())
     assert np.abs(out[0] - out[1]).max() <= 1e-6 * max(1.0, np.abs(out[0]).max())
+
+
+def test_second_backward_of_one_forward(cuda):
+    """Two backwards of ONE forward return the same gradients: the second clears the splat-gradient
+    records the first accumulated into (the reference zeroes its gradient state on every backward,
+    pipeline.cu:680-688)."""
+    from photara_studio_amd.rasterizer import Rasterizer
+    from photara_studio_amd.scenes import model_from, workload_cameras
+    arrays = small_scene(2000, 1.0, 9)
+    cams = workload_cameras("c0")
+    opts = training_options()
+    model = model_from(arrays, cuda)
+    r = Rasterizer()
+    res = r.forward_views(model, cams, opts)
+    gimg = torch.full_like(res[0].color, 1.0 / res[0].color[0].numel())
+    names = ("means", "log_scales", "quaternions", "opacity_logits", "sh", "refine_weight")
+    first = r.backward_views(model, res, [gimg])
+    a = {k: getattr(first, k).cpu().numpy().copy() for k in names}
+    second = r.backward_views(model, res, [gimg])
+    for k in names:
+        b = getattr(second, k).cpu().numpy()
+        assert np.abs(a[k] - b).max() <= 1e-6 * max(1.0, np.abs(a[k]).max()), k
+        assert np.abs(a[k]).max() > 0.0, k

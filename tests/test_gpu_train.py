@@ -144,3 +144,39 @@ def test_ssim_step_gradient_is_the_loss_gradient(cuda, oracle):
         assert torch.equal(grad, g)
         total += float(loss)
     assert abs(total - step.loss()) <= 1e-6 * abs(total)
+
+
+@pytest.mark.parametrize("mode", ["graph", "eager"])
+def test_overflow_mid_training_grows_and_reruns(cuda, oracle, mode):
+    """The instance list outgrows the workspace between steps (all log-scales grow by 0.4 after
+    two steps). The overflowed step leaves the model unchanged; the driver finds it from the step's
+    counters, grows the workspace, re-captures and re-runs it with its own step index, so the run
+    equals one with ample capacity step for step."""
+    _, _, _, _, _, ample = _setup(cuda, oracle)
+    _, _, _, _, _, tight = _setup(cuda, oracle)
+    need = tight.ensure_capacity()
+    from photara_studio_amd.train import TrainStep
+    tight = TrainStep(tight.model, tight.cameras, tight.targets, tight.options, tight.lrs,
+                      instance_capacity=need + 64, sh_optimizer="fp32")
+    cap0 = tight.capacity
+    run = tight.step_graph if mode == "graph" else tight.step_eager
+    for k in range(4):
+        print(f"[train] overflow {mode} step {k + 1}", flush=True)
+        if k == 2:
+            with torch.no_grad():
+                ample.model.log_scales.add_(0.4)
+                tight.model.log_scales.add_(0.4)
+        ample.step_eager()
+        run()
+        assert abs(ample.loss() - tight.loss()) <= 1e-5 * abs(ample.loss()), k
+        assert tight.step_index == ample.step_index == k + 1
+        for key in KEYS:
+            me = getattr(ample.adam, MOMENTS[key] + "_first")
+            mt = getattr(tight.adam, MOMENTS[key] + "_first")
+            rel = float((me - mt).abs().max() / me.abs().max().clamp_min(1e-30))
+            assert rel <= 1e-3, (k, key, rel)
+            strong = me.abs() > 1e-2 * me.abs().max()
+            pe, pt = getattr(ample.model, key), getattr(tight.model, key)
+            assert float((pe - pt).abs()[strong].max()) <= 1e-5, (k, key)
+    assert tight.overflow_recoveries == 1 and tight.capacity > cap0
+    assert not tight.counts().overflow

@@ -178,3 +178,34 @@ def test_batched_gradients_equal_sum_of_single_views(oracle):
     for key in ("means", "log_scales", "quaternions", "opacity_logits", "sh"):
         ref = total[key]
         assert np.abs(gb[key] - ref).max() <= 1e-5 * max(np.abs(ref).max(), 1e-30), key
+
+
+@pytest.mark.parametrize("fp64_conic", [True, False])
+def test_elongated_splat_gradients_against_fp64(oracle, fp64_conic):
+    """Elongated splats (one axis ~20x the others in 3D, so up to ~400x in the 2D covariance's
+    eigenvalues) seen obliquely: the conic backward's terms cancel by about a c / det and the
+    long-axis scale gradient cancels again in the projection. With the FP64 route for
+    ill-conditioned 2x2 blocks the float oracle's log-scale and rotation gradients stay within
+    2e-5 (max-abs error over max-abs value) of the decision-matched FP64 oracle; without it the
+    same scene is several times worse."""
+    from photara_studio_amd.scenes import look_at
+    rng = np.random.default_rng(4)
+    arrays = small_scene(600, 0.8, 21)
+    ls = rng.normal(-4.3, 0.2, size=(600, 3)).astype(np.float32)
+    ls[np.arange(600), rng.integers(0, 3, 600)] = rng.normal(-1.3, 0.2, size=600).astype(np.float32)
+    arrays["log_scales"] = ls
+    cams = [look_at((1.1, -0.7, 3.3), width=160, height=120)]
+    opts = training_options(fp64_conic=fp64_conic)
+    f32 = oracle.OracleRun(oracle.Scene(arrays), cams, opts, 32)
+    gimg = [np.random.default_rng(3).normal(0, 1e-3, size=c.shape).astype(np.float32) for c in f32.color]
+    g32 = f32.backward(gimg)
+    g64 = oracle.OracleRun(oracle.Scene(arrays), cams, opts, 65).backward(gimg)
+
+    def rel(k):
+        return float(np.abs(g32[k].astype(np.float64) - g64[k]).max() / np.abs(g64[k]).max())
+    err = max(rel("log_scales"), rel("quaternions"))
+    print(f"[elongated] fp64_conic={fp64_conic}: {err:.2e}")
+    if fp64_conic:
+        assert err <= 2e-5, err
+    else:
+        assert err > 1e-4, err
