@@ -9,10 +9,10 @@ from .rasterizer import (Camera, GaussianModel, ModelGradients, RasterizationPol
                          Rasterizer, RenderResult, SHAdamUpdate, ShAdamQuant, StructureAdamUpdate,
                          configure_rasterization)
 from .autograd import rasterize
-from . import densify
+from . import checkpoint, densify
 
 __all__ = [
     "Camera", "GaussianModel", "ModelGradients", "RasterizationPolicy", "RasterizeOptions", "Rasterizer",
-    "RenderResult", "SHAdamUpdate", "ShAdamQuant", "StructureAdamUpdate", "configure_rasterization", "densify",
-    "rasterize",
+    "RenderResult", "SHAdamUpdate", "ShAdamQuant", "StructureAdamUpdate", "checkpoint", "configure_rasterization",
+    "densify", "rasterize",
 ]
