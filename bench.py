@@ -67,6 +67,27 @@ def profiled_traffic(kernel, workload):
     return None, None
 
 
+def profiled_counters(kernel, workload):
+    """Measured limiter figures of `kernel` (a stage name) from the `counters:` lines of the
+    rocprofv3 summaries under profiles/ (VALU busy fraction, LDS bank-conflict fraction, atomic
+    rate, ...), or None."""
+    for f in sorted((ROOT / "profiles").glob("*.md")):
+        for line in f.read_text().splitlines():
+            if not line.startswith("counters:"):
+                continue
+            kv = dict(x.split("=", 1) for x in line[len("counters:"):].split())
+            if kv.pop("kernel", None) == kernel and kv.pop("workload", None) == workload:
+                out = {}
+                for k, v in kv.items():
+                    try:
+                        out[k] = float(v)
+                    except ValueError:
+                        out[k] = v
+                out["source"] = f"profiles/{f.name}"
+                return out
+    return None
+
+
 def make_targets(n, cams, opts, device, seed):
     from photara_studio_amd.rasterizer import Rasterizer
     from photara_studio_amd.scenes import model_from, synthetic_torch
@@ -93,55 +114,129 @@ def timed(fn, steps, warmup):
     return (time.perf_counter() - t0) / steps
 
 
-def cpu_baseline(arrays, cams, opts, seconds):
-    """The CPU oracle's training step (forward + L1 + backward + Adam) on the host cores, over a
-    bounded sample: whole steps of one 1080p view of the same scene until ~`seconds` elapse."""
-    import ctypes as C
-    import numpy as np
+def host_cpu():
+    """CPU model name (/proc/cpuinfo) and the number of logical CPUs the process may use."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except AttributeError:
+        nproc = os.cpu_count() or 1
+    return model, nproc
+
+
+class CpuStepper:
+    """The CPU oracle's whole training step (forward + L1 + backward + Adam, float, OpenMP) over
+    one Gaussian set and its views: the reference's math restated on the host cores."""
+
+    def __init__(self, arrays, cams, opts, targets, threads):
+        import ctypes as C
+        import numpy as np
+        from oracle import pyoracle as po
+        from photara_studio_amd import _abi
+        from photara_studio_amd.rasterizer import make_camera
+        self.C, self.np = C, np
+        self.scene = po.Scene(arrays)
+        self.lib = po.load()
+        self.threads = threads
+        self.g = self.scene.gaussians()
+        self.s = po.settings_for(self.scene, opts)
+        nv = len(cams)
+        self.nv = nv
+        self.cams = (_abi.pht_camera * nv)(*[make_camera(c, opts) for c in cams])
+        self.color = [np.zeros((3, c.height, c.width), np.float32) for c in cams]
+        self.alpha = [np.zeros((c.height, c.width), np.float32) for c in cams]
+        self.outs = (_abi.pht_view_outputs * nv)()
+        for k in range(nv):
+            self.outs[k].color, self.outs[k].alpha = self.color[k].ctypes.data, self.alpha[k].ctypes.data
+        self.targets = [np.ascontiguousarray(t, np.float32) for t in targets]
+        self.grad = [np.zeros_like(c) for c in self.color]
+        self.tp = (C.c_void_p * nv)(*[t.ctypes.data for t in self.targets])
+        self.gp = (C.c_void_p * nv)(*[g.ctypes.data for g in self.grad])
+        a = self.scene.a
+        self.m = {k: np.zeros_like(a[k]) for k in ("means", "log_scales", "quaternions", "opacity_logits", "sh")}
+        self.v = {k: np.zeros_like(a[k]) for k in self.m}
+        ad = _abi.pht_adam()
+        for k in self.m:
+            setattr(ad, k, a[k].ctypes.data)
+            setattr(ad, "m_" + k, self.m[k].ctypes.data)
+            setattr(ad, "v_" + k, self.v[k].ctypes.data)
+        ad.lr_means, ad.lr_scales, ad.lr_quaternions, ad.lr_opacity, ad.lr_sh0, ad.lr_sh_rest = \
+            1.6e-4, 5e-3, 1e-3, 5e-2, 2.5e-3, 1.25e-4
+        ad.beta1, ad.beta2, ad.epsilon = 0.9, 0.999, 1e-15
+        ad.min_log_scale, ad.max_log_scale = -float("inf"), float("inf")
+        self.adam = ad
+        self.t = 0
+
+    def step(self) -> float:
+        """One training step; returns its wall-clock seconds (steady clock, no I/O)."""
+        np, C = self.np, self.C
+        self.t += 1
+        self.adam.correction1 = float(np.float32(1) - np.float32(0.9) ** np.float32(self.t))
+        self.adam.correction2 = float(np.float32(1) - np.float32(0.999) ** np.float32(self.t))
+        t0 = time.perf_counter()
+        self.lib.oracle_train_step(C.byref(self.g), self.cams, self.nv, C.byref(self.s), self.outs, self.tp, self.gp,
+                                   C.byref(self.adam), self.threads, 0, 0.2, 1.0)
+        return time.perf_counter() - t0
+
+
+def cpu_render(arrays, cams, opts, threads):
     from oracle import pyoracle as po
-    from photara_studio_amd import _abi
-    from photara_studio_amd.rasterizer import make_camera
-    threads = int(os.environ.get("PHT_CPU_THREADS", "0")) or min(16, os.cpu_count() or 1)
-    scene = po.Scene(arrays)
-    lib = po.load()
+    return [c.copy() for c in po.OracleRun(po.Scene(arrays), cams, opts, 32, threads=threads, key_cap=1).color]
+
+
+def cpu_baseline(arrays, cams, opts, seconds):
+    """The CPU oracle's training step on the host cores.
+    - The bench workload, bounded: whole steps of one 1920x1080 view of the same 3M-Gaussian scene
+      (against the GPU run's target of that view) until ~`seconds` elapse -> view-steps/s.
+    - The small CPU configuration (10,000 Gaussians, one 256x256 view from (0,0,4), target rendered
+      with means perturbed by N(0, 0.01)): median of 10 steps after 2 warm-ups.
+    - 1,000,000 Gaussians in [-3,3]^3, one 1920x1080 view at distance 8, target rendered from a
+      seeded ground-truth set: 3 steps after 1 warm-up.
+    Threads: all CPUs the process may use, at most 16 (a one-GPU machine's CPU share)."""
+    import numpy as np
+    from photara_studio_amd.scenes import synthetic_arrays, workload_cameras
+    model, nproc = host_cpu()
+    threads = int(os.environ.get("PHT_CPU_THREADS", "0")) or min(16, nproc)
     cam = cams[0]
-    g = scene.gaussians()
-    s = po.settings_for(scene, opts)
-    cams_c = (_abi.pht_camera * 1)(make_camera(cam, opts))
-    color = np.zeros((3, cam.height, cam.width), np.float32)
-    alpha = np.zeros((cam.height, cam.width), np.float32)
-    outs = (_abi.pht_view_outputs * 1)()
-    outs[0].color, outs[0].alpha = color.ctypes.data, alpha.ctypes.data
-    target = np.full_like(color, 0.5)
-    grad = np.zeros_like(color)
-    tp = (C.c_void_p * 1)(target.ctypes.data)
-    gp = (C.c_void_p * 1)(grad.ctypes.data)
-    st = {k: np.zeros_like(scene.a[k]) for k in ("means", "log_scales", "quaternions", "opacity_logits", "sh")}
-    st2 = {k: np.zeros_like(scene.a[k]) for k in st}
-    a = _abi.pht_adam()
-    a.means, a.log_scales = scene.a["means"].ctypes.data, scene.a["log_scales"].ctypes.data
-    a.quaternions, a.opacity_logits, a.sh = (scene.a["quaternions"].ctypes.data,
-                                             scene.a["opacity_logits"].ctypes.data, scene.a["sh"].ctypes.data)
-    a.m_means, a.v_means = st["means"].ctypes.data, st2["means"].ctypes.data
-    a.m_log_scales, a.v_log_scales = st["log_scales"].ctypes.data, st2["log_scales"].ctypes.data
-    a.m_quaternions, a.v_quaternions = st["quaternions"].ctypes.data, st2["quaternions"].ctypes.data
-    a.m_opacity_logits, a.v_opacity_logits = st["opacity_logits"].ctypes.data, st2["opacity_logits"].ctypes.data
-    a.m_sh, a.v_sh = st["sh"].ctypes.data, st2["sh"].ctypes.data
-    a.lr_means, a.lr_scales, a.lr_quaternions, a.lr_opacity, a.lr_sh0, a.lr_sh_rest = 1.6e-4, 5e-3, 1e-3, 5e-2, 2.5e-3, 1.25e-4
-    a.beta1, a.beta2, a.epsilon = 0.9, 0.999, 1e-15
-    a.min_log_scale, a.max_log_scale = -float("inf"), float("inf")
-    steps, t0 = 0, time.perf_counter()
+    tgt = arrays.pop("_target0")
+    st = CpuStepper(arrays, [cam], opts, [tgt], threads)
+    times = []
+    t_start = time.perf_counter()
     while True:
-        steps += 1
-        a.correction1 = float(np.float32(1) - np.float32(0.9) ** np.float32(steps))
-        a.correction2 = float(np.float32(1) - np.float32(0.999) ** np.float32(steps))
-        lib.oracle_train_step(C.byref(g), cams_c, 1, C.byref(s), outs, tp, gp, C.byref(a), threads, 0, 0.2, 1.0)
-        el = time.perf_counter() - t0
-        if el >= seconds or steps >= 50:
+        times.append(st.step())
+        if time.perf_counter() - t_start >= seconds or len(times) >= 50:
             break
-    return {"value": steps / el, "unit": "view-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{steps} full CPU-oracle training steps (1 view 1920x1080, {scene.n} Gaussians) "
-                      f"in {el:.1f}s"}
+    del st
+    out = {"value": len(times) / sum(times), "unit": "view-steps/s", "cores": threads, "kind": "port",
+           "sample": f"{len(times)} full CPU-oracle training steps (forward + L1 + backward + Adam) of 1 view "
+                     f"{cam.width}x{cam.height} of the {len(arrays['means'])}-Gaussian bench scene, "
+                     f"{sum(times):.1f}s",
+           "cpu_model": model, "nproc": nproc, "threads": threads}
+    # the two CPU-sized configurations
+    c0 = synthetic_arrays(10_000, 1.0, 0)
+    c0_cams = workload_cameras("c0")
+    pert = dict(c0)
+    pert["means"] = (c0["means"] + np.random.default_rng(1).normal(0, 0.01, c0["means"].shape)).astype(np.float32)
+    st = CpuStepper(c0, c0_cams, opts, cpu_render(pert, c0_cams, opts, threads), threads)
+    for _ in range(2):
+        st.step()
+    t0s = sorted(st.step() for _ in range(10))
+    out["c0_step_ms_median"] = round(1e3 * (t0s[4] + t0s[5]) / 2, 3)
+    c1 = synthetic_arrays(1_000_000, 3.0, 0)
+    c1_cams = workload_cameras("c1")
+    st = CpuStepper(c1, c1_cams, opts, cpu_render(synthetic_arrays(1_000_000, 3.0, 1), c1_cams, opts, threads),
+                    threads)
+    st.step()
+    t1s = [st.step() for _ in range(3)]
+    out["c1_step_ms"] = [round(1e3 * t, 1) for t in t1s]
+    return out
 
 
 def densify_bench(model, cams, targets, opts, refines=3, seed=3, sh_optimizer="fp32"):
@@ -214,8 +309,11 @@ def main():
                    help="ring: --views 1920x1080 views on a radius-8 ring; c2: 2x1080p + 3x720p + 3x540p")
     p.add_argument("--tile", type=int, default=16)
     p.add_argument("--batch", type=int, default=64)
-    p.add_argument("--autotune", action="store_true",
-                   help="sweep tile {16,8} x batch {32,64,128,256} for the batched step and time the fastest")
+    p.add_argument("--waves", type=int, default=1, help="blend waves per workgroup (1, 2, 4) without --autotune")
+    p.add_argument("--queue", action="store_true", help="persistent work-queue blends without --autotune")
+    p.add_argument("--no-autotune", dest="autotune", action="store_false",
+                   help="skip the launch-shape sweep (tile {16,8} x batch {32..256}, then waves {1,2,4} x "
+                        "grid/queue) that picks the batched step's shape")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="first copy 1 GiB device-to-device (a kernel of known traffic for counter scaling)")
     args = p.parse_args()
@@ -242,14 +340,27 @@ def main():
     cams = workload_cameras("c2" if args.workload == "c2" else f"bench{nv}")
     nv = len(cams)
     res_label = ",".join(sorted({f"{c.width}x{c.height}" for c in cams}, reverse=True))
-    opts = RasterizeOptions(active_sh_degree=3, kernel_size=0.3, tile_size=args.tile, batch=args.batch)
+    opts = RasterizeOptions(active_sh_degree=3, kernel_size=0.3, tile_size=args.tile, batch=args.batch,
+                            blend_waves=args.waves, blend_queue=args.queue)
     opts.rasterization.compensate_2d = False
     print(f"[bench] {n} gaussians, {nv} views ({res_label}), tile {args.tile}, batch {args.batch}", flush=True)
     targets = make_targets(n, cams, opts, dev, seed=1)
     arrays = synthetic_torch(n, 3.0, 0, dev)
 
+    # launch shape of the batched step (the plain step below takes the same shape)
+    model = to_low_precision(model_from(arrays, dev), torch.bfloat16)
+    bopts = opts
+    tune_rows = None
+    if args.autotune:
+        from photara_studio_amd.autotune import autotune
+        bopts, tune_rows = autotune(model, cams, targets, opts, LearningRates(), steps=3, warmup=1,
+                                    log=lambda m: print(m, flush=True))
+        print(f"[bench] autotuned: tile {bopts.tile_size}, batch {bopts.batch}, waves {bopts.blend_waves}, "
+              f"{'queue' if bopts.blend_queue else 'grid'}", flush=True)
+        torch.cuda.empty_cache()
+
     # plain FP32 step: one view per step, eager launches
-    plain = TrainStep(model_from(arrays, dev), [cams[0]], [targets[0]], opts, LearningRates(), sh_optimizer="fp32")
+    plain = TrainStep(model_from(arrays, dev), [cams[0]], [targets[0]], bopts, LearningRates(), sh_optimizer="fp32")
     plain.ensure_capacity()
     plain_steps = args.plain_steps or 4 * args.steps
     plain_s = timed(plain.step_eager, plain_steps, args.warmup)
@@ -261,14 +372,6 @@ def main():
     torch.cuda.empty_cache()
 
     # batched BF16 graph step: all views per step
-    model = to_low_precision(model_from(arrays, dev), torch.bfloat16)
-    bopts = opts
-    tune_rows = None
-    if args.autotune:
-        from photara_studio_amd.autotune import autotune
-        bopts, tune_rows = autotune(model, cams, targets, opts, LearningRates(), steps=3, warmup=1,
-                                    log=lambda m: print(m, flush=True))
-        print(f"[bench] autotuned: tile {bopts.tile_size}, batch {bopts.batch}", flush=True)
     batched = TrainStep(model, cams, targets, bopts, LearningRates(), sh_optimizer=args.sh_optimizer)
     batched.ensure_capacity()
     batched.capture()
@@ -297,6 +400,7 @@ def main():
     wname = "c2mixed" if args.workload == "c2" else f"{nv}x1080p"
     workload_key = f"{n // 1_000_000}M-{wname}-bf16" if n % 1_000_000 == 0 else f"{n}-{wname}-bf16"
     traffic, traffic_src = profiled_traffic(dominant, workload_key)
+    limiter = profiled_counters(dominant, workload_key)
     print(f"[bench] stages ms: " + ", ".join(f"{k}={v:.3f}" for k, v in acc.items()), flush=True)
 
     # the same batched step with the other SH optimizer, on a fresh model
@@ -330,8 +434,11 @@ def main():
     if not args.no_cpu_baseline:
         import numpy as np
         host = {k: v.cpu().numpy() for k, v in arrays.items()}
-        print(f"[bench] CPU baseline (~{args.cpu_seconds:.0f}s)", flush=True)
+        host["_target0"] = targets[0].cpu().numpy()
+        print(f"[bench] CPU baseline (~{args.cpu_seconds:.0f}s + the 10k and 1M configurations)", flush=True)
         cpu = cpu_baseline(host, cams, opts, args.cpu_seconds)
+        print(f"[bench] CPU: {cpu['value']:.3f} view-steps/s, 10k median {cpu['c0_step_ms_median']} ms, "
+              f"1M {cpu['c1_step_ms']} ms ({cpu['threads']} threads, {cpu['cpu_model']})", flush=True)
 
     line = {
         "metric": METRIC,
@@ -348,11 +455,14 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"3dgs-train-step {n} gaussians x {nv} views {res_label}, bf16 storage, hipGraph",
                    "gaussians": n, "views": nv, "resolution": res_label, "tile": int(bopts.tile_size),
-                   "batch": int(bopts.batch), "storage": "bf16 sh/log_scales/opacity, fp32 masters",
+                   "batch": int(bopts.batch), "blend_waves": int(bopts.blend_waves),
+                   "blend_launch": "persistent work queue" if bopts.blend_queue else "flat grid",
+                   "storage": "bf16 sh/log_scales/opacity, fp32 masters",
                    "sh_optimizer": ("FP32 moments (as the plain step)" if args.sh_optimizer == "fp32" else
                                     "quantised (fp16 update + 8-bit log-second)") + ", fp32 structure moments",
                    "parallelism": "none (1 GPU)"},
-        "autotune": [[r.tile_size, r.batch, round(r.ms_per_step, 3)] for r in tune_rows] if tune_rows else None,
+        "autotune": [[r.tile_size, r.batch, r.blend_waves, "queue" if r.blend_queue else "grid",
+                      round(r.ms_per_step, 3)] for r in tune_rows] if tune_rows else None,
         "plain_fp32": {"ms_per_step": round(plain_s * 1e3, 4), "steps_per_s": round(1.0 / plain_s, 3),
                        "instances": int(pc.instances)},
         "speedup_vs_plain_fp32": round((nv / step_s) * plain_s, 3),
@@ -365,7 +475,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(dom_gbps, 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(dom_gbps / HBM_PEAK_GBPS, 4), "traffic": traffic,
                      "algorithmic_bytes": int(ab[dominant]),
-                     "traffic_source": f"profiles/{traffic_src}" if traffic_src else None},
+                     "traffic_source": f"profiles/{traffic_src}" if traffic_src else None,
+                     "limiter": limiter},
         "cpu_baseline": cpu,
         "densify": dens,
     }

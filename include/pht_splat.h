@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define PHT_ABI_VERSION 3
+#define PHT_ABI_VERSION 4
 #define PHT_MAX_VIEWS 32
 
 /* Status codes. The reference throws std::invalid_argument for bad inputs
@@ -73,9 +73,14 @@ typedef struct pht_settings {
     int32_t compensate_2d;    /* opacity *= sqrt(det0/det1) */
     int32_t tile_size;        /* 16 (reference) or 8 */
     int32_t batch;            /* LDS batch depth of the blend kernels: 32, 64, 128 or 256 */
-    int32_t fp64_conic;       /* route near-singular 2x2 inverses through FP64 */
+    int32_t fp64_conic;       /* route near-singular 2x2 inverses (and the backward of
+                                 ill-conditioned ones) through FP64 */
     int32_t require_depth;    /* must be 0: median-depth/normal channels are not built */
-    int32_t reserved;
+    int16_t blend_waves;      /* blend launch shape: waves (one (view, tile) item each) per
+                                 workgroup, 1, 2 or 4 (0 = 1) */
+    int16_t blend_queue;      /* 1: blends run as one persistent work-queue launch (resident
+                                 workgroups pull (view, tile) items from a device counter);
+                                 0: one flat grid of work items */
 } pht_settings;
 
 /* Gaussian parameters (raw, pre-activation), reference GaussianModel (types.hpp:36-56)
@@ -224,6 +229,19 @@ size_t pht_l1_ssim_scratch_bytes(int width, int height);
 int pht_l1_ssim_loss(const float* color, const float* target, const float* mask, float* grad_color,
                      int accumulate, float* loss_sum, void* scratch, int width, int height,
                      float ssim_weight, float photometric_weight, void* stream);
+
+/* ---- pht_l1_ssim_loss without caller scratch and without any allocation. No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). The per-tile loss partials live in a static
+ * device array of PHT_NOALLOC_SSIM_TILES floats that the library holds on each GPU it runs on
+ * until the process ends; a view needs ceil(W/16) * ceil(H/16) <= PHT_NOALLOC_SSIM_TILES
+ * (PHT_EINVAL otherwise). Same arguments and results as pht_l1_ssim_loss with loss_sum; loss_sum
+ * may be NULL (then the array is not used). Calls that overlap on one GPU (concurrent streams)
+ * race on that array. ---- */
+#define PHT_NOALLOC_SSIM_TILES 65536
+int pht_l1_ssim_loss_noalloc(const float* color, const float* target, const float* mask, float* grad_color,
+                             int accumulate, float* loss_sum, int width, int height, float ssim_weight,
+                             float photometric_weight, void* stream);
 
 /* ---- standalone Adam (adam_kernel / adam_active_prefix_kernel, cuda_ops.cu:1336-1358,1506-1546):
  * group_stride/active/secondary lr follow the reference signature. ---- */

@@ -280,8 +280,12 @@ void run_forward(const pht_gaussians& g, const pht_camera* cams, int n_views, co
 // composited sums, the gradients) is computed in double. The image is piecewise smooth in the
 // inputs, with jumps where a pair crosses the 1/255 cut-off; comparing on the same piece isolates
 // the arithmetic error from such threshold flips.
+// With `double_decisions` (precision 66, "order-matched FP64") only the float run's tile lists and
+// blend order are kept; which pairs pass the 1/255 cut-off and where a pixel terminates are decided
+// in double. Against the plain FP64 oracle this isolates the order of splats whose depths are
+// within float rounding of each other (the reference sorts by float32 depth bits).
 void run_forward_matched(const pht_gaussians& g, const pht_settings& s, const State<float>& f,
-                         State<double>& st, int threads) {
+                         State<double>& st, int threads, bool double_decisions = false) {
     st.views = f.views;
     st.splat_of = f.splat_of;
     st.keys = f.keys;
@@ -332,6 +336,18 @@ void run_forward_matched(const pht_gaussians& g, const pht_settings& s, const St
                 float T32 = 1.0f;
                 double Tr = 1.0, C[3] = {0.0, 0.0, 0.0};
                 for (uint32_t k = b; k < e; ++k) {
+                    if (double_decisions) {
+                        const SplatRec<double>& r = st.splats[st.vals[k]];
+                        double alpha, G;
+                        if (!pair_alpha(r.conic[0], r.conic[1], r.conic[2], r.opacity, r.mean2d[0] - double(px),
+                                        r.mean2d[1] - double(py), clip, alpha, G))
+                            continue;
+                        const double test = Tr * (1.0 - alpha);
+                        if (test < K<double>::transmittance_floor()) break;
+                        for (int ch = 0; ch < 3; ++ch) C[ch] = fma(r.rgb[ch], alpha * Tr, C[ch]);
+                        Tr = test;
+                        continue;
+                    }
                     const SplatRec<float>& q = f.splats[f.vals[k]];
                     float a32, G32;
                     if (!pair_alpha(q.conic[0], q.conic[1], q.conic[2], q.opacity, q.mean2d[0] - float(px),
@@ -820,16 +836,17 @@ int oracle_enumerate(float mx, float my, const float* conic3, float op, int grid
 
 // Forward. precision 32: parity oracle (float outputs); 64: accuracy oracle (double results
 // rounded to float in `outs`); 65: accuracy oracle on the float run's decisions (see
-// run_forward_matched; `outs` is not written). Keys/values/ranges/n_contrib/final_T of the float run are copied
+// run_forward_matched; `outs` is not written); 66: on the float run's tile lists and order only
+// (images only: no backward). Keys/values/ranges/n_contrib/final_T of the float run are copied
 // when the pointers are non-null.
 int oracle_forward(int precision, const pht_gaussians* g, const pht_camera* cams, int n_views,
                    const pht_settings* s, pht_view_outputs* outs, uint64_t* keys, uint32_t* values,
                    int64_t key_cap, uint32_t* ranges, uint32_t* n_contrib, float* final_T,
                    pht_counts* counts, int threads) {
     g_alpha_clip = s->alpha_clip;
-    if (precision == 65) {
+    if (precision == 65 || precision == 66) {
         run_forward<float>(*g, cams, n_views, *s, g_state32, nullptr, threads);
-        run_forward_matched(*g, *s, g_state32, g_state64, threads);
+        run_forward_matched(*g, *s, g_state32, g_state64, threads, precision == 66);
         if (counts) {
             counts->visible = int64_t(g_state64.splats.size());
             counts->instances = int64_t(g_state64.keys.size());
@@ -929,7 +946,7 @@ int oracle_bruteforce(float* color_concat, float* alpha_concat, float* final_T) 
 // Images of the last forward at its own precision (double for precision 64): colour per view
 // [3,H,W] and alpha per view [H,W], each concatenated over views.
 int oracle_images(int precision, double* color, double* alpha) {
-    if (precision == 64 || precision == 65) {
+    if (precision == 64 || precision == 65 || precision == 66) {
         std::copy(g_state64.color.begin(), g_state64.color.end(), color);
         std::copy(g_state64.alpha.begin(), g_state64.alpha.end(), alpha);
     } else {
@@ -945,6 +962,7 @@ int oracle_backward(int precision, const pht_gaussians* g, const pht_settings* s
                     void* means, void* log_scales, void* quats, void* logits, void* sh, void* colors,
                     void* refine, int threads) {
     const int64_t N = g->count;
+    if (precision == 66) return -1; // order-matched FP64 renders images only
     if (precision == 64 || precision == 65) {
         std::vector<SplatGradAcc<double>>& acc = g_acc64;
         run_blend_backward<double>(*g, *s, g_state64, dl, acc, threads, precision == 65 ? &g_state32 : nullptr);

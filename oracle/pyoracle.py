@@ -144,7 +144,9 @@ def settings_for(scene: Scene, options: RasterizeOptions) -> _abi.pht_settings:
 class OracleRun:
     """One oracle forward (and optional backward) over a list of cameras. precision: 32 = float
     parity oracle, 64 = double accuracy oracle, 65 = double values on the float run's discrete
-    decisions (tile lists, 1/255 skips, early termination, SH clamps)."""
+    decisions (tile lists, 1/255 skips, early termination, SH clamps), 66 = double values and
+    double skip / termination decisions on the float run's tile lists and blend order (images
+    only)."""
 
     def __init__(self, scene: Scene, cameras: Sequence[Camera], options: RasterizeOptions, precision: int = 32,
                  threads: Optional[int] = None, key_cap: int = 1 << 26):
@@ -197,6 +199,8 @@ class OracleRun:
             off += P
 
     def backward(self, grad_colors: Sequence[np.ndarray], grad_alphas: Optional[Sequence] = None) -> Dict[str, np.ndarray]:
+        if self.precision == 66:
+            raise ValueError("the order-matched FP64 oracle (precision 66) renders images only")
         lib = load()
         nv = len(self.cameras)
         gcs = [f32(g) for g in grad_colors]

@@ -36,7 +36,8 @@ class pht_settings(C.Structure):
         ("kernel_size", C.c_float), ("scale_modifier", C.c_float),
         ("alpha_clip", C.c_float), ("near_plane", C.c_float),
         ("compensate_2d", C.c_int32), ("tile_size", C.c_int32), ("batch", C.c_int32),
-        ("fp64_conic", C.c_int32), ("require_depth", C.c_int32), ("reserved", C.c_int32),
+        ("fp64_conic", C.c_int32), ("require_depth", C.c_int32), ("blend_waves", C.c_int16),
+        ("blend_queue", C.c_int16),
     ]
 
 
@@ -176,6 +177,8 @@ SIGNATURES = {
     "pht_l1_ssim_scratch_bytes": (C.c_size_t, [C.c_int, C.c_int]),
     "pht_l1_ssim_loss": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                                    C.c_int, C.c_int, C.c_float, C.c_float, C.c_void_p]),
+    "pht_l1_ssim_loss_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                           C.c_int, C.c_int, C.c_float, C.c_float, C.c_void_p]),
     "pht_adam_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_int64,
                                 C.c_float, C.c_float, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_float,
                                 C.c_float, C.c_float, C.c_float, C.c_int32, C.c_void_p]),

@@ -11,7 +11,7 @@ from typing import Optional
 
 import torch
 
-from .rasterizer import Camera, GaussianModel, RasterizeOptions, Rasterizer
+from .rasterizer import Camera, GaussianModel, RasterizeOptions, Rasterizer, RenderResult
 
 
 class _Rasterize(torch.autograd.Function):
@@ -22,14 +22,21 @@ class _Rasterize(torch.autograd.Function):
                               opacity_logits=opacity_logits.detach().contiguous(), sh=sh.detach().contiguous(),
                               sh_degree=sh_degree)
         rendered = rasterizer.forward(model, camera, options)
-        ctx.model, ctx.rendered, ctx.rasterizer = model, rendered, rasterizer
+        # The backward needs the render's context and image shapes, not its images: keeping the
+        # returned colour tensor here would close a reference cycle (output -> graph node -> ctx ->
+        # output), and the forward's workspace would stay reserved until Python's cycle collector
+        # ran, so a training loop would reserve a new workspace every step.
+        meta = RenderResult(color=torch.empty(rendered.color.shape, device="meta"),
+                            alpha=torch.empty(rendered.alpha.shape, device="meta"), context=rendered.context,
+                            rendered_instances=rendered.rendered_instances)
+        ctx.model, ctx.rendered, ctx.rasterizer = model, meta, rasterizer
         ctx.opacity_shape = opacity_logits.shape
         return rendered.color, rendered.alpha
 
     @staticmethod
     def backward(ctx, grad_color, grad_alpha):
         if grad_color is None:
-            grad_color = torch.zeros_like(ctx.rendered.color)
+            grad_color = torch.zeros(ctx.rendered.color.shape, device=ctx.model.means.device)
         g = ctx.rasterizer.backward(ctx.model, ctx.rendered, grad_color.contiguous(),
                                     None if grad_alpha is None else grad_alpha.contiguous())
         return (g.means, g.log_scales, g.quaternions, g.opacity_logits.view(ctx.opacity_shape), g.sh,

@@ -236,6 +236,8 @@ int pht_forward(pht_workspace* ws, const pht_gaussians* g, const pht_camera* cam
     if (s->batch != 32 && s->batch != 64 && s->batch != 128 && s->batch != 256)
         return fail(PHT_EINVAL, "batch must be 32, 64, 128 or 256");
     if (s->tile_size == 8 && s->batch > 128) return fail(PHT_EINVAL, "8x8 tiles support batch <= 128");
+    if (s->blend_waves != 0 && s->blend_waves != 1 && s->blend_waves != 2 && s->blend_waves != 4)
+        return fail(PHT_EINVAL, "blend_waves must be 1, 2 or 4");
     hipStream_t st = static_cast<hipStream_t>(stream);
     // per-view tables
     std::vector<DevView> hv(static_cast<size_t>(n_views));
@@ -291,14 +293,16 @@ int pht_forward(pht_workspace* ws, const pht_gaussians* g, const pht_camera* cam
     ws->mark(2, 0, st);
     launch_tile_offsets(ws->dtiles[dsel], &ws->counters->visible, uint32_t(n_slots), ws->offsets, ws->scan_scratch,
                         &ws->counters->instances, &ws->counters->overflow, ws->inst_cap, st);
+    // 16-bit tile keys whenever every global tile id fits (the bench's 8 x 8160 tiles do)
+    const bool narrow = tiles <= 65536;
     launch_emit_instances(ws->dvals[dsel], ws->dtiles[dsel], ws->offsets, ws->counters, uint32_t(n_slots), ws->rec,
-                          ws->views, ws->vp_log2, ts, ws->inst_cap, ws->tkeys[0], ws->tvals[0], st);
+                          ws->views, ws->vp_log2, ts, ws->inst_cap, ws->tkeys[0], ws->tvals[0], narrow, st);
     ws->mark(2, 1, st);
     ws->mark(3, 0, st);
     const int tbits = bits_for(uint64_t(tiles - 1));
     const int tsel = launch_radix_sort(ws->tkeys, ws->tvals, nullptr, &ws->counters->instances, ws->inst_cap, tbits,
-                                       ws->hist, ws->digit_total, st);
-    launch_ranges(ws->tkeys[tsel], ws->counters, ws->inst_cap, uint32_t(tiles), ws->ranges, st);
+                                       ws->hist, ws->digit_total, st, narrow);
+    launch_ranges(ws->tkeys[tsel], ws->counters, ws->inst_cap, uint32_t(tiles), ws->ranges, narrow, st);
     launch_tile_order(ws->ranges, uint32_t(tiles), ws->tile_order, ws->order_hist, st);
     ws->mark(3, 1, st);
     BlendArgs ba;
@@ -317,8 +321,9 @@ int pht_forward(pht_workspace* ws, const pht_gaussians* g, const pht_camera* cam
     ba.slot_cap = ws->slot_cap;
     ba.n_gauss = g->count;
     ba.want_refine = 0;
+    ba.queue = s->blend_queue ? ws->counters->queue : nullptr;
     ws->mark(4, 0, st);
-    launch_blend_forward(ba, int(tiles), ts, s->batch, st);
+    launch_blend_forward(ba, int(tiles), ts, s->batch, s->blend_waves, st);
     ws->mark(4, 1, st);
     if (int r = check_hip(hipGetLastError(), "forward launch")) return r;
 
@@ -416,11 +421,13 @@ int pht_backward_densify(pht_workspace* ws, const pht_gaussians* g, const pht_vi
     ba.slot_cap = ws->slot_cap;
     ba.n_gauss = ws->gaussians;
     ba.want_refine = (grads && grads->refine_weight) || densify ? 1 : 0;
+    ba.queue = ws->settings.blend_queue ? ws->counters->queue : nullptr;
     ws->mark(5, 0, st);
     if (ws->bwd_done)
         launch_zero_f32(ws->sgrad, int64_t(ws->n_views) * ws->gaussians * kGradStride, st);
     ws->bwd_done = true;
-    launch_blend_backward(ba, int(ws->tiles_total), ws->settings.tile_size, ws->settings.batch, st);
+    launch_blend_backward(ba, int(ws->tiles_total), ws->settings.tile_size, ws->settings.batch,
+                          ws->settings.blend_waves, st);
     ws->mark(5, 1, st);
     GaussArgs ga;
     ga.g = *g;
@@ -486,6 +493,25 @@ int pht_l1_ssim_loss(const float* color, const float* target, const float* mask,
     if (loss_sum && !scratch) return fail(PHT_EINVAL, "loss_sum needs scratch (pht_l1_ssim_scratch_bytes)");
     launch_l1_ssim(color, target, mask, grad_color, accumulate, loss_sum, static_cast<float*>(scratch), width, height,
                    ssim_weight, photometric_weight, static_cast<hipStream_t>(stream));
+    return check_hip(hipGetLastError(), "l1+ssim launch");
+}
+
+int pht_l1_ssim_loss_noalloc(const float* color, const float* target, const float* mask, float* grad_color,
+                             int accumulate, float* loss_sum, int width, int height, float ssim_weight,
+                             float photometric_weight, void* stream) {
+    if (!color || !target || !grad_color) return fail(PHT_EINVAL, "color, target and grad_color are required");
+    if (width <= 10 || height <= 10) return fail(PHT_EINVAL, "fused SSIM requires width and height > 10");
+    if (l1_ssim_scratch_floats(width, height) > size_t(PHT_NOALLOC_SSIM_TILES))
+        return fail(PHT_EINVAL, "view too large for the no-allocation L1+SSIM loss: ceil(W/16) * ceil(H/16) must "
+                                "be at most " + std::to_string(PHT_NOALLOC_SSIM_TILES));
+    float* partials = nullptr;
+    if (loss_sum) {
+        hipError_t err = hipSuccess;
+        partials = l1_ssim_static_scratch(&err);
+        if (!partials) return check_hip(err == hipSuccess ? hipErrorInvalidSymbol : err, "static scratch address");
+    }
+    launch_l1_ssim(color, target, mask, grad_color, accumulate, loss_sum, partials, width, height, ssim_weight,
+                   photometric_weight, static_cast<hipStream_t>(stream));
     return check_hip(hipGetLastError(), "l1+ssim launch");
 }
 
@@ -709,7 +735,7 @@ int pht_debug_lists(pht_workspace* ws, uint64_t* keys, uint32_t* values, uint32_
     hipStream_t st = static_cast<hipStream_t>(stream);
     if (keys || values)
         launch_debug_keys(ws->tkeys[ws->tile_sel], ws->tvals[ws->tile_sel], ws->meta, ws->counters, ws->inst_cap,
-                          ws->vp_log2, keys, values, st);
+                          ws->vp_log2, keys, values, ws->tiles_total <= 65536, st);
     if (ranges)
         if (int r = check_hip(hipMemcpyAsync(ranges, ws->ranges, size_t(ws->tiles_total) * sizeof(uint2),
                                              hipMemcpyDeviceToDevice, st), "ranges copy"))

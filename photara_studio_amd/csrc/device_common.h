@@ -26,20 +26,22 @@ struct Counters {
     uint32_t overflow;    // instances > capacity
     uint32_t fp64;        // FP64-routed conics
     uint32_t scan_total;  // scratch total of the last chunked scan
-    uint32_t pad[3];
+    uint32_t queue[2];    // persistent blend work-queue heads (forward, backward)
+    uint32_t pad;
 };
 
 // Blend record, one per (gaussian, view) slot (slot = g * vp + v), 64 bytes so that every
 // gather of it by the tile kernels is one aligned 64-byte HBM sector:
-//   a = (mean.x, mean.y, a2, b2)          b = (c2, opacity, r, g)
-//   c = (b, ext.x, ext.y, qmax)           d = (conic.x, conic.y, conic.z, opacity)
+//   a = (mean.x, mean.y, a2, b2)          d = (conic.x, conic.y, conic.z, opacity)
+//   b = (c2, opacity, r, g)               c = (b, ext.x, ext.y, qmax)
 // (a2, b2, c2) = the conic scaled for base-2 exponents (conic_log2), read by the per-pair test;
 // ext = half extents of the box around the alpha >= 1/255 ellipse (grown by 1% + 1 px) and
 // qmax = the ellipse's grown bound on the quadratic form, used to cull 8x8 pixel patches
-// (+inf = no culling, -inf = reaches no pixel); the conic itself feeds the tile walk (which reads
-// a and d only) and the gradient terms.
+// (+inf = no culling, -inf = reaches no pixel); the conic itself feeds the tile walk and the
+// gradient terms. The members are laid out a, d, b, c: the emission's random gathers (a and d
+// only, in depth order) then touch the record's first 32 bytes alone.
 struct SplatRec {
-    float4 a, b, c, d;
+    float4 a, d, b, c;
 };
 // Slot metadata (16 bytes, read in slot order): (0, depth_key, tiles, flags),
 // flags: bits 0-2 SH clamp, bit 3 fp64 conic.

@@ -130,9 +130,11 @@ struct TileCoord {
     int tile, v, tx, ty;
 };
 
-__device__ __forceinline__ TileCoord tile_of_block(const BlendArgs& a, const DevView*& dv) {
+// Work item -> (view, tile): the heaviest-first order when one is given, else the flat grid's
+// XCD-contiguous map (consecutive items of one XCD take neighbouring tiles).
+__device__ __forceinline__ TileCoord tile_of_item(const BlendArgs& a, int item, int n_items, const DevView*& dv) {
     TileCoord c;
-    c.tile = a.tile_order ? int(a.tile_order[blockIdx.x]) : xcd_contiguous(int(blockIdx.x), int(gridDim.x));
+    c.tile = a.tile_order ? int(a.tile_order[item]) : (a.queue ? item : xcd_contiguous(item, n_items));
     dv = &view_of_tile(a.views, a.n_views, c.tile, c.v);
     const int local = c.tile - dv->vi.tile_base;
     c.tx = local % dv->vi.grid_x;
@@ -140,26 +142,46 @@ __device__ __forceinline__ TileCoord tile_of_block(const BlendArgs& a, const Dev
     return c;
 }
 
-template <int TS, int BATCH>
-// 8 waves per SIMD: the kernel fits 64 VGPRs without spilling (69 unconstrained, 7 waves), and the
-// extra wave hides more of the per-instance LDS reads (measured: 1.92 -> 1.83 ms on the batched
-// step). The backward does not fit its bound without spilling and keeps the default.
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_blend_fwd(BlendArgs a) {
-    constexpr int Q = (TS / 8) * (TS / 8);
-    {
-        // Clear the splat-gradient records the backward accumulates into (n_views * N records of
-        // 12 floats, view-major): each block streams zeros over its share. The stores retire in
-        // the background of this VALU-bound kernel instead of taking HBM time of their own.
-        const int64_t n4 = int64_t(a.n_views) * a.n_gauss * (kGradStride / 4);
-        const int64_t nb = gridDim.x, b = blockIdx.x;
-        const int64_t lo = n4 * b / nb, hi = n4 * (b + 1) / nb;
-        float4* g4 = reinterpret_cast<float4*>(a.sgrad);
-        for (int64_t j = lo + threadIdx.x; j < hi; j += 64) g4[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+// LDS hand-over between the lanes of one wave. Each wave stages into its own LDS region and no
+// other wave reads it, so no workgroup barrier is needed (the waves of a workgroup run different
+// tiles, with different trip counts): a wave's LDS operations complete in issue order, and the
+// fences keep the compiler from moving LDS accesses across this point.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The work loop of a blend launch. Flat grid: wave w of block b takes item b * WPB + w.
+// Persistent queue (a.queue set): the grid holds only as many workgroups as are resident at once,
+// and each wave takes the next item from a device counter until the items run out (every wave
+// reaches the exit: the counter only grows, and an item >= n_items ends the loop).
+template <int WPB, class F>
+__device__ __forceinline__ void for_each_item(const BlendArgs& a, uint32_t* queue, int n_items, F&& body) {
+    const int wave = int(__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)), lane = int(threadIdx.x) & 63;
+    if (!queue) {
+        const int item = int(blockIdx.x) * WPB + wave;
+        if (item < n_items) body(item);
+        return;
     }
+    for (;;) {
+        uint32_t got = 0;
+        if (lane == 0) got = atomicAdd(queue, 1u);
+        const int item = int(__builtin_amdgcn_readfirstlane(got));
+        if (item >= n_items) return;
+        body(item);
+    }
+}
+
+// One (view, tile) work item of the forward: front-to-back blend of the tile's sorted list.
+template <int TS, int BATCH>
+__device__ __forceinline__ void blend_fwd_item(const BlendArgs& a, int item, int n_items, float4* sA, float4* sB,
+                                               float* sC, uint32_t* sM, uint32_t* sV) {
+    constexpr int Q = (TS / 8) * (TS / 8);
     const DevView* dvp;
-    const TileCoord tc = tile_of_block(a, dvp);
+    const TileCoord tc = tile_of_item(a, item, n_items, dvp);
     const DevView& dv = *dvp;
-    const int lane = threadIdx.x;
+    const int lane = int(threadIdx.x) & 63;
     const int W = dv.vi.cam.width, H = dv.vi.cam.height;
     // A pixel that has stopped (or lies outside the image) carries its transmittance negated: the
     // saturation test T (1 - alpha) < floor then holds for it at every later pair, so it needs no
@@ -173,14 +195,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
         pyf[k] = float(py);
         T[k] = px < W && py < H ? 1.0f : -1.0f;
     }
-    if (a.counters->overflow) return; // incomplete instance list: the host re-runs with more capacity
     const uint2 range = a.ranges[tc.tile];
-
-    __shared__ float4 sA[BATCH];
-    __shared__ float4 sB[BATCH];
-    __shared__ float sC[BATCH];
-    __shared__ uint32_t sM[BATCH];
-    __shared__ uint32_t sV[BATCH];
 
     float C0[Q], C1[Q], C2[Q];
     uint32_t last[Q];
@@ -193,7 +208,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
 #pragma unroll
     for (int k = 0; k < Q; ++k)
         if (__ballot(T[k] > 0.0f) != 0) live |= 1u << k;
-    const float clip = a.alpha_clip;
+    // canonical once, so the per-pair min(clip, .) needs no canonicalisation of its own
+    const float clip = __builtin_canonicalizef(a.alpha_clip);
     const float t_floor = K<float>::transmittance_floor();
     uint32_t my_slot[BATCH / 64 > 0 ? BATCH / 64 : 1];
     for (uint32_t base = range.x; base < range.y && live != 0; base += BATCH) {
@@ -217,7 +233,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
                 sV[j] = 0u;
             }
         }
-        __syncthreads();
+        wave_lds_sync();
         for (uint32_t j = 0; j < cnt; ++j) {
             const uint32_t m = __builtin_amdgcn_readfirstlane(sM[j]) & live;
             if (m == 0) continue;
@@ -248,7 +264,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
 #pragma unroll
         for (int k = 0; k < Q; ++k)
             if (__ballot(T[k] > 0.0f) == 0) live &= ~(1u << k);
-        __syncthreads();
+        wave_lds_sync();
         if (dv.visibility) {
 #pragma unroll
             for (int q = 0; q < (BATCH + 63) / 64; ++q) {
@@ -274,24 +290,47 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
     }
 }
 
+template <int TS, int BATCH, int WPB>
+// 8 waves per SIMD: the kernel fits 64 VGPRs without spilling (69 unconstrained, 7 waves), and the
+// extra wave hides more of the per-instance LDS reads (measured: 1.92 -> 1.83 ms on the batched
+// step). The backward does not fit its bound without spilling and keeps the default.
+__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_blend_fwd(BlendArgs a,
+                                                                                                   int n_items) {
+    {
+        // Clear the splat-gradient records the backward accumulates into (n_views * N records of
+        // 12 floats, view-major): each block streams zeros over its share. The stores retire in
+        // the background of this VALU-bound kernel instead of taking HBM time of their own.
+        const int64_t n4 = int64_t(a.n_views) * a.n_gauss * (kGradStride / 4);
+        const int64_t nb = gridDim.x, b = blockIdx.x;
+        const int64_t lo = n4 * b / nb, hi = n4 * (b + 1) / nb;
+        float4* g4 = reinterpret_cast<float4*>(a.sgrad);
+        for (int64_t j = lo + threadIdx.x; j < hi; j += 64 * WPB) g4[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    if (a.counters->overflow) return; // incomplete instance list: the host re-runs with more capacity
+    __shared__ float4 sA[WPB][BATCH];
+    __shared__ float4 sB[WPB][BATCH];
+    __shared__ float sC[WPB][BATCH];
+    __shared__ uint32_t sM[WPB][BATCH];
+    __shared__ uint32_t sV[WPB][BATCH];
+    const int w = int(__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)); // wave-uniform
+    for_each_item<WPB>(a, a.queue ? a.queue : nullptr, n_items, [&](int item) {
+        blend_fwd_item<TS, BATCH>(a, item, n_items, sA[w], sB[w], sC[w], sM[w], sV[w]);
+    });
+}
+
+// One (view, tile) work item of the backward: back-to-front from the stored final transmittance.
 template <int TS, int BATCH, bool REFINE>
-__global__ __launch_bounds__(64) void k_blend_bwd(BlendArgs a) {
+__device__ __forceinline__ void blend_bwd_item(const BlendArgs& a, int item, int n_items, float4* sA, float4* sB,
+                                               float4* sC, uint32_t* sM, uint32_t* sS) {
     constexpr int Q = (TS / 8) * (TS / 8);
     constexpr int NG = 10;
     const DevView* dvp;
-    const TileCoord tc = tile_of_block(a, dvp);
+    const TileCoord tc = tile_of_item(a, item, n_items, dvp);
     const DevView& dv = *dvp;
-    const int lane = threadIdx.x;
+    const int lane = int(threadIdx.x) & 63;
     const int slot_g = kScatterSlot[lane & 15];
     const int W = dv.vi.cam.width, H = dv.vi.cam.height;
-    if (a.counters->overflow) return;
     const uint2 range = a.ranges[tc.tile];
-
-    __shared__ float4 sA[BATCH];
-    __shared__ float4 sB[BATCH];
-    __shared__ float4 sC[BATCH]; // (colour b, conic a, b, c): the gradient terms take the conic itself
-    __shared__ uint32_t sM[BATCH];
-    __shared__ uint32_t sS[BATCH];
 
     // E = (colour composited behind) . dL/dC + T_final * dL/dT_final, the pixel's whole
     // "behind" term of dL/dalpha (times 1/(1 - alpha)); it is carried as one scalar instead of
@@ -328,12 +367,13 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendArgs a) {
         tile_max = qmax[k] > tile_max ? qmax[k] : tile_max;
     }
 
-    const float clip = a.alpha_clip;
+    // canonical once, so the per-pair min(clip, .) needs no canonicalisation of its own
+    const float clip = __builtin_canonicalizef(a.alpha_clip);
     const float fW = float(W), fH = float(H);
     for (int hi = int(tile_max); hi > 0; hi -= BATCH) {
         const int lo = hi - BATCH > 0 ? hi - BATCH : 0;
         const int cnt = hi - lo;
-        __syncthreads(); // the previous batch's LDS reads are complete
+        wave_lds_sync(); // the previous batch's LDS reads are complete
 #pragma unroll
         for (int q = 0; q < (BATCH + 63) / 64; ++q) {
             const int j = q * 64 + lane;
@@ -351,7 +391,7 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendArgs a) {
                 sS[j] = slot;
             }
         }
-        __syncthreads();
+        wave_lds_sync();
         for (int j = cnt - 1; j >= 0; --j) {
             const uint32_t idx = uint32_t(lo + j);
             uint32_t m = __builtin_amdgcn_readfirstlane(sM[j]);
@@ -374,51 +414,112 @@ __global__ __launch_bounds__(64) void k_blend_bwd(BlendArgs a) {
                     pair_alpha_flat<true>(A.z, A.w, B.x, B.y, dx, dy, clip, alpha, G) && idx < last[k];
                 if (__ballot(contrib) == 0) continue;
                 any = true;
-                // Branch-free over the wave: lanes without a contributing pair zero the three
-                // factors every term is a product of (dL/dalpha, dL/dG, the blend weight) and
-                // their offsets, so their terms are exact zeros and T, E keep their values.
+                // Branch-free over the wave: lanes without a contributing pair zero the factors
+                // every term is a product of (dL/dalpha, hence dL/dG, and the blend weight); G and
+                // the offsets are finite on every lane (a rejected pair's G is 1), so their terms
+                // are exact zeros and T, E keep their values.
                 // (The recovered T and the gradients are not compared bit for bit: the hardware
                 // reciprocal / square root and fused multiply-adds are used throughout.)
-                const float ox = contrib ? dx : 0.0f, oy = contrib ? dy : 0.0f;
+                const float ox = dx, oy = dy;
                 const float inv_1ma = __builtin_amdgcn_rcpf(1.0f - alpha);
                 const float Tb = T[k] * inv_1ma;
                 const float wgt = contrib ? alpha * Tb : 0.0f;
                 const float c_dot = fmaf(cb, gp2[k], fmaf(B.w, gp1[k], B.z * gp0[k]));
                 const float dL_dalpha = contrib ? fmaf(Tb, c_dot, -E[k] * inv_1ma) : 0.0f;
                 const float dL_dG = B.y * dL_dalpha;
-                const float gdx = G * ox, gdy = G * oy;
-                // dL/dmean2d = dL/dG * dG/d(dx, dy), dG/ddx = -G (a dx + b dy), dG/ddy = -G (b dx + c dy)
-                const float ddx = -dL_dG * fmaf(gdx, Cc.y, gdy * Cc.z);
-                const float ddy = -dL_dG * fmaf(gdy, Cc.w, gdx * Cc.z);
-                r[0] += ddx;
-                r[1] += ddy;
-                // dL/dconic = -dL/dG * G * (dx^2, dx dy, dy^2) / 2
-                const float h = -0.5f * dL_dG;
-                const float hx = h * gdx;
-                r[2] = fmaf(hx, ox, r[2]);
-                r[3] = fmaf(hx, oy, r[3]);
-                r[4] = fmaf(h * gdy, oy, r[4]);
+                if constexpr (REFINE) {
+                    const float gdx = G * ox, gdy = G * oy;
+                    // dL/dmean2d = dL/dG * dG/d(dx, dy), dG/ddx = -G (a dx + b dy), dG/ddy = -G (b dx + c dy)
+                    const float ddx = -dL_dG * fmaf(gdx, Cc.y, gdy * Cc.z);
+                    const float ddy = -dL_dG * fmaf(gdy, Cc.w, gdx * Cc.z);
+                    r[0] += ddx;
+                    r[1] += ddy;
+                    // dL/dconic = -dL/dG * G * (dx^2, dx dy, dy^2) / 2
+                    const float h = -0.5f * dL_dG;
+                    const float hx = h * gdx;
+                    r[2] = fmaf(hx, ox, r[2]);
+                    r[3] = fmaf(hx, oy, r[3]);
+                    r[4] = fmaf(h * gdy, oy, r[4]);
+                    // the refine statistic is a norm per pixel: it needs the per-pair dL/dmean2d
+                    const float rx = ddx * fW, ry = ddy * fH;
+                    r[9] = fmaf(__builtin_amdgcn_sqrtf(fmaf(rx, rx, ry * ry)), refine_den[k], r[9]);
+                } else {
+                    // Without it, both gradients are linear in the weighted moments of the offsets,
+                    // w = dL/dG * G: sum w (dx, dy) and sum w (dx^2, dx dy, dy^2). The conic and the
+                    // -1/2 are applied once per (instance, tile) after the reduction.
+                    const float wG = dL_dG * G;
+                    const float t = wG * ox, u = wG * oy;
+                    r[0] += t;
+                    r[1] += u;
+                    r[2] = fmaf(t, ox, r[2]);
+                    r[3] = fmaf(t, oy, r[3]);
+                    r[4] = fmaf(u, oy, r[4]);
+                }
                 r[5] = fmaf(G, dL_dalpha, r[5]);
                 r[6] = fmaf(wgt, gp0[k], r[6]);
                 r[7] = fmaf(wgt, gp1[k], r[7]);
                 r[8] = fmaf(wgt, gp2[k], r[8]);
-                if constexpr (REFINE) {
-                    const float rx = ddx * fW, ry = ddy * fH;
-                    r[9] = fmaf(__builtin_amdgcn_sqrtf(fmaf(rx, rx, ry * ry)), refine_den[k], r[9]);
-                }
                 E[k] = fmaf(c_dot, wgt, E[k]);
                 T[k] = contrib ? Tb : T[k];
             }
             if (!any) continue;
-            const float tot = wave_scatter10(r, lane);
+            float tot = wave_scatter10(r, lane);
+            if constexpr (!REFINE) {
+                // moments -> gradients: dL/dmean2d = -(conic . (sum w dx, sum w dy)),
+                // dL/dconic = -1/2 (sum w dx^2, sum w dx dy, sum w dy^2); the first-order moments
+                // sit in lanes 0 and 8 (kScatterSlot)
+                const float sx = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, tot), 0));
+                const float sy = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, tot), 8));
+                const float4 Cc = sC[j];
+                if (slot_g == 0) tot = -fmaf(Cc.y, sx, Cc.z * sy);
+                else if (slot_g == 1) tot = -fmaf(Cc.w, sy, Cc.z * sx);
+                else if (slot_g >= 2 && slot_g <= 4) tot = -0.5f * tot;
+            }
             if (lane < 16 && slot_g >= 0 && tot != 0.0f)
                 atomic_add_f32(a.sgrad + grad_offset(sS[j], a.vp_log2, a.n_gauss) + slot_g, tot);
         }
     }
 }
 
-template <int TS, int BATCH> static void fwd_launch(const BlendArgs& a, int n_tiles, hipStream_t st) {
-    hipLaunchKernelGGL((k_blend_fwd<TS, BATCH>), dim3(n_tiles), dim3(64), 0, st, a);
+template <int TS, int BATCH, int WPB, bool REFINE>
+__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(5))) void k_blend_bwd(BlendArgs a, int n_items) {
+    if (a.counters->overflow) return;
+    __shared__ float4 sA[WPB][BATCH];
+    __shared__ float4 sB[WPB][BATCH];
+    __shared__ float4 sC[WPB][BATCH]; // (colour b, conic a, b, c): the gradient terms take the conic itself
+    __shared__ uint32_t sM[WPB][BATCH];
+    __shared__ uint32_t sS[WPB][BATCH];
+    const int w = int(__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)); // wave-uniform
+    for_each_item<WPB>(a, a.queue ? a.queue + 1 : nullptr, n_items, [&](int item) {
+        blend_bwd_item<TS, BATCH, REFINE>(a, item, n_items, sA[w], sB[w], sC[w], sM[w], sS[w]);
+    });
+}
+
+// Workgroups of `kernel` resident on the whole device at once (occupancy per CU x CUs): the grid of
+// a persistent work-queue launch.
+static int resident_blocks(const void* kernel, int threads) {
+    int per_cu = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 1;
+    return per_cu * cus;
+}
+
+template <class K> static unsigned grid_for(K kernel, const BlendArgs& a, int n_items, int wpb) {
+    int blocks = (n_items + wpb - 1) / wpb;
+    if (a.queue) blocks = std::min(blocks, resident_blocks(reinterpret_cast<const void*>(kernel), 64 * wpb));
+    return unsigned(blocks);
+}
+
+template <int TS, int BATCH, int WPB> static void fwd_launch(const BlendArgs& a, int n_items, hipStream_t st) {
+    const auto k = k_blend_fwd<TS, BATCH, WPB>;
+    hipLaunchKernelGGL(k, dim3(grid_for(k, a, n_items, WPB)), dim3(64 * WPB), 0, st, a, n_items);
+}
+template <int TS, int BATCH> static void fwd_shape(const BlendArgs& a, int n_items, int waves, hipStream_t st) {
+    if (waves >= 4) fwd_launch<TS, BATCH, 4>(a, n_items, st);
+    else if (waves == 2) fwd_launch<TS, BATCH, 2>(a, n_items, st);
+    else fwd_launch<TS, BATCH, 1>(a, n_items, st);
 }
 // ---------------------------------------------------------------- dispatch order of the tiles
 // Blocks start in blockIdx order. A heavy tile that starts last runs as a lone wave after the rest
@@ -481,36 +582,52 @@ void launch_tile_order(const uint2* ranges, uint32_t n_tiles, uint32_t* order, u
     k_order_scatter<<<blocks, 256, 0, st>>>(ranges, n_tiles, hist, order);
 }
 
-template <int TS, int BATCH> static void bwd_launch(const BlendArgs& a, int n_tiles, hipStream_t st) {
-    if (a.want_refine) hipLaunchKernelGGL((k_blend_bwd<TS, BATCH, true>), dim3(n_tiles), dim3(64), 0, st, a);
-    else hipLaunchKernelGGL((k_blend_bwd<TS, BATCH, false>), dim3(n_tiles), dim3(64), 0, st, a);
+template <int TS, int BATCH, int WPB, bool R> static void bwd_launch(const BlendArgs& a, int n_items, hipStream_t st) {
+    const auto k = k_blend_bwd<TS, BATCH, WPB, R>;
+    hipLaunchKernelGGL(k, dim3(grid_for(k, a, n_items, WPB)), dim3(64 * WPB), 0, st, a, n_items);
 }
-
-void launch_blend_forward(const BlendArgs& a, int n_tiles, int tile_size, int batch, hipStream_t st) {
-    if (n_tiles <= 0) return;
-    if (tile_size == 8) {
-        if (batch <= 32) fwd_launch<8, 32>(a, n_tiles, st);
-        else if (batch <= 64) fwd_launch<8, 64>(a, n_tiles, st);
-        else fwd_launch<8, 128>(a, n_tiles, st);
+template <int TS, int BATCH> static void bwd_shape(const BlendArgs& a, int n_items, int waves, hipStream_t st) {
+    if (a.want_refine) {
+        if (waves >= 4) bwd_launch<TS, BATCH, 4, true>(a, n_items, st);
+        else if (waves == 2) bwd_launch<TS, BATCH, 2, true>(a, n_items, st);
+        else bwd_launch<TS, BATCH, 1, true>(a, n_items, st);
     } else {
-        if (batch <= 32) fwd_launch<16, 32>(a, n_tiles, st);
-        else if (batch <= 64) fwd_launch<16, 64>(a, n_tiles, st);
-        else if (batch <= 128) fwd_launch<16, 128>(a, n_tiles, st);
-        else fwd_launch<16, 256>(a, n_tiles, st);
+        if (waves >= 4) bwd_launch<TS, BATCH, 4, false>(a, n_items, st);
+        else if (waves == 2) bwd_launch<TS, BATCH, 2, false>(a, n_items, st);
+        else bwd_launch<TS, BATCH, 1, false>(a, n_items, st);
     }
 }
 
-void launch_blend_backward(const BlendArgs& a, int n_tiles, int tile_size, int batch, hipStream_t st) {
+__global__ void k_queue_reset(uint32_t* q) { *q = 0u; }
+
+void launch_blend_forward(const BlendArgs& a, int n_tiles, int tile_size, int batch, int waves, hipStream_t st) {
     if (n_tiles <= 0) return;
     if (tile_size == 8) {
-        if (batch <= 32) bwd_launch<8, 32>(a, n_tiles, st);
-        else if (batch <= 64) bwd_launch<8, 64>(a, n_tiles, st);
-        else bwd_launch<8, 128>(a, n_tiles, st);
+        if (batch <= 32) fwd_shape<8, 32>(a, n_tiles, waves, st);
+        else if (batch <= 64) fwd_shape<8, 64>(a, n_tiles, waves, st);
+        else fwd_shape<8, 128>(a, n_tiles, waves, st);
     } else {
-        if (batch <= 32) bwd_launch<16, 32>(a, n_tiles, st);
-        else if (batch <= 64) bwd_launch<16, 64>(a, n_tiles, st);
-        else if (batch <= 128) bwd_launch<16, 128>(a, n_tiles, st);
-        else bwd_launch<16, 256>(a, n_tiles, st);
+        if (batch <= 32) fwd_shape<16, 32>(a, n_tiles, waves, st);
+        else if (batch <= 64) fwd_shape<16, 64>(a, n_tiles, waves, st);
+        else if (batch <= 128) fwd_shape<16, 128>(a, n_tiles, waves, st);
+        else fwd_shape<16, 256>(a, n_tiles, waves, st);
+    }
+}
+
+void launch_blend_backward(const BlendArgs& a, int n_tiles, int tile_size, int batch, int waves, hipStream_t st) {
+    if (n_tiles <= 0) return;
+    // the backward's queue counter starts from zero at every backward (a forward may be
+    // differentiated more than once)
+    if (a.queue) hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(1), 0, st, a.queue + 1);
+    if (tile_size == 8) {
+        if (batch <= 32) bwd_shape<8, 32>(a, n_tiles, waves, st);
+        else if (batch <= 64) bwd_shape<8, 64>(a, n_tiles, waves, st);
+        else bwd_shape<8, 128>(a, n_tiles, waves, st);
+    } else {
+        if (batch <= 32) bwd_shape<16, 32>(a, n_tiles, waves, st);
+        else if (batch <= 64) bwd_shape<16, 64>(a, n_tiles, waves, st);
+        else if (batch <= 128) bwd_shape<16, 128>(a, n_tiles, waves, st);
+        else bwd_shape<16, 256>(a, n_tiles, waves, st);
     }
 }
 

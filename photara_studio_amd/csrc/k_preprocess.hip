@@ -163,10 +163,11 @@ __global__ __launch_bounds__(256) void k_preprocess(PreArgs a, int64_t n_slots) 
 // or that would overflow the instance capacity, falls back to per-lane emission.
 constexpr int kEmitCols = 512; // column records per wave
 
+template <class K>
 __global__ __launch_bounds__(256) void k_emit(const uint32_t* sorted_slots, const uint32_t* sorted_tiles,
                                               const uint32_t* offsets, const Counters* counters, uint32_t slot_cap,
                                               const SplatRec* rec, const DevView* views, int vp_log2, int ts,
-                                              uint32_t inst_cap, uint32_t* keys_out, uint32_t* vals_out) {
+                                              uint32_t inst_cap, K* keys_out, uint32_t* vals_out) {
     __shared__ uint2 s_col[4][kEmitCols]; // (run position of the column, lane << 26 | u << 13 | v_min)
     __shared__ uint32_t s_slot[4][64];
     __shared__ uint32_t s_tb[4][64];
@@ -214,7 +215,7 @@ __global__ __launch_bounds__(256) void k_emit(const uint32_t* sorted_slots, cons
                     for (int tv = v0; tv < v1; ++tv) {
                         if (k < count) {
                             const int tx = tw.iterate_x ? tu : tv, ty = tw.iterate_x ? tv : tu;
-                            keys_out[off + k] = tb + uint32_t(ty) * gx + uint32_t(tx);
+                            keys_out[off + k] = K(tb + uint32_t(ty) * gx + uint32_t(tx));
                             vals_out[off + k] = slot;
                         }
                         ++k;
@@ -261,7 +262,7 @@ __global__ __launch_bounds__(256) void k_emit(const uint32_t* sorted_slots, cons
             const uint32_t g = s_gx[w][L];
             const bool it = (g >> 31) != 0;
             const uint32_t tx = it ? u : v, ty = it ? v : u;
-            keys_out[run_base + o] = s_tb[w][L] + ty * (g & 0x7fffffffu) + tx;
+            keys_out[run_base + o] = K(s_tb[w][L] + ty * (g & 0x7fffffffu) + tx);
             vals_out[run_base + o] = s_slot[w][L];
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -270,9 +271,20 @@ __global__ __launch_bounds__(256) void k_emit(const uint32_t* sorted_slots, cons
     }
 }
 
+// Four consecutive keys as one vector load (16 bytes of 32-bit keys, 8 bytes of 16-bit keys).
+__device__ __forceinline__ void load_key4(const uint32_t* keys, uint32_t q, uint32_t out[4]) {
+    const uint4 v = reinterpret_cast<const uint4*>(keys)[q];
+    out[0] = v.x; out[1] = v.y; out[2] = v.z; out[3] = v.w;
+}
+__device__ __forceinline__ void load_key4(const uint16_t* keys, uint32_t q, uint32_t out[4]) {
+    const uint2 v = reinterpret_cast<const uint2*>(keys)[q];
+    out[0] = v.x & 0xffffu; out[1] = v.x >> 16; out[2] = v.y & 0xffffu; out[3] = v.y >> 16;
+}
+
 // Per-tile [start, end) of the tile-sorted list (extract_tile_ranges, render_forward.cu:215-232).
-// Each thread takes four consecutive keys (one 16-byte load) and their two outer neighbours.
-__global__ __launch_bounds__(256) void k_ranges(const uint32_t* keys, const Counters* counters, uint32_t inst_cap,
+// Each thread takes four consecutive keys (one vector load) and their two outer neighbours.
+template <class K>
+__global__ __launch_bounds__(256) void k_ranges(const K* keys, const Counters* counters, uint32_t inst_cap,
                                                 uint32_t n_tiles, uint2* ranges) {
     if (counters->overflow) return; // the list is incomplete: leave every range empty
     const uint32_t ni = counters->instances;
@@ -282,17 +294,16 @@ __global__ __launch_bounds__(256) void k_ranges(const uint32_t* keys, const Coun
         const uint32_t i0 = q << 2;
         uint32_t k[6]; // k[0] = keys[i0 - 1], k[1..4] = keys[i0 .. i0 + 3], k[5] = keys[i0 + 4]
         if (i0 + 3u < n) {
-            const uint4 v = reinterpret_cast<const uint4*>(keys)[q];
-            k[1] = v.x; k[2] = v.y; k[3] = v.z; k[4] = v.w;
+            load_key4(keys, q, k + 1);
         } else {
-            for (int e = 0; e < 4; ++e) k[1 + e] = i0 + uint32_t(e) < n ? keys[i0 + uint32_t(e)] : 0xffffffffu;
+            for (int e = 0; e < 4; ++e) k[1 + e] = i0 + uint32_t(e) < n ? uint32_t(keys[i0 + uint32_t(e)]) : 0xffffffffu;
         }
         // The neighbours come from the adjacent lanes, which hold the adjacent quads (lane l + 1
         // holds quad q + 1 whenever key i0 + 4 exists); only the wave's end lanes load them.
         const int lane = int(threadIdx.x & 63u);
         const uint32_t up = __shfl_up(k[4], 1, 64), down = __shfl_down(k[1], 1, 64);
-        k[0] = i0 == 0u ? 0xffffffffu : (lane > 0 ? up : keys[i0 - 1u]);
-        k[5] = i0 + 4u < n ? (lane < 63 ? down : keys[i0 + 4u]) : 0xffffffffu;
+        k[0] = i0 == 0u ? 0xffffffffu : (lane > 0 ? up : uint32_t(keys[i0 - 1u]));
+        k[5] = i0 + 4u < n ? (lane < 63 ? down : uint32_t(keys[i0 + 4u])) : 0xffffffffu;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const uint32_t i = i0 + uint32_t(e);
@@ -304,7 +315,8 @@ __global__ __launch_bounds__(256) void k_ranges(const uint32_t* keys, const Coun
     }
 }
 
-__global__ __launch_bounds__(256) void k_debug_keys(const uint32_t* tile_keys, const uint32_t* slots,
+template <class K>
+__global__ __launch_bounds__(256) void k_debug_keys(const K* tile_keys, const uint32_t* slots,
                                                     const uint4* meta, const Counters* counters, uint32_t inst_cap,
                                                     int vp_log2, uint64_t* keys_out, uint32_t* vals_out) {
     const uint32_t ni = counters->instances;
@@ -370,25 +382,43 @@ void launch_preprocess(const PreArgs& a, int64_t n_slots, hipStream_t st) {
     else pre_deg<PHT_LP_NONE>(a, n_slots, st);
 }
 
+// Tile keys are 16-bit when every global tile id fits (narrow), else 32-bit; the key buffers are
+// allocated as 32-bit arrays either way.
 void launch_emit_instances(const uint32_t* sorted_slots, const uint32_t* sorted_tiles, const uint32_t* offsets,
                            const Counters* counters, uint32_t slot_cap, const SplatRec* rec, const DevView* views,
                            int vp_log2, int tile_size, uint32_t inst_cap, uint32_t* keys_out, uint32_t* vals_out,
-                           hipStream_t st) {
-    hipLaunchKernelGGL(k_emit, dim3(grid_stride_blocks(slot_cap)), dim3(256), 0, st, sorted_slots, sorted_tiles,
-                       offsets, counters, slot_cap, rec, views, vp_log2, tile_size, inst_cap, keys_out, vals_out);
+                           bool narrow, hipStream_t st) {
+    const dim3 grid(grid_stride_blocks(slot_cap));
+    if (narrow)
+        hipLaunchKernelGGL(k_emit<uint16_t>, grid, dim3(256), 0, st, sorted_slots, sorted_tiles, offsets, counters,
+                           slot_cap, rec, views, vp_log2, tile_size, inst_cap, reinterpret_cast<uint16_t*>(keys_out),
+                           vals_out);
+    else
+        hipLaunchKernelGGL(k_emit<uint32_t>, grid, dim3(256), 0, st, sorted_slots, sorted_tiles, offsets, counters,
+                           slot_cap, rec, views, vp_log2, tile_size, inst_cap, keys_out, vals_out);
 }
 
 void launch_ranges(const uint32_t* keys, const Counters* counters, uint32_t inst_cap, uint32_t n_tiles,
-                   uint2* ranges, hipStream_t st) {
-    hipLaunchKernelGGL(k_ranges, dim3(grid_stride_blocks(inst_cap)), dim3(256), 0, st, keys, counters, inst_cap,
-                       n_tiles, ranges);
+                   uint2* ranges, bool narrow, hipStream_t st) {
+    const dim3 grid(grid_stride_blocks(inst_cap));
+    if (narrow)
+        hipLaunchKernelGGL(k_ranges<uint16_t>, grid, dim3(256), 0, st, reinterpret_cast<const uint16_t*>(keys),
+                           counters, inst_cap, n_tiles, ranges);
+    else
+        hipLaunchKernelGGL(k_ranges<uint32_t>, grid, dim3(256), 0, st, keys, counters, inst_cap, n_tiles, ranges);
 }
 
 void launch_debug_keys(const uint32_t* tile_keys, const uint32_t* slots, const uint4* meta,
                        const Counters* counters, uint32_t inst_cap, int vp_log2, uint64_t* keys_out,
-                       uint32_t* vals_out, hipStream_t st) {
-    hipLaunchKernelGGL(k_debug_keys, dim3(grid_stride_blocks(inst_cap)), dim3(256), 0, st, tile_keys, slots, meta,
-                       counters, inst_cap, vp_log2, keys_out, vals_out);
+                       uint32_t* vals_out, bool narrow, hipStream_t st) {
+    const dim3 grid(grid_stride_blocks(inst_cap));
+    if (narrow)
+        hipLaunchKernelGGL(k_debug_keys<uint16_t>, grid, dim3(256), 0, st,
+                           reinterpret_cast<const uint16_t*>(tile_keys), slots, meta, counters, inst_cap, vp_log2,
+                           keys_out, vals_out);
+    else
+        hipLaunchKernelGGL(k_debug_keys<uint32_t>, grid, dim3(256), 0, st, tile_keys, slots, meta, counters,
+                           inst_cap, vp_log2, keys_out, vals_out);
 }
 
 } // namespace pht

@@ -145,9 +145,13 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_write(Src src, Sink sink,
 // flight (four times as many as 256-thread blocks; the six passes of the bench workload went
 // from 456 to 409 us, still short of HBM rate, with the LDS atomics the other limit).
 constexpr int kUpThreads = 1024;
-__global__ __launch_bounds__(kUpThreads) void k_radix_upsweep(const uint32_t* keys, const uint32_t* n_ptr,
+// K = uint32_t, or uint16_t for tile keys when every global tile id fits 16 bits (half the key
+// bytes of every pass). A 16-byte vector holds KV keys.
+template <class K>
+__global__ __launch_bounds__(kUpThreads) void k_radix_upsweep(const K* keys, const uint32_t* n_ptr,
                                                               uint32_t n_const, uint32_t cap, int shift, int nb,
                                                               uint32_t* hist) {
+    constexpr uint32_t KV = 16u / sizeof(K);
     __shared__ uint32_t s_h[4][256];
     const int t = threadIdx.x, w = t >> 8;
     s_h[w][t & 255] = 0;
@@ -155,27 +159,35 @@ __global__ __launch_bounds__(kUpThreads) void k_radix_upsweep(const uint32_t* ke
     const uint32_t n = load_count(n_ptr, n_const, cap);
     uint32_t lo, hi;
     chunk_of(n, nb, blockIdx.x, lo, hi);
-    // lo is a multiple of 256 keys, so the uint4 view of [lo, hi4) is 16-byte aligned; four
-    // uint4 loads per thread are issued before any of them is counted (bytes in flight)
-    const uint32_t hi4 = lo + ((hi - lo) & ~3u);
+    // lo is a multiple of 256 keys, so the 16-byte view of [lo, hiv) is aligned; four vector
+    // loads per thread are issued before any of them is counted (bytes in flight)
+    const uint32_t hiv = lo + ((hi - lo) & ~(KV - 1u));
     const uint4* k4 = reinterpret_cast<const uint4*>(keys);
-    constexpr uint32_t kStep = 4u * kUpThreads;
-    auto count4 = [&](const uint4& q) {
-        atomicAdd(&s_h[w][(q.x >> shift) & 255u], 1u);
-        atomicAdd(&s_h[w][(q.y >> shift) & 255u], 1u);
-        atomicAdd(&s_h[w][(q.z >> shift) & 255u], 1u);
-        atomicAdd(&s_h[w][(q.w >> shift) & 255u], 1u);
+    constexpr uint32_t kStep = KV * kUpThreads;
+    auto count_word = [&](uint32_t x) {
+        if constexpr (sizeof(K) == 4) {
+            atomicAdd(&s_h[w][(x >> shift) & 255u], 1u);
+        } else {
+            atomicAdd(&s_h[w][((x & 0xffffu) >> shift) & 255u], 1u);
+            atomicAdd(&s_h[w][((x >> 16) >> shift) & 255u], 1u);
+        }
     };
-    uint32_t i = lo + 4u * t;
-    for (; i + 3u * kStep < hi4; i += 4u * kStep) {
+    auto count_vec = [&](const uint4& q) {
+        count_word(q.x);
+        count_word(q.y);
+        count_word(q.z);
+        count_word(q.w);
+    };
+    uint32_t i = lo + KV * t;
+    for (; i + 3u * kStep < hiv; i += 4u * kStep) {
         uint4 q[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) q[u] = k4[(i + uint32_t(u) * kStep) >> 2];
+        for (int u = 0; u < 4; ++u) q[u] = k4[(i + uint32_t(u) * kStep) / KV];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) count4(q[u]);
+        for (int u = 0; u < 4; ++u) count_vec(q[u]);
     }
-    for (; i < hi4; i += kStep) count4(k4[i >> 2]);
-    for (uint32_t i = hi4 + t; i < hi; i += kUpThreads) atomicAdd(&s_h[w][(keys[i] >> shift) & 255u], 1u);
+    for (; i < hiv; i += kStep) count_vec(k4[i / KV]);
+    for (uint32_t i = hiv + t; i < hi; i += kUpThreads) atomicAdd(&s_h[w][(uint32_t(keys[i]) >> shift) & 255u], 1u);
     __syncthreads();
     if (t < 256) hist[size_t(t) * nb + blockIdx.x] = (s_h[0][t] + s_h[1][t]) + (s_h[2][t] + s_h[3][t]);
 }
@@ -200,17 +212,17 @@ __global__ __launch_bounds__(1024) void k_radix_scan(uint32_t* hist, int nb, uin
     if (t == 0) digit_total[blockIdx.x] = total;
 }
 
-template <bool AUX, int BT, int R>
-__global__ __launch_bounds__(BT) void k_radix_downsweep(const uint32_t* keys_in, const uint32_t* vals_in,
-                                                                  const uint32_t* aux_in, uint32_t* keys_out,
+template <class K, bool AUX, int BT, int R>
+__global__ __launch_bounds__(BT) void k_radix_downsweep(const K* keys_in, const uint32_t* vals_in,
+                                                                  const uint32_t* aux_in, K* keys_out,
                                                                   uint32_t* vals_out, uint32_t* aux_out,
                                                                   const uint32_t* n_ptr, uint32_t n_const,
                                                                   uint32_t cap, int shift, int nb,
                                                                   const uint32_t* hist, const uint32_t* digit_total) {
     constexpr int NW = BT / 64;
     constexpr int kSortTile = BT * R;
-    static_assert(kSortTile * 4 >= BT / 64 * 256 * 8, "the digit-match tables live in s_key");
-    __shared__ __align__(8) uint32_t s_key[kSortTile];
+    static_assert(kSortTile * int(sizeof(K)) >= BT / 64 * 256 * 8, "the digit-match tables live in s_key");
+    __shared__ __align__(8) K s_key[kSortTile];
     __shared__ uint32_t s_val[kSortTile];
     __shared__ uint32_t s_aux[AUX ? kSortTile : 1];
     __shared__ uint32_t s_cnt[NW][256]; // per-wave running digit counts, then per-wave offsets
@@ -240,7 +252,7 @@ __global__ __launch_bounds__(BT) void k_radix_downsweep(const uint32_t* keys_in,
         for (int r = 0; r < R; ++r) {
             const uint32_t i = wb + uint32_t(r) * 64u + uint32_t(lane);
             const bool valid = i < hi;
-            key[r] = valid ? keys_in[i] : 0u;
+            key[r] = valid ? uint32_t(keys_in[i]) : 0u;
             val[r] = valid ? vals_in[i] : 0u;
             if constexpr (AUX) aux[r] = valid ? aux_in[i] : 0u;
         }
@@ -301,7 +313,7 @@ __global__ __launch_bounds__(BT) void k_radix_downsweep(const uint32_t* keys_in,
             if (i < hi) {
                 const uint32_t d = (key[r] >> shift) & 255u;
                 const uint32_t p = s_start[d] + s_cnt[w][d] + rank[r];
-                s_key[p] = key[r];
+                s_key[p] = K(key[r]);
                 s_val[p] = val[r];
                 if constexpr (AUX) s_aux[p] = aux[r];
             }
@@ -310,11 +322,11 @@ __global__ __launch_bounds__(BT) void k_radix_downsweep(const uint32_t* keys_in,
         __syncthreads();
         const uint32_t tile_n = hi - base < uint32_t(kSortTile) ? hi - base : uint32_t(kSortTile);
         for (uint32_t j = t; j < tile_n; j += BT) {
-            const uint32_t k = s_key[j];
+            const uint32_t k = uint32_t(s_key[j]);
             const uint32_t d = (k >> shift) & 255u;
             const uint32_t dst = s_gbase[d] + (j - s_start[d]);
             if (dst >= n) continue; // unreachable for consistent ranks; keeps every store in bounds
-            keys_out[dst] = k;
+            keys_out[dst] = K(k);
             vals_out[dst] = s_val[j];
             if constexpr (AUX) aux_out[dst] = s_aux[j];
         }
@@ -359,22 +371,33 @@ void launch_tile_offsets(const uint32_t* sorted_tiles, const uint32_t* visible_p
 }
 
 int launch_radix_sort(uint32_t* keys[2], uint32_t* vals[2], uint32_t* aux[2], const uint32_t* n_ptr, uint32_t cap,
-                      int bits, uint32_t* hist, uint32_t* digit_total, hipStream_t st) {
+                      int bits, uint32_t* hist, uint32_t* digit_total, hipStream_t st, bool narrow) {
     // 1024-thread blocks, one resident per CU: at most 256 chunks
     const int nb = blocks_for(cap) < 256 ? blocks_for(cap) : 256;
     int cur = 0;
     for (int shift = 0; shift < bits; shift += 8) {
-        hipLaunchKernelGGL(k_radix_upsweep, dim3(nb), dim3(kUpThreads), 0, st, keys[cur], n_ptr, 0u, cap, shift,
-                           nb, hist);
-        hipLaunchKernelGGL(k_radix_scan, dim3(256), dim3(1024), 0, st, hist, nb, digit_total);
-        if (aux)
-            hipLaunchKernelGGL((k_radix_downsweep<true, 1024, 8>), dim3(nb), dim3(1024), 0, st, keys[cur], vals[cur],
-                               aux[cur], keys[cur ^ 1], vals[cur ^ 1], aux[cur ^ 1], n_ptr, 0u, cap, shift, nb,
-                               hist, digit_total);
-        else
-            hipLaunchKernelGGL((k_radix_downsweep<false, 1024, 16>), dim3(nb), dim3(1024), 0, st, keys[cur], vals[cur],
-                               nullptr, keys[cur ^ 1], vals[cur ^ 1], nullptr, n_ptr, 0u, cap, shift, nb, hist,
+        if (narrow) {
+            uint16_t* kin = reinterpret_cast<uint16_t*>(keys[cur]);
+            uint16_t* kout = reinterpret_cast<uint16_t*>(keys[cur ^ 1]);
+            hipLaunchKernelGGL(k_radix_upsweep<uint16_t>, dim3(nb), dim3(kUpThreads), 0, st, kin, n_ptr, 0u, cap,
+                               shift, nb, hist);
+            hipLaunchKernelGGL(k_radix_scan, dim3(256), dim3(1024), 0, st, hist, nb, digit_total);
+            hipLaunchKernelGGL((k_radix_downsweep<uint16_t, false, 1024, 16>), dim3(nb), dim3(1024), 0, st, kin,
+                               vals[cur], nullptr, kout, vals[cur ^ 1], nullptr, n_ptr, 0u, cap, shift, nb, hist,
                                digit_total);
+        } else {
+            hipLaunchKernelGGL(k_radix_upsweep<uint32_t>, dim3(nb), dim3(kUpThreads), 0, st, keys[cur], n_ptr, 0u,
+                               cap, shift, nb, hist);
+            hipLaunchKernelGGL(k_radix_scan, dim3(256), dim3(1024), 0, st, hist, nb, digit_total);
+            if (aux)
+                hipLaunchKernelGGL((k_radix_downsweep<uint32_t, true, 1024, 8>), dim3(nb), dim3(1024), 0, st,
+                                   keys[cur], vals[cur], aux[cur], keys[cur ^ 1], vals[cur ^ 1], aux[cur ^ 1], n_ptr,
+                                   0u, cap, shift, nb, hist, digit_total);
+            else
+                hipLaunchKernelGGL((k_radix_downsweep<uint32_t, false, 1024, 16>), dim3(nb), dim3(1024), 0, st,
+                                   keys[cur], vals[cur], nullptr, keys[cur ^ 1], vals[cur ^ 1], nullptr, n_ptr, 0u,
+                                   cap, shift, nb, hist, digit_total);
+        }
         cur ^= 1;
     }
     return cur;

@@ -17,6 +17,8 @@
 // two images and writing one gradient instead of round-tripping three partial maps per channel.
 // The loss is summed over each tile's own valid pixels into one partial per workgroup, reduced
 // in a fixed order by a second one-block kernel.
+#include <mutex>
+
 #include "launch.h"
 
 namespace pht {
@@ -202,6 +204,35 @@ void launch_l1_ssim(const float* color, const float* target, const float* mask, 
                        height, ssim_weight, norm);
     if (loss_sum)
         hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, partials, int(grid.x * grid.y), loss_sum);
+}
+
+// ---------------------------------------------------------------- no-allocation variant
+// The per-tile partials of pht_l1_ssim_loss_noalloc: a library-owned device array, one per GPU.
+// Host code never passes this name to a kernel (on the host it is only a stand-in for the
+// symbol): its device address is looked up once per device with hipGetSymbolAddress.
+static __device__ __align__(256) float scratch[PHT_NOALLOC_SSIM_TILES];
+
+constexpr int kMaxDevices = 64;
+
+float* l1_ssim_static_scratch(hipError_t* err) {
+    static float* ptr[kMaxDevices] = {};
+    static hipError_t status[kMaxDevices] = {};
+    static std::once_flag once[kMaxDevices];
+    int dev = -1;
+    *err = hipGetDevice(&dev);
+    if (*err != hipSuccess) return nullptr;
+    if (dev < 0 || dev >= kMaxDevices) {
+        *err = hipErrorInvalidDevice;
+        return nullptr;
+    }
+    // with `dev` current: each GPU holds its own copy of the array at its own address
+    std::call_once(once[dev], [&] {
+        void* p = nullptr;
+        status[dev] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        ptr[dev] = status[dev] == hipSuccess ? static_cast<float*>(p) : nullptr;
+    });
+    *err = status[dev];
+    return ptr[dev];
 }
 
 } // namespace pht

@@ -34,7 +34,8 @@ struct BlendArgs {
     uint32_t slot_cap;
     int64_t n_gauss;
     int want_refine; // backward: accumulate the refine statistic (only when a caller reads it)
-    const uint32_t* tile_order; // tile of each block (heaviest first), or null for the XCD-contiguous map
+    const uint32_t* tile_order; // tile of each work item (heaviest first), or null for the XCD-contiguous map
+    uint32_t* queue; // persistent work-queue counters [forward, backward] (zeroed per launch), or null: flat grid
 };
 
 // The statistics of a batched backward's views, from its per-(gaussian, view) refine statistic.
@@ -85,21 +86,21 @@ void launch_tile_offsets(const uint32_t* sorted_tiles, const uint32_t* visible_p
 // returns the buffer index (0/1) holding the result. hist holds 256 * kScanBlocks counts,
 // digit_total 256.
 int launch_radix_sort(uint32_t* keys[2], uint32_t* vals[2], uint32_t* aux[2], const uint32_t* n_ptr, uint32_t cap,
-                      int bits, uint32_t* hist, uint32_t* digit_total, hipStream_t st);
+                      int bits, uint32_t* hist, uint32_t* digit_total, hipStream_t st, bool narrow = false);
 
 // k_preprocess.hip
 void launch_upload_views(DevView* dst, const DevView* host_views, int n_views, hipStream_t st);
 void launch_clear(Counters* counters, uint2* ranges, uint32_t n_tiles, hipStream_t st);
 void launch_preprocess(const PreArgs& a, int64_t n_slots, hipStream_t st);
 void launch_emit_instances(const uint32_t* sorted_slots, const uint32_t* sorted_tiles, const uint32_t* offsets,
-                           const Counters* counters, uint32_t slot_cap, const SplatRec* rec,
-                           const DevView* views, int vp_log2, int tile_size, uint32_t inst_cap,
-                           uint32_t* keys_out, uint32_t* vals_out, hipStream_t st);
+                           const Counters* counters, uint32_t slot_cap, const SplatRec* rec, const DevView* views,
+                           int vp_log2, int tile_size, uint32_t inst_cap, uint32_t* keys_out, uint32_t* vals_out,
+                           bool narrow, hipStream_t st);
 void launch_ranges(const uint32_t* keys, const Counters* counters, uint32_t inst_cap, uint32_t n_tiles,
-                   uint2* ranges, hipStream_t st);
+                   uint2* ranges, bool narrow, hipStream_t st);
 void launch_debug_keys(const uint32_t* tile_keys, const uint32_t* slots, const uint4* meta,
                        const Counters* counters, uint32_t inst_cap, int vp_log2, uint64_t* keys_out,
-                       uint32_t* vals_out, hipStream_t st);
+                       uint32_t* vals_out, bool narrow, hipStream_t st);
 
 void launch_debug_splat_grads(const float* sgrad, const uint32_t* view_mask, int64_t n, int n_views, int vp_log2,
                               float* out, hipStream_t st);
@@ -108,8 +109,8 @@ void launch_debug_splat_grads(const float* sgrad, const uint32_t* view_mask, int
 // order: n_tiles entries; hist: 512 words of scratch.
 // The tiles heaviest first, by instance count.
 void launch_tile_order(const uint2* ranges, uint32_t n_tiles, uint32_t* order, uint32_t* hist, hipStream_t st);
-void launch_blend_forward(const BlendArgs& a, int n_tiles, int tile_size, int batch, hipStream_t st);
-void launch_blend_backward(const BlendArgs& a, int n_tiles, int tile_size, int batch, hipStream_t st);
+void launch_blend_forward(const BlendArgs& a, int n_tiles, int tile_size, int batch, int waves, hipStream_t st);
+void launch_blend_backward(const BlendArgs& a, int n_tiles, int tile_size, int batch, int waves, hipStream_t st);
 
 // k_gaussian.hip
 void launch_gaussian_backward(const GaussArgs& a, bool adam, hipStream_t st);
@@ -134,6 +135,9 @@ void launch_l1_views(const L1Views& v, float* loss_sum, float weight, hipStream_
 void launch_l1_loss(const float* color, const float* target, float* grad, float* loss_sum, int64_t pixels,
                     float weight, hipStream_t st);
 void launch_zero_f32(float* p, int64_t n, hipStream_t st);
+// Device address of the static per-tile partials array of the no-allocation L1+SSIM entry point on
+// the current device (looked up once per device), or null with *err set.
+float* l1_ssim_static_scratch(hipError_t* err);
 
 // k_densify.hip
 struct DensifyObsSet {

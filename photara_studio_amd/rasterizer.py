@@ -83,9 +83,13 @@ class RasterizeOptions:
     require_depth: bool = False
     record_backward_state: bool = True
     colors_precomp: Optional[torch.Tensor] = None  # [N,3]
-    # gfx950 launch shape: tile edge (16 = reference, or 8) and LDS batch depth.
+    # gfx950 launch shape: tile edge (16 = reference, or 8), LDS batch depth, waves (one (view,
+    # tile) item each) per blend workgroup, and whether the blends run as one persistent
+    # work-queue launch (resident workgroups pull items from a device counter) or a flat grid.
     tile_size: int = 16
     batch: int = 64
+    blend_waves: int = 1
+    blend_queue: bool = False
     fp64_conic: bool = True
 
 
@@ -329,6 +333,8 @@ def make_settings(model: GaussianModel, options: RasterizeOptions) -> _abi.pht_s
     s.compensate_2d = 1 if options.rasterization.compensate_2d else 0
     s.tile_size = int(options.tile_size)
     s.batch = int(options.batch)
+    s.blend_waves = int(options.blend_waves)
+    s.blend_queue = 1 if options.blend_queue else 0
     s.fp64_conic = 1 if options.fp64_conic else 0
     s.require_depth = 1 if options.require_depth else 0
     return s

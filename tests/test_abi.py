@@ -61,7 +61,7 @@ def test_struct_layouts_match_header(tmp_path):
 def test_workspace_bytes_host_only():
     from photara_studio_amd import _abi
     lib = _abi.load()
-    assert lib.pht_abi_version() == 3
+    assert lib.pht_abi_version() == 4
     cap = _abi.pht_capacity(10000, 1, 16, 256, 65536, 1 << 20)
     b1 = lib.pht_workspace_bytes(C.byref(cap))
     cap2 = _abi.pht_capacity(10000, 1, 16, 256, 65536, 1 << 21)

@@ -12,7 +12,7 @@ pytestmark = pytest.mark.gpu
 
 
 def test_autotune_sweep_and_cache(cuda, tmp_path):
-    from photara_studio_amd.autotune import CANDIDATES, autotune
+    from photara_studio_amd.autotune import BLOCK_SHAPES, CANDIDATES, autotune
     from photara_studio_amd.rasterizer import Rasterizer
     from photara_studio_amd.scenes import model_from, ring_cameras
     from photara_studio_amd.train import LearningRates
@@ -26,11 +26,14 @@ def test_autotune_sweep_and_cache(cuda, tmp_path):
     before = model.means.clone()
     print("[autotune] sweep", flush=True)
     best, rows = autotune(model, cams, targets, opts, LearningRates(), steps=1, warmup=1, cache=tmp_path / "t.json")
-    expected = [(t, b) for t, b in CANDIDATES if not (t == 8 and b > 128)]
-    assert [(x.tile_size, x.batch) for x in rows] == expected
+    expected = [(t, b, 1, False) for t, b in CANDIDATES if not (t == 8 and b > 128)]
+    first = min(rows[:len(expected)], key=lambda x: x.ms_per_step)
+    expected += [(first.tile_size, first.batch, w, q) for w, q in BLOCK_SHAPES if (w, q) != (1, False)]
+    assert [(x.tile_size, x.batch, x.blend_waves, x.blend_queue) for x in rows] == expected
     assert all(x.ms_per_step > 0 for x in rows)
-    assert (best.tile_size, best.batch) == min(((x.tile_size, x.batch, x.ms_per_step) for x in rows),
-                                               key=lambda t: t[2])[:2]
+    fastest = min(rows, key=lambda x: x.ms_per_step)
+    assert (best.tile_size, best.batch, best.blend_waves, best.blend_queue) == \
+        (fastest.tile_size, fastest.batch, fastest.blend_waves, fastest.blend_queue)
     assert torch.equal(model.means, before), "tuning must not train the caller's model"
     # the tuned shape renders the reference shape's image bit for bit
     a = r.forward_views(model, cams, opts)
@@ -40,5 +43,6 @@ def test_autotune_sweep_and_cache(cuda, tmp_path):
         np.testing.assert_array_equal(x.alpha.cpu().numpy(), y.alpha.cpu().numpy())
     # cached: same answer, no timing
     best2, rows2 = autotune(model, cams, targets, opts, LearningRates(), steps=1, warmup=1, cache=tmp_path / "t.json")
-    assert (best2.tile_size, best2.batch) == (best.tile_size, best.batch)
+    assert (best2.tile_size, best2.batch, best2.blend_waves, best2.blend_queue) == \
+        (best.tile_size, best.batch, best.blend_waves, best.blend_queue)
     assert [x.ms_per_step for x in rows2] == [x.ms_per_step for x in rows]

@@ -20,7 +20,12 @@ def test_autograd_fit_improves_psnr(cuda):
     targets = [res.color.clone() for res in Rasterizer().forward_views(model_from(gt, cuda), cams, opts)]
     params = {k: v.clone() for k, v in synthetic_torch(50_000, 1.0, 2, cuda).items()}
     print("[fit] 300 autograd steps", flush=True)
-    curve = fit(params, cams, targets, opts, steps=300, every=100, log=lambda m: print(m, flush=True))
+    torch.cuda.synchronize()
+    mem = []
+    curve = fit(params, cams, targets, opts, steps=300, every=100,
+                log=lambda m: (print(m, flush=True), mem.append(torch.cuda.memory_allocated())))
+    # one workspace serves every step: device memory does not grow with the step count
+    assert max(mem[1:]) - mem[1] <= 64 << 20, mem
     assert [s for s, _ in curve] == [0, 100, 200, 300]
     assert all(torch.isfinite(v).all() for v in params.values())
     assert curve[-1][1] > curve[0][1] + 2.0, curve

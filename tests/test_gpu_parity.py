@@ -153,6 +153,29 @@ def test_tile_and_batch_shapes(cuda, oracle, ts, batch):
     assert _rel(grads.sh.cpu().numpy(), og["sh"]) <= GRAD_TOL
 
 
+@pytest.mark.parametrize("ts,batch,waves,queue", [(16, 64, 1, True), (16, 64, 2, False), (16, 128, 4, True),
+                                                   (8, 64, 2, True), (16, 32, 4, False)])
+def test_blend_block_shapes(cuda, oracle, ts, batch, waves, queue):
+    """Blend workgroups of 1, 2 or 4 waves (each wave its own (view, tile) item) on the flat grid
+    or as one persistent work-queue launch: the same keys, contributor counts, final T and image
+    bits as the oracle, and the same gradients, for a batch of views of two resolutions; a second
+    backward of the same forward (the backward's queue restarts) returns the same gradients."""
+    from photara_studio_amd.scenes import look_at
+    arrays = small_scene(4000, 1.0, 12)
+    cams = [look_at((0.0, 0.0, 4.0), width=256, height=256), look_at((1.5, 0.5, 3.5), width=200, height=152)]
+    opts = training_options(tile_size=ts, batch=batch, blend_waves=waves, blend_queue=queue)
+    print(f"[parity] tile {ts} batch {batch} waves {waves} queue {queue}", flush=True)
+    r, model, res, lists = _gpu_render(arrays, cams, opts, cuda)
+    orc = oracle.OracleRun(oracle.Scene(arrays), cams, opts, 32)
+    _assert_forward_equal(res, lists, orc, f"ts={ts} batch={batch} waves={waves} queue={queue}")
+    gimg = _grad_images(res, 4)
+    og = orc.backward(gimg)
+    for _ in range(2):
+        grads = r.backward_views(model, res, [torch.from_numpy(g).to(cuda) for g in gimg])
+        for k in ("means", "log_scales", "opacity_logits", "sh"):
+            assert _rel(getattr(grads, k).cpu().numpy().reshape(og[k].shape), og[k]) <= GRAD_TOL, k
+
+
 @pytest.mark.parametrize("lp", ["bf16", "fp16"])
 def test_reduced_precision_storage(cuda, oracle, lp):
     from photara_studio_amd.scenes import workload_cameras
@@ -272,3 +295,19 @@ def test_second_backward_of_one_forward(cuda):
         b = getattr(second, k).cpu().numpy()
         assert np.abs(a[k] - b).max() <= 1e-6 * max(1.0, np.abs(a[k]).max()), k
         assert np.abs(a[k]).max() > 0.0, k
+
+
+def test_wide_tile_keys(cuda, oracle):
+    """More than 65,536 tiles in one batch (two 2048x2048 views of 8x8 tiles: 131,072 tiles):
+    the tile keys are 32-bit through emission, sort and ranges (16-bit below that), with the same
+    keys, ranges, contributor counts and image bits as the oracle."""
+    from photara_studio_amd.scenes import look_at
+    arrays = small_scene(3000, 1.0, 14)
+    cams = [look_at((0.0, 0.0, 4.0), width=2048, height=2048), look_at((2.0, 1.0, 3.0), width=2048, height=2048)]
+    opts = training_options(tile_size=8, batch=64)
+    print("[parity] 131072 tiles (32-bit tile keys)", flush=True)
+    r, model, res, lists = _gpu_render(arrays, cams, opts, cuda)
+    assert len(lists["ranges"]) // 2 == 131072
+    orc = oracle.OracleRun(oracle.Scene(arrays), cams, opts, 32)
+    _assert_forward_equal(res, lists, orc, "wide keys")
+    assert int((lists["keys"] >> np.uint64(32)).max()) >= 65536

@@ -209,3 +209,22 @@ def test_elongated_splat_gradients_against_fp64(oracle, fp64_conic):
         assert err <= 2e-5, err
     else:
         assert err > 1e-4, err
+
+
+def test_order_matched_fp64_oracle(oracle):
+    """Precision 66 keeps the float run's tile lists and blend order and decides the 1/255 cut-off
+    and termination in double. Where those decisions agree with the float run it is the
+    decision-matched FP64 oracle (65) bit for bit; elsewhere a flip moves a pixel by at most about
+    one alpha floor (1/255) times a colour."""
+    arrays, cams = _scene_and_cams(4000, 6)
+    opts = training_options()
+    scene = oracle.Scene(arrays)
+    m65 = oracle.OracleRun(scene, cams, opts, 65)
+    m66 = oracle.OracleRun(scene, cams, opts, 66)
+    p64 = oracle.OracleRun(scene, cams, opts, 64)
+    d = np.abs(m66.color_hp[0] - m65.color_hp[0]).max(axis=0)
+    assert (d == 0).mean() >= 0.999, (d > 0).sum()
+    assert d.max() <= 1.2 / 255.0
+    assert np.abs(m66.color_hp[0] - p64.color_hp[0]).max() <= 1.2 / 255.0
+    with pytest.raises(ValueError):
+        m66.backward([np.zeros_like(c) for c in m66.color])
