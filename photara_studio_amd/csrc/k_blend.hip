@@ -482,7 +482,10 @@ __device__ __forceinline__ void blend_bwd_item(const BlendArgs& a, int item, int
 }
 
 template <int TS, int BATCH, int WPB, bool REFINE>
-__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(5))) void k_blend_bwd(BlendArgs a, int n_items) {
+// Without the refine statistic the kernel fits 80 VGPRs (6 waves per SIMD) without spilling; with it,
+// 5 waves.
+__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(REFINE ? 5 : 6))) void k_blend_bwd(BlendArgs a,
+                                                                                                      int n_items) {
     if (a.counters->overflow) return;
     __shared__ float4 sA[WPB][BATCH];
     __shared__ float4 sB[WPB][BATCH];

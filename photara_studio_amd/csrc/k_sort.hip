@@ -221,13 +221,15 @@ __global__ __launch_bounds__(BT) void k_radix_downsweep(const K* keys_in, const 
                                                                   const uint32_t* hist, const uint32_t* digit_total) {
     constexpr int NW = BT / 64;
     constexpr int kSortTile = BT * R;
-    static_assert(kSortTile * int(sizeof(K)) >= BT / 64 * 256 * 8, "the digit-match tables live in s_key");
-    __shared__ __align__(8) K s_key[kSortTile];
+    static_assert(kSortTile * 4 >= BT / 64 * 256 * 8, "the digit-match tables live in s_key");
+    // keys are staged as 32-bit words in LDS whatever their width in memory: scattering 16-bit keys
+    // to random positions would put two lanes on one 32-bit bank word
+    __shared__ __align__(8) uint32_t s_key[kSortTile];
     __shared__ uint32_t s_val[kSortTile];
     __shared__ uint32_t s_aux[AUX ? kSortTile : 1];
     __shared__ uint32_t s_cnt[NW][256]; // per-wave running digit counts, then per-wave offsets
     __shared__ uint32_t s_start[256];   // digit start inside the reordered tile
-    __shared__ uint32_t s_gbase[256];   // global position of the tile's first key of each digit
+    __shared__ uint32_t s_shift[256];   // global position of the digit's run minus its start in the tile
     __shared__ uint32_t s_w[NW];
     const int t = threadIdx.x, lane = lane_id(), w = t >> 6;
     const bool digit_thread = t < 256; // thread t < 256 owns digit t in the per-tile bookkeeping
@@ -302,7 +304,7 @@ __global__ __launch_bounds__(BT) void k_radix_downsweep(const K* keys_in, const 
             const uint32_t inc = block_inclusive_scan<NW>(s, s_w, all);
             if (digit_thread) {
                 s_start[t] = inc - s;
-                s_gbase[t] = run;
+                s_shift[t] = run - (inc - s); // modulo 2^32: + a position inside the run = its destination
                 run += s;
             }
         }
@@ -313,7 +315,7 @@ __global__ __launch_bounds__(BT) void k_radix_downsweep(const K* keys_in, const 
             if (i < hi) {
                 const uint32_t d = (key[r] >> shift) & 255u;
                 const uint32_t p = s_start[d] + s_cnt[w][d] + rank[r];
-                s_key[p] = K(key[r]);
+                s_key[p] = key[r];
                 s_val[p] = val[r];
                 if constexpr (AUX) s_aux[p] = aux[r];
             }
@@ -322,9 +324,9 @@ __global__ __launch_bounds__(BT) void k_radix_downsweep(const K* keys_in, const 
         __syncthreads();
         const uint32_t tile_n = hi - base < uint32_t(kSortTile) ? hi - base : uint32_t(kSortTile);
         for (uint32_t j = t; j < tile_n; j += BT) {
-            const uint32_t k = uint32_t(s_key[j]);
+            const uint32_t k = s_key[j];
             const uint32_t d = (k >> shift) & 255u;
-            const uint32_t dst = s_gbase[d] + (j - s_start[d]);
+            const uint32_t dst = s_shift[d] + j;
             if (dst >= n) continue; // unreachable for consistent ranks; keeps every store in bounds
             keys_out[dst] = K(k);
             vals_out[dst] = s_val[j];
