@@ -49,6 +49,15 @@ struct SplatRec {
 // records are stored view-major: slot g * vp + v owns record v * n + g (n Gaussians), so a wave
 // of consecutive Gaussians reads one view's records contiguously.
 constexpr int kGradStride = 12;
+
+// Per-Gaussian view-mask word: bit v = visible in view v. With at most kMaskClampViews view slots
+// bits 8 + v, 16 + v and 24 + v also hold the SH clamp flags of view v (the slot's meta.w & 7), so
+// the per-Gaussian backward reads one word per Gaussian instead of one 16-byte record per view.
+constexpr int kMaskClampViewsLog2 = 3;
+constexpr int kMaskClampViews = 1 << kMaskClampViewsLog2;
+__device__ __forceinline__ uint32_t mask_clamp_flags(uint32_t mask, int v) {
+    return ((mask >> (8 + v)) & 1u) | (((mask >> (16 + v)) & 1u) << 1) | (((mask >> (24 + v)) & 1u) << 2);
+}
 __device__ __forceinline__ int64_t grad_offset(int64_t slot, int vp_log2, int64_t n) {
     return ((slot & ((int64_t(1) << vp_log2) - 1)) * n + (slot >> vp_log2)) * kGradStride;
 }

@@ -15,6 +15,13 @@
 
 namespace pht {
 
+// SH clamp flags of Gaussian i in view v: from its view-mask word when there are at most
+// kMaskClampViews view slots, else from the slot's meta record.
+__device__ __forceinline__ uint32_t slot_clamp(const GaussArgs& a, uint32_t mask, int64_t i, int v) {
+    if (a.vp_log2 <= kMaskClampViewsLog2) return mask_clamp_flags(mask, v);
+    return a.meta[(i << a.vp_log2) + v].w & 7u;
+}
+
 template <int LP> __device__ __forceinline__ float ld_lp(const float* f, const uint16_t* lp, int64_t i) {
     if constexpr (LP == PHT_LP_NONE) return f[i];
     else return lp_to_f32(lp[i], LP);
@@ -304,7 +311,7 @@ __global__ __launch_bounds__(256) void k_gaussian_bwd(GaussArgs a) {
                 } else {
                     float d[3], raw[3];
                     view_dir(mean, cam.position, d, raw);
-                    sh_backward_dir(DEG, raw, d, my_sh, a.meta[(i << a.vp_log2) + v].w & 7u, dcol, sg.mean);
+                    sh_backward_dir(DEG, raw, d, my_sh, slot_clamp(a, mask, i, v), dcol, sg.mean);
                 }
                 refine += g2.y;
             }
@@ -332,7 +339,7 @@ __global__ __launch_bounds__(256) void k_gaussian_bwd(GaussArgs a) {
                 const float dcol[3] = {g1.z, g1.w, g2.x};
                 float d[3], raw[3];
                 view_dir(mean, a.views[v].vi.cam.position, d, raw);
-                sh_backward_coef(DEG, d, a.meta[(i << a.vp_log2) + v].w & 7u, dcol, gsh);
+                sh_backward_coef(DEG, d, slot_clamp(a, mask, i, v), dcol, gsh);
             }
         }
     }

@@ -94,6 +94,7 @@ __global__ __launch_bounds__(256) void k_preprocess(PreArgs a, int64_t n_slots) 
     const bool in_range = t < n_slots;
     const bool valid = in_range && v < a.n_views;
     bool visible = false, fp64 = false;
+    uint32_t clamp_flags = 0;
     if (valid) {
         const DevView& dv = a.views[v];
         const float mean[3] = {a.g.means[3 * gi], a.g.means[3 * gi + 1], a.g.means[3 * gi + 2]};
@@ -135,6 +136,7 @@ __global__ __launch_bounds__(256) void k_preprocess(PreArgs a, int64_t n_slots) 
                 r.c = make_float4(rgb[2], ex, ey, qmax);
                 r.d = make_float4(p.conic[0], p.conic[1], p.conic[2], p.opacity);
                 a.meta[t] = make_uint4(0u, f2u(p.depth), touched, clamp | (fp64 ? 8u : 0u));
+                clamp_flags = clamp;
             }
         }
         if (dv.radii) dv.radii[gi] = visible ? radius : 0;
@@ -143,10 +145,21 @@ __global__ __launch_bounds__(256) void k_preprocess(PreArgs a, int64_t n_slots) 
     if (in_range) a.vis_flag[t] = visible ? 1u : 0u;
     const uint64_t bal = __ballot(visible);
     const int lane = lane_id();
+    // with at most 8 view slots the SH clamp flags of every view ride in the same word
+    uint64_t cb[3] = {0, 0, 0};
+    if (vp <= kMaskClampViews) {
+#pragma unroll
+        for (int b = 0; b < 3; ++b) cb[b] = __ballot(visible && ((clamp_flags >> b) & 1u));
+    }
     if (in_range && v == 0) {
         const int base = lane & ~(vp - 1);
-        const uint64_t grp = vp >= 64 ? bal : ((bal >> base) & ((uint64_t(1) << vp) - 1u));
-        a.view_mask[gi] = uint32_t(grp);
+        const uint64_t sel = vp >= 64 ? ~uint64_t(0) : ((uint64_t(1) << vp) - 1u);
+        uint32_t word = uint32_t((bal >> base) & sel);
+        if (vp <= kMaskClampViews) {
+#pragma unroll
+            for (int b = 0; b < 3; ++b) word |= uint32_t((cb[b] >> base) & sel) << (8 * (b + 1));
+        }
+        a.view_mask[gi] = word;
     }
     const uint64_t fb = __ballot(fp64);
     if (lane == 0 && fb) atomicAdd(&a.counters->fp64, uint32_t(__popcll(fb)));
@@ -272,41 +285,60 @@ __global__ __launch_bounds__(256) void k_emit(const uint32_t* sorted_slots, cons
 }
 
 // Four consecutive keys as one vector load (16 bytes of 32-bit keys, 8 bytes of 16-bit keys).
-__device__ __forceinline__ void load_key4(const uint32_t* keys, uint32_t q, uint32_t out[4]) {
-    const uint4 v = reinterpret_cast<const uint4*>(keys)[q];
-    out[0] = v.x; out[1] = v.y; out[2] = v.z; out[3] = v.w;
+// kRangeKeys consecutive keys per thread, loaded as 16-byte vectors (two for 16-bit keys, four for
+// 32-bit): with one key vector per thread the walk had too few bytes in flight per CU to stream at
+// HBM rate.
+constexpr int kRangeKeys = 16;
+__device__ __forceinline__ void load_keys(const uint32_t* keys, uint32_t g, uint32_t out[kRangeKeys]) {
+    const uint4* p = reinterpret_cast<const uint4*>(keys) + uint64_t(g) * (kRangeKeys / 4);
+#pragma unroll
+    for (int v = 0; v < kRangeKeys / 4; ++v) {
+        const uint4 x = p[v];
+        out[4 * v] = x.x; out[4 * v + 1] = x.y; out[4 * v + 2] = x.z; out[4 * v + 3] = x.w;
+    }
 }
-__device__ __forceinline__ void load_key4(const uint16_t* keys, uint32_t q, uint32_t out[4]) {
-    const uint2 v = reinterpret_cast<const uint2*>(keys)[q];
-    out[0] = v.x & 0xffffu; out[1] = v.x >> 16; out[2] = v.y & 0xffffu; out[3] = v.y >> 16;
+__device__ __forceinline__ void load_keys(const uint16_t* keys, uint32_t g, uint32_t out[kRangeKeys]) {
+    const uint4* p = reinterpret_cast<const uint4*>(keys) + uint64_t(g) * (kRangeKeys / 8);
+#pragma unroll
+    for (int v = 0; v < kRangeKeys / 8; ++v) {
+        const uint4 x = p[v];
+        const uint32_t w[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            out[8 * v + 2 * e] = w[e] & 0xffffu;
+            out[8 * v + 2 * e + 1] = w[e] >> 16;
+        }
+    }
 }
 
 // Per-tile [start, end) of the tile-sorted list (extract_tile_ranges, render_forward.cu:215-232).
-// Each thread takes four consecutive keys (one vector load) and their two outer neighbours.
+// Each thread takes kRangeKeys consecutive keys and their two outer neighbours.
 template <class K>
 __global__ __launch_bounds__(256) void k_ranges(const K* keys, const Counters* counters, uint32_t inst_cap,
                                                 uint32_t n_tiles, uint2* ranges) {
     if (counters->overflow) return; // the list is incomplete: leave every range empty
+    constexpr uint32_t E = kRangeKeys;
     const uint32_t ni = counters->instances;
     const uint32_t n = ni < inst_cap ? ni : inst_cap;
-    const uint32_t quads = (n + 3u) >> 2;
-    for (uint32_t q = blockIdx.x * 256u + threadIdx.x; q < quads; q += gridDim.x * 256u) {
-        const uint32_t i0 = q << 2;
-        uint32_t k[6]; // k[0] = keys[i0 - 1], k[1..4] = keys[i0 .. i0 + 3], k[5] = keys[i0 + 4]
-        if (i0 + 3u < n) {
-            load_key4(keys, q, k + 1);
+    const uint32_t groups = (n + E - 1u) / E;
+    for (uint32_t g = blockIdx.x * 256u + threadIdx.x; g < groups; g += gridDim.x * 256u) {
+        const uint32_t i0 = g * E;
+        uint32_t k[E + 2]; // k[0] = keys[i0 - 1], k[1..E] = keys[i0 .. i0 + E - 1], k[E + 1] = keys[i0 + E]
+        if (i0 + E - 1u < n) {
+            load_keys(keys, g, k + 1);
         } else {
-            for (int e = 0; e < 4; ++e) k[1 + e] = i0 + uint32_t(e) < n ? uint32_t(keys[i0 + uint32_t(e)]) : 0xffffffffu;
-        }
-        // The neighbours come from the adjacent lanes, which hold the adjacent quads (lane l + 1
-        // holds quad q + 1 whenever key i0 + 4 exists); only the wave's end lanes load them.
-        const int lane = int(threadIdx.x & 63u);
-        const uint32_t up = __shfl_up(k[4], 1, 64), down = __shfl_down(k[1], 1, 64);
-        k[0] = i0 == 0u ? 0xffffffffu : (lane > 0 ? up : uint32_t(keys[i0 - 1u]));
-        k[5] = i0 + 4u < n ? (lane < 63 ? down : uint32_t(keys[i0 + 4u])) : 0xffffffffu;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const uint32_t i = i0 + uint32_t(e);
+            for (uint32_t e = 0; e < E; ++e) k[1 + e] = i0 + e < n ? uint32_t(keys[i0 + e]) : 0xffffffffu;
+        }
+        // The neighbours come from the adjacent lanes, which hold the adjacent groups (lane l + 1
+        // holds group g + 1 whenever key i0 + E exists); only the wave's end lanes load them.
+        const int lane = int(threadIdx.x & 63u);
+        const uint32_t up = __shfl_up(k[E], 1, 64), down = __shfl_down(k[1], 1, 64);
+        k[0] = i0 == 0u ? 0xffffffffu : (lane > 0 ? up : uint32_t(keys[i0 - 1u]));
+        k[E + 1] = i0 + E < n ? (lane < 63 ? down : uint32_t(keys[i0 + E])) : 0xffffffffu;
+#pragma unroll
+        for (uint32_t e = 0; e < E; ++e) {
+            const uint32_t i = i0 + e;
             const uint32_t tile = k[1 + e];
             if (i >= n || tile >= n_tiles) continue;
             if (i == 0 || k[e] != tile) ranges[tile].x = i;
@@ -400,7 +432,9 @@ void launch_emit_instances(const uint32_t* sorted_slots, const uint32_t* sorted_
 
 void launch_ranges(const uint32_t* keys, const Counters* counters, uint32_t inst_cap, uint32_t n_tiles,
                    uint2* ranges, bool narrow, hipStream_t st) {
-    const dim3 grid(grid_stride_blocks(inst_cap));
+    // one group of kRangeKeys keys per thread up to 65535 blocks, grid-stride beyond
+    const uint64_t blocks = (uint64_t(inst_cap) + 256u * kRangeKeys - 1u) / (256u * kRangeKeys);
+    const dim3 grid(unsigned(blocks < 1 ? 1 : (blocks > 65535 ? 65535 : blocks)));
     if (narrow)
         hipLaunchKernelGGL(k_ranges<uint16_t>, grid, dim3(256), 0, st, reinterpret_cast<const uint16_t*>(keys),
                            counters, inst_cap, n_tiles, ranges);
