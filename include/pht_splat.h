@@ -202,6 +202,17 @@ void pht_workspace_destroy(pht_workspace* ws);
 int pht_forward(pht_workspace* ws, const pht_gaussians* g, const pht_camera* cams, int n_views,
                 const pht_settings* s, pht_view_outputs* out, int check_overflow, void* stream);
 
+/* ---- pht_forward followed by the L1 loss of every view (the trainer's compute_step_loss L1 leg,
+ * trainer.cpp:2201 / training_dispatch.cpp:9-23 into loss_kernel, cuda_ops.cu:886-898, full image,
+ * no mask), fused: the blend's epilogue writes
+ * grad_color[v] = weight * sign(color - target[v]) / (3 W H) from the colour it composites and
+ * adds sum |color - target[v]| * weight / (3 W H) to *loss_sum (device float, may be NULL).
+ * The same colours, gradients and arguments as pht_forward then pht_l1_loss per view; the loss
+ * sum is added in another order. No single reference function: it fuses the two calls. ---- */
+int pht_forward_l1(pht_workspace* ws, const pht_gaussians* g, const pht_camera* cams, int n_views,
+                   const pht_settings* s, pht_view_outputs* out, const float* const* targets,
+                   float* const* grad_color, float* loss_sum, float weight, int check_overflow, void* stream);
+
 /* ---- backward: replaces splat_drender::Rasterizer::backward (pipeline.cu:651-712) and
  * CudaRasterizer::backward (rasterizer.cu:495-743). Uses the state recorded by the last
  * pht_forward on this workspace (PHT_ESTATE if none). With adam != NULL the per-Gaussian

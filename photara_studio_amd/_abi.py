@@ -169,6 +169,9 @@ SIGNATURES = {
     "pht_workspace_bytes": (C.c_size_t, [C.POINTER(pht_capacity)]),
     "pht_workspace_create": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(pht_capacity), C.POINTER(C.c_void_p)]),
     "pht_workspace_destroy": (None, [C.c_void_p]),
+    "pht_forward_l1": (C.c_int, [C.c_void_p, C.POINTER(pht_gaussians), C.POINTER(pht_camera), C.c_int,
+                                 C.POINTER(pht_settings), C.POINTER(pht_view_outputs), C.POINTER(C.c_void_p),
+                                 C.POINTER(C.c_void_p), C.c_void_p, C.c_float, C.c_int, C.c_void_p]),
     "pht_forward": (C.c_int, [C.c_void_p, C.POINTER(pht_gaussians), C.POINTER(pht_camera), C.c_int,
                               C.POINTER(pht_settings), C.POINTER(pht_view_outputs), C.c_int, C.c_void_p]),
     "pht_backward": (C.c_int, [C.c_void_p, C.POINTER(pht_gaussians), C.POINTER(pht_view_grads),

@@ -64,7 +64,7 @@ struct pht_workspace {
     DevView* views = nullptr;
     Counters* counters = nullptr;
     SplatRec* rec = nullptr;
-    uint4* meta = nullptr;
+    uint2* meta = nullptr;
     uint32_t* vis_flag = nullptr;
     float* sgrad = nullptr;
     uint32_t* view_mask = nullptr;
@@ -117,7 +117,7 @@ static void carve_workspace(pht_workspace& w, char* base, const pht_capacity& ca
     w.views = c.take<DevView>(PHT_MAX_VIEWS);
     w.counters = c.take<Counters>(1);
     w.rec = c.take<SplatRec>(slots);
-    w.meta = c.take<uint4>(slots);
+    w.meta = c.take<uint2>(slots);
     w.vis_flag = c.take<uint32_t>(slots);
     w.sgrad = c.take<float>(slots * kGradStride);
     w.view_mask = c.take<uint32_t>(size_t(cap.gaussians));
@@ -212,8 +212,18 @@ void pht_workspace_destroy(pht_workspace* ws) {
     delete ws;
 }
 
-int pht_forward(pht_workspace* ws, const pht_gaussians* g, const pht_camera* cams, int n_views,
-                const pht_settings* s, pht_view_outputs* out, int check_overflow, void* stream) {
+// The fused L1 loss of pht_forward_l1: per-view targets and gradient outputs, the loss sum (or
+// null) and the photometric weight.
+struct FusedL1 {
+    const float* const* targets;
+    float* const* grads;
+    float* loss_sum;
+    float weight;
+};
+
+static int forward_impl(pht_workspace* ws, const pht_gaussians* g, const pht_camera* cams, int n_views,
+                        const pht_settings* s, pht_view_outputs* out, int check_overflow, void* stream,
+                        const FusedL1* l1) {
     if (!ws || !g || !cams || !s || !out) return fail(PHT_EINVAL, "null argument");
     ws->fwd_valid = false;
     if (s->require_depth)
@@ -261,6 +271,14 @@ int pht_forward(pht_workspace* ws, const pht_gaussians* g, const pht_camera* cam
         d.alpha = out[v].alpha;
         d.radii = out[v].radii;
         d.visibility = out[v].visibility;
+        if (l1) {
+            if (!l1->targets[v] || !l1->grads[v]) return fail(PHT_EINVAL, "the fused L1 loss needs a target and a gradient per view");
+            d.l1_target = l1->targets[v];
+            d.l1_grad = l1->grads[v];
+            // the separate loss kernel's scale, in the same float operations
+            const float inv_p = 1.0f / float(int64_t(c.width) * c.height);
+            d.l1_scale = (l1->weight * inv_p) / 3.0f;
+        }
         tiles += d.vi.tile_count;
         pixels += int64_t(c.width) * c.height;
     }
@@ -322,6 +340,7 @@ int pht_forward(pht_workspace* ws, const pht_gaussians* g, const pht_camera* cam
     ba.n_gauss = g->count;
     ba.want_refine = 0;
     ba.queue = s->blend_queue ? ws->counters->queue : nullptr;
+    ba.loss_sum = l1 ? l1->loss_sum : nullptr;
     ws->mark(4, 0, st);
     launch_blend_forward(ba, int(tiles), ts, s->batch, s->blend_waves, st);
     ws->mark(4, 1, st);
@@ -348,6 +367,19 @@ int pht_forward(pht_workspace* ws, const pht_gaussians* g, const pht_camera* cam
         }
     }
     return PHT_OK;
+}
+
+int pht_forward(pht_workspace* ws, const pht_gaussians* g, const pht_camera* cams, int n_views,
+                const pht_settings* s, pht_view_outputs* out, int check_overflow, void* stream) {
+    return forward_impl(ws, g, cams, n_views, s, out, check_overflow, stream, nullptr);
+}
+
+int pht_forward_l1(pht_workspace* ws, const pht_gaussians* g, const pht_camera* cams, int n_views,
+                   const pht_settings* s, pht_view_outputs* out, const float* const* targets,
+                   float* const* grad_color, float* loss_sum, float weight, int check_overflow, void* stream) {
+    if (!targets || !grad_color) return fail(PHT_EINVAL, "targets and grad_color are required");
+    const FusedL1 l1{targets, grad_color, loss_sum, weight};
+    return forward_impl(ws, g, cams, n_views, s, out, check_overflow, stream, &l1);
 }
 
 int pht_backward(pht_workspace* ws, const pht_gaussians* g, const pht_view_grads* dl, pht_model_grads* grads,
@@ -796,29 +828,21 @@ int pht_step_graph_create_densify(pht_workspace* ws, const pht_gaussians* g, con
     int rc = check_hip(hipStreamBeginCapture(cap_stream, hipStreamCaptureModeThreadLocal), "begin capture");
     if (rc == PHT_OK) {
         if (loss_sum) launch_zero_f32(loss_sum, 1, cap_stream);
-        rc = pht_forward(ws, g, cams, n_views, s, out, 0, cap_stream);
+        // the L1 loss is fused into the forward's blend; L1+SSIM runs as its own kernel per view
+        if (lc.kind == PHT_LOSS_L1) {
+            const FusedL1 l1{targets, grad_color_bufs, loss_sum, lc.photometric_weight};
+            rc = forward_impl(ws, g, cams, n_views, s, out, 0, cap_stream, &l1);
+        } else {
+            rc = pht_forward(ws, g, cams, n_views, s, out, 0, cap_stream);
+        }
         std::vector<pht_view_grads> dl(static_cast<size_t>(n_views));
-        L1Views l1{};
         for (int v = 0; rc == PHT_OK && v < n_views; ++v) {
-            const int64_t P = int64_t(cams[v].width) * cams[v].height;
-            if (lc.kind == PHT_LOSS_L1_SSIM) {
+            if (lc.kind == PHT_LOSS_L1_SSIM)
                 rc = pht_l1_ssim_loss(out[v].color, targets[v], nullptr, grad_color_bufs[v], 0, loss_sum,
                                       loss_sum ? lc.ssim_scratch[v] : nullptr, cams[v].width, cams[v].height,
                                       lc.ssim_weight, lc.photometric_weight, cap_stream);
-            } else {
-                if (!out[v].color || !targets[v] || !grad_color_bufs[v] || P <= 0) rc = fail(PHT_EINVAL, "bad L1 view");
-                l1.color[l1.n] = out[v].color;
-                l1.target[l1.n] = targets[v];
-                l1.grad[l1.n] = grad_color_bufs[v];
-                l1.pixels[l1.n] = P;
-                ++l1.n;
-            }
             dl[size_t(v)].grad_color = grad_color_bufs[v];
             dl[size_t(v)].grad_alpha = nullptr;
-        }
-        if (rc == PHT_OK && l1.n > 0) {
-            launch_l1_views(l1, loss_sum, lc.photometric_weight, cap_stream);
-            rc = check_hip(hipGetLastError(), "l1 launch");
         }
         if (rc == PHT_OK) rc = pht_backward_densify(ws, g, dl.data(), nullptr, adam, densify, cap_stream);
         hipGraph_t graph = nullptr;

@@ -17,7 +17,20 @@ struct DevView {
     float* visibility;     // [N] or null
     const float* grad_color; // [3,H,W] (backward)
     const float* grad_alpha; // [H,W] or null (backward)
+    // forward with the L1 loss fused into the blend's epilogue (pht_forward_l1), else null
+    const float* l1_target; // [3,H,W]
+    float* l1_grad;         // [3,H,W] dL/dC out
+    float l1_scale;         // weight / (3 P), the per-pixel gradient magnitude
+    float l1_pad;
 };
+
+// One L1 term (loss_kernel, photara/src/splat/cuda_ops.cu:886-898): returns |c - t| and sets
+// g = scale * sign(c - t).
+__device__ __forceinline__ float l1_term(float c, float t, float scale, float& g) {
+    const float d = c - t;
+    g = scale * float((d > 0.0f) - (d < 0.0f));
+    return fabsf(d);
+}
 
 // Device counters, cleared at the start of every forward.
 struct Counters {
@@ -43,15 +56,19 @@ struct Counters {
 struct SplatRec {
     float4 a, d, b, c;
 };
-// Slot metadata (16 bytes, read in slot order): (0, depth_key, tiles, flags),
-// flags: bits 0-2 SH clamp, bit 3 fp64 conic.
-// Splat gradient (12 floats): dmean.xy, dconic.xyz, dopacity, dcolor.rgb, refine, 2 spare. The
-// records are stored view-major: slot g * vp + v owns record v * n + g (n Gaussians), so a wave
-// of consecutive Gaussians reads one view's records contiguously.
-constexpr int kGradStride = 12;
+// Splat gradient (10 floats, read as five 8-byte pairs): dmean.xy, dconic.xyz, dopacity,
+// dcolor.rgb, refine. The records are stored view-major: slot g * vp + v owns record v * n + g
+// (n Gaussians), so a wave of consecutive Gaussians reads one view's records contiguously.
+constexpr int kGradStride = 10;
+
+// Per-slot meta record (uint2) written by the preprocess for visible slots: x = depth bits (the
+// depth sort key), y = touched tile count | SH clamp flags << kMetaClampShift. A view has fewer
+// than 2^26 tiles (at most 8191 per axis), so the count never reaches the flag bits.
+constexpr uint32_t kMetaClampShift = 29;
+constexpr uint32_t kMetaCountMask = (1u << kMetaClampShift) - 1u;
 
 // Per-Gaussian view-mask word: bit v = visible in view v. With at most kMaskClampViews view slots
-// bits 8 + v, 16 + v and 24 + v also hold the SH clamp flags of view v (the slot's meta.w & 7), so
+// bits 8 + v, 16 + v and 24 + v also hold the SH clamp flags of view v (also in the slot's meta record), so
 // the per-Gaussian backward reads one word per Gaussian instead of one 16-byte record per view.
 constexpr int kMaskClampViewsLog2 = 3;
 constexpr int kMaskClampViews = 1 << kMaskClampViewsLog2;

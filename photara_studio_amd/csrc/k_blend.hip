@@ -274,19 +274,37 @@ __device__ __forceinline__ void blend_fwd_item(const BlendArgs& a, int item, int
         }
     }
     const int64_t P = int64_t(W) * H;
+    // With the fused L1 loss the epilogue also writes dL/dC = scale * sign(C - target) from the
+    // colour it has just composited (the same float value the separate loss kernel would read
+    // back) and adds the tile's sum of |C - target| to the loss.
+    const bool l1 = dv.l1_target != nullptr; // uniform over the item
+    float l1_part = 0.0f;
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
         const int px = int(pxf[k]), py = int(pyf[k]);
         if (px < W && py < H) {
             const int64_t pid = int64_t(py) * W + px;
             const float Tk = fabsf(T[k]);
-            dv.color[pid] = C0[k] + Tk * dv.vi.cam.background[0];
-            dv.color[P + pid] = C1[k] + Tk * dv.vi.cam.background[1];
-            dv.color[2 * P + pid] = C2[k] + Tk * dv.vi.cam.background[2];
+            const float c[3] = {C0[k] + Tk * dv.vi.cam.background[0], C1[k] + Tk * dv.vi.cam.background[1],
+                                C2[k] + Tk * dv.vi.cam.background[2]};
+#pragma unroll
+            for (int ch = 0; ch < 3; ++ch) dv.color[ch * P + pid] = c[ch];
             dv.alpha[pid] = 1.0f - Tk;
             a.final_T[dv.vi.pixel_base + pid] = Tk;
             a.n_contrib[dv.vi.pixel_base + pid] = last[k];
+            if (l1) {
+#pragma unroll
+                for (int ch = 0; ch < 3; ++ch) {
+                    float g;
+                    l1_part += l1_term(c[ch], dv.l1_target[ch * P + pid], dv.l1_scale, g);
+                    dv.l1_grad[ch * P + pid] = g;
+                }
+            }
         }
+    }
+    if (l1 && a.loss_sum) {
+        l1_part = wave_sum_to_last(l1_part);
+        if (lane == 63) atomic_add_f32(a.loss_sum, l1_part * dv.l1_scale);
     }
 }
 
@@ -298,13 +316,14 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(8, 8))
                                                                                                    int n_items) {
     {
         // Clear the splat-gradient records the backward accumulates into (n_views * N records of
-        // 12 floats, view-major): each block streams zeros over its share. The stores retire in
-        // the background of this VALU-bound kernel instead of taking HBM time of their own.
-        const int64_t n4 = int64_t(a.n_views) * a.n_gauss * (kGradStride / 4);
+        // kGradStride floats, view-major): each block streams zeros over its share. The stores
+        // retire in the background of this VALU-bound kernel instead of taking HBM time of their own.
+        static_assert(kGradStride % 2 == 0, "records are cleared as float pairs");
+        const int64_t n2 = int64_t(a.n_views) * a.n_gauss * (kGradStride / 2);
         const int64_t nb = gridDim.x, b = blockIdx.x;
-        const int64_t lo = n4 * b / nb, hi = n4 * (b + 1) / nb;
-        float4* g4 = reinterpret_cast<float4*>(a.sgrad);
-        for (int64_t j = lo + threadIdx.x; j < hi; j += 64 * WPB) g4[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+        const int64_t lo = n2 * b / nb, hi = n2 * (b + 1) / nb;
+        float2* g2 = reinterpret_cast<float2*>(a.sgrad);
+        for (int64_t j = lo + threadIdx.x; j < hi; j += 64 * WPB) g2[j] = make_float2(0.f, 0.f);
     }
     if (a.counters->overflow) return; // incomplete instance list: the host re-runs with more capacity
     __shared__ float4 sA[WPB][BATCH];

@@ -19,7 +19,7 @@ namespace pht {
 // kMaskClampViews view slots, else from the slot's meta record.
 __device__ __forceinline__ uint32_t slot_clamp(const GaussArgs& a, uint32_t mask, int64_t i, int v) {
     if (a.vp_log2 <= kMaskClampViewsLog2) return mask_clamp_flags(mask, v);
-    return a.meta[(i << a.vp_log2) + v].w & 7u;
+    return a.meta[(i << a.vp_log2) + v].y >> kMetaClampShift;
 }
 
 template <int LP> __device__ __forceinline__ float ld_lp(const float* f, const uint16_t* lp, int64_t i) {
@@ -299,13 +299,13 @@ __global__ __launch_bounds__(256) void k_gaussian_bwd(GaussArgs a) {
             const Activated<float> act = activate<float>(ls, q, logit, hf, filt);
             for (int v = 0; v < a.n_views; ++v) {
                 if (!((mask >> v) & 1u)) continue;
-                const float4* gp = reinterpret_cast<const float4*>(a.sgrad + (v * n + i) * kGradStride);
-                const float4 g0 = gp[0], g1 = gp[1], g2 = gp[2];
-                const float dconic[4] = {g0.z, g0.w, g1.x, g1.y};
+                const float2* gp = reinterpret_cast<const float2*>(a.sgrad + (v * n + i) * kGradStride);
+                const float2 g0 = gp[0], g1 = gp[1], g2 = gp[2], g3 = gp[3], g4 = gp[4];
+                const float dconic[4] = {g1.x, g1.y, g2.x, g2.y};
                 const float dm2[2] = {g0.x, g0.y};
                 const pht_camera& cam = a.views[v].vi.cam;
                 splat_backward_pinhole(mean, act, cam, a.s, dconic, dm2, sg);
-                const float dcol[3] = {g1.z, g1.w, g2.x};
+                const float dcol[3] = {g3.x, g3.y, g4.x};
                 if (precomp) {
                     gcol[0] += dcol[0]; gcol[1] += dcol[1]; gcol[2] += dcol[2];
                 } else {
@@ -313,7 +313,7 @@ __global__ __launch_bounds__(256) void k_gaussian_bwd(GaussArgs a) {
                     view_dir(mean, cam.position, d, raw);
                     sh_backward_dir(DEG, raw, d, my_sh, slot_clamp(a, mask, i, v), dcol, sg.mean);
                 }
-                refine += g2.y;
+                refine += g4.y;
             }
             chain_raw(act, hf, filt, sg, gls, gq, glogit);
         }
@@ -334,9 +334,9 @@ __global__ __launch_bounds__(256) void k_gaussian_bwd(GaussArgs a) {
         if (!precomp) {
             for (int v = 0; v < a.n_views; ++v) {
                 if (!((mask >> v) & 1u)) continue;
-                const float4* gp = reinterpret_cast<const float4*>(a.sgrad + (v * n + i) * kGradStride);
-                const float4 g1 = gp[1], g2 = gp[2];
-                const float dcol[3] = {g1.z, g1.w, g2.x};
+                const float2* gp = reinterpret_cast<const float2*>(a.sgrad + (v * n + i) * kGradStride);
+                const float2 g3 = gp[3], g4 = gp[4];
+                const float dcol[3] = {g3.x, g3.y, g4.x};
                 float d[3], raw[3];
                 view_dir(mean, a.views[v].vi.cam.position, d, raw);
                 sh_backward_coef(DEG, d, slot_clamp(a, mask, i, v), dcol, gsh);
@@ -539,11 +539,6 @@ void launch_adam_standalone(float* p, const float* g, float* m, float* v, int64_
 // loss_kernel L1 part (photara/src/splat/cuda_ops.cu:886-898), no mask, full image. A bounded
 // grid strides over the pixels (four at a time when the planes are 16-byte aligned), so the
 // optional loss sum takes one atomic per block.
-__device__ __forceinline__ float l1_term(float c, float t, float scale, float& g) {
-    const float d = c - t;
-    g = scale * float((d > 0.0f) - (d < 0.0f));
-    return fabsf(d);
-}
 
 // One view's L1 term over a bounded grid of `nblocks` blocks (this block is `block`).
 template <bool VEC>
@@ -587,13 +582,6 @@ __device__ __forceinline__ void l1_view(const float* color, const float* target,
     }
 }
 
-// All views of a step in one launch: grid (blocks per view, views).
-template <bool VEC>
-__global__ __launch_bounds__(256) void k_l1_views(L1Views v, float* loss_sum, float weight) {
-    const int k = blockIdx.y;
-    l1_view<VEC>(v.color[k], v.target[k], v.grad[k], loss_sum, v.pixels[k], weight, blockIdx.x, gridDim.x);
-}
-
 template <bool VEC>
 __global__ __launch_bounds__(256) void k_l1(const float* color, const float* target, float* grad, float* loss_sum,
                                             int64_t pixels, float weight) {
@@ -610,25 +598,6 @@ void launch_l1_loss(const float* color, const float* target, float* grad, float*
     if (blocks > 512) blocks = 512;
     if (vec) hipLaunchKernelGGL(k_l1<true>, dim3(unsigned(blocks)), dim3(256), 0, st, color, target, grad, loss_sum, pixels, weight);
     else hipLaunchKernelGGL(k_l1<false>, dim3(unsigned(blocks)), dim3(256), 0, st, color, target, grad, loss_sum, pixels, weight);
-}
-
-void launch_l1_views(const L1Views& v, float* loss_sum, float weight, hipStream_t st) {
-    if (v.n <= 0) return;
-    bool vec = true;
-    int64_t most = 0;
-    for (int k = 0; k < v.n; ++k) {
-        vec = vec && (v.pixels[k] & 3) == 0 &&
-              ((reinterpret_cast<uintptr_t>(v.color[k]) | reinterpret_cast<uintptr_t>(v.target[k]) |
-                reinterpret_cast<uintptr_t>(v.grad[k])) & 15u) == 0;
-        most = v.pixels[k] > most ? v.pixels[k] : most;
-    }
-    if (most <= 0) return;
-    const int64_t items = vec ? most >> 2 : most;
-    int64_t blocks = (items + 255) / 256;
-    if (blocks > 128) blocks = 128;
-    const dim3 grid(unsigned(blocks), unsigned(v.n));
-    if (vec) hipLaunchKernelGGL(k_l1_views<true>, grid, dim3(256), 0, st, v, loss_sum, weight);
-    else hipLaunchKernelGGL(k_l1_views<false>, grid, dim3(256), 0, st, v, loss_sum, weight);
 }
 
 __global__ void k_zero_f32(float* p, int64_t n) {

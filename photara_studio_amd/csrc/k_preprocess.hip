@@ -135,7 +135,7 @@ __global__ __launch_bounds__(256) void k_preprocess(PreArgs a, int64_t n_slots) 
                 r.b = make_float4(c2, p.opacity, rgb[0], rgb[1]);
                 r.c = make_float4(rgb[2], ex, ey, qmax);
                 r.d = make_float4(p.conic[0], p.conic[1], p.conic[2], p.opacity);
-                a.meta[t] = make_uint4(0u, f2u(p.depth), touched, clamp | (fp64 ? 8u : 0u));
+                a.meta[t] = make_uint2(f2u(p.depth), touched | (clamp << kMetaClampShift));
                 clamp_flags = clamp;
             }
         }
@@ -349,13 +349,13 @@ __global__ __launch_bounds__(256) void k_ranges(const K* keys, const Counters* c
 
 template <class K>
 __global__ __launch_bounds__(256) void k_debug_keys(const K* tile_keys, const uint32_t* slots,
-                                                    const uint4* meta, const Counters* counters, uint32_t inst_cap,
+                                                    const uint2* meta, const Counters* counters, uint32_t inst_cap,
                                                     int vp_log2, uint64_t* keys_out, uint32_t* vals_out) {
     const uint32_t ni = counters->instances;
     const uint32_t n = ni < inst_cap ? ni : inst_cap;
     for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
         const uint32_t slot = slots[i];
-        if (keys_out) keys_out[i] = (uint64_t(tile_keys[i]) << 32) | meta[slot].y;
+        if (keys_out) keys_out[i] = (uint64_t(tile_keys[i]) << 32) | meta[slot].x;
         if (vals_out) vals_out[i] = slot >> vp_log2;
     }
 }
@@ -442,7 +442,7 @@ void launch_ranges(const uint32_t* keys, const Counters* counters, uint32_t inst
         hipLaunchKernelGGL(k_ranges<uint32_t>, grid, dim3(256), 0, st, keys, counters, inst_cap, n_tiles, ranges);
 }
 
-void launch_debug_keys(const uint32_t* tile_keys, const uint32_t* slots, const uint4* meta,
+void launch_debug_keys(const uint32_t* tile_keys, const uint32_t* slots, const uint2* meta,
                        const Counters* counters, uint32_t inst_cap, int vp_log2, uint64_t* keys_out,
                        uint32_t* vals_out, bool narrow, hipStream_t st) {
     const dim3 grid(grid_stride_blocks(inst_cap));

@@ -46,32 +46,55 @@ __device__ __forceinline__ uint32_t block_inclusive_scan(uint32_t v, uint32_t* s
 }
 
 // ---------------------------------------------------------------- value sources / sinks
+// kBinary: every value is 0 or 1, so a wave's inclusive scan is a ballot and a popcount.
 struct SrcFlags { // visibility flags of the dense slot array
+    static constexpr bool kBinary = true;
     const uint32_t* flags;
     __device__ uint32_t operator()(uint32_t i) const { return flags[i]; }
 };
 struct SrcTiles { // tile counts, already permuted into depth order by the depth sort
+    static constexpr bool kBinary = false;
     const uint32_t* counts;
     __device__ uint32_t operator()(uint32_t i) const { return counts[i]; }
 };
+
+template <class Src> __device__ __forceinline__ uint32_t src_wave_inclusive_scan(uint32_t v) {
+    if constexpr (Src::kBinary) return uint32_t(__popcll(__ballot(v != 0u) & __lanemask_lt())) + v;
+    else return wave_inclusive_scan(v);
+}
+// A sink's load(i, v) fetches what it stores for element i before the scan needs it, so that a
+// block's loads of every row are in flight together; store(i, excl, v, payload) writes it.
 struct SinkCompact { // visible slot -> (depth key, slot, tile count) at its compact position
-    const uint4* meta;
+    const uint2* meta;
     uint32_t* keys;
     uint32_t* vals;
     uint32_t* tiles;
-    __device__ void operator()(uint32_t i, uint32_t excl, uint32_t v) const {
+    using Payload = uint2;
+    __device__ Payload load(uint32_t i, uint32_t v) const {
+        if (!v) return make_uint2(0u, 0u);
+        const uint2 c = meta[i];
+        return make_uint2(c.x, c.y & kMetaCountMask);
+    }
+    __device__ void store(uint32_t i, uint32_t excl, uint32_t v, Payload c) const {
         if (v) {
-            const uint4 c = meta[i];
-            keys[excl] = c.y;
+            keys[excl] = c.x;
             vals[excl] = i;
-            tiles[excl] = c.z;
+            tiles[excl] = c.y;
         }
     }
 };
 struct SinkOffsets {
     uint32_t* out;
-    __device__ void operator()(uint32_t i, uint32_t excl, uint32_t) const { out[i] = excl; }
+    struct Payload {};
+    __device__ Payload load(uint32_t, uint32_t) const { return {}; }
+    __device__ void store(uint32_t i, uint32_t excl, uint32_t, Payload) const { out[i] = excl; }
 };
+
+// A block walks its chunk kScanRows rows of kScanThreads consecutive elements at a time (row k of
+// an iteration = base + 256 k .. base + 256 k + 255): every row's loads and stores are coalesced,
+// all rows' loads are issued before any value is used, and the block synchronises once per
+// kScanRows * 256 elements instead of once per row.
+constexpr int kScanRows = 8;
 
 template <class Src>
 __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(Src src, const uint32_t* n_ptr, uint32_t n_const,
@@ -81,7 +104,16 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(Src src, const uin
     uint32_t lo, hi;
     chunk_of(n, nb, blockIdx.x, lo, hi);
     uint32_t acc = 0;
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += kScanThreads) acc += src(i);
+    for (uint32_t base = lo; base < hi; base += kScanRows * kScanThreads) {
+        uint32_t v[kScanRows];
+#pragma unroll
+        for (int k = 0; k < kScanRows; ++k) {
+            const uint32_t i = base + uint32_t(k) * kScanThreads + threadIdx.x;
+            v[k] = i < hi ? src(i) : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < kScanRows; ++k) acc += v[k];
+    }
     uint32_t total;
     block_inclusive_scan(acc, s_w, total);
     if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
@@ -114,18 +146,45 @@ template <class Src, class Sink>
 __global__ __launch_bounds__(kScanThreads) void k_scan_write(Src src, Sink sink, const uint32_t* n_ptr,
                                                              uint32_t n_const, uint32_t cap, int nb,
                                                              const uint32_t* block_offsets) {
-    __shared__ uint32_t s_w[4];
+    constexpr int NW = kScanThreads / 64;
+    // per-row wave totals, double-buffered: iteration j + 2 rewrites a buffer only after every
+    // thread has passed iteration j + 1's barrier, so after its reads of iteration j
+    __shared__ uint32_t s_w[2][kScanRows][NW];
     const uint32_t n = load_count(n_ptr, n_const, cap);
     uint32_t lo, hi;
     chunk_of(n, nb, blockIdx.x, lo, hi);
+    const int lane = lane_id(), w = int(threadIdx.x) >> 6;
     uint32_t running = block_offsets[blockIdx.x];
-    for (uint32_t base = lo; base < hi; base += kScanThreads) {
-        const uint32_t i = base + threadIdx.x;
-        const uint32_t v = i < hi ? src(i) : 0u;
-        uint32_t total;
-        const uint32_t inc = block_inclusive_scan(v, s_w, total);
-        if (i < hi) sink(i, running + inc - v, v);
-        running += total;
+    int buf = 0;
+    for (uint32_t base = lo; base < hi; base += kScanRows * kScanThreads, buf ^= 1) {
+        uint32_t v[kScanRows], inc[kScanRows];
+        typename Sink::Payload pay[kScanRows];
+#pragma unroll
+        for (int k = 0; k < kScanRows; ++k) {
+            const uint32_t i = base + uint32_t(k) * kScanThreads + threadIdx.x;
+            v[k] = i < hi ? src(i) : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < kScanRows; ++k) {
+            const uint32_t i = base + uint32_t(k) * kScanThreads + threadIdx.x;
+            pay[k] = sink.load(i < hi ? i : 0u, i < hi ? v[k] : 0u);
+            inc[k] = src_wave_inclusive_scan<Src>(v[k]);
+            if (lane == 63) s_w[buf][k][w] = inc[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kScanRows; ++k) {
+            uint32_t before_w = 0, row_total = 0;
+#pragma unroll
+            for (int q = 0; q < NW; ++q) {
+                const uint32_t x = s_w[buf][k][q];
+                before_w += q < w ? x : 0u;
+                row_total += x;
+            }
+            const uint32_t i = base + uint32_t(k) * kScanThreads + threadIdx.x;
+            if (i < hi) sink.store(i, running + before_w + inc[k] - v[k], v[k], pay[k]);
+            running += row_total;
+        }
     }
 }
 
@@ -140,8 +199,8 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_write(Src src, Sink sink,
 // array in the write-out, and half the per-tile bookkeeping of 8192-key tiles); the depth sort,
 // with a third array in LDS, 1024 x 8.
 
-// 1024 threads per chunk (16 waves; four LDS histograms, each shared by four waves): the
-// chunks are the downsweep's, one per CU, so the block's width sets how many loads a CU has in
+// 1024 threads per chunk (16 waves sharing the block's LDS sub-histograms): the chunks are the
+// downsweep's, one per CU, so the block's width sets how many loads a CU has in
 // flight (four times as many as 256-thread blocks; the six passes of the bench workload went
 // from 456 to 409 us, still short of HBM rate, with the LDS atomics the other limit).
 constexpr int kUpThreads = 1024;
@@ -152,10 +211,16 @@ __global__ __launch_bounds__(kUpThreads) void k_radix_upsweep(const K* keys, con
                                                               uint32_t n_const, uint32_t cap, int shift, int nb,
                                                               uint32_t* hist) {
     constexpr uint32_t KV = 16u / sizeof(K);
-    __shared__ uint32_t s_h[4][256];
-    const int t = threadIdx.x, w = t >> 8;
-    s_h[w][t & 255] = 0;
+    // kUpSubs sub-histograms, lane l counting into sub-histogram l % kUpSubs. Their rows are 257
+    // words apart, so the lanes of one instruction that hold the same digit hit kUpSubs different
+    // banks: a pass whose keys share one or two digit values (the top byte of depth keys) no
+    // longer serialises 32 lanes on one LDS word.
+    constexpr int kUpSubs = 8, kUpPitch = 257;
+    __shared__ uint32_t s_h[kUpSubs * kUpPitch];
+    const int t = threadIdx.x;
+    for (int k = t; k < kUpSubs * kUpPitch; k += kUpThreads) s_h[k] = 0;
     __syncthreads();
+    uint32_t* h = s_h + (t & (kUpSubs - 1)) * kUpPitch;
     const uint32_t n = load_count(n_ptr, n_const, cap);
     uint32_t lo, hi;
     chunk_of(n, nb, blockIdx.x, lo, hi);
@@ -166,10 +231,10 @@ __global__ __launch_bounds__(kUpThreads) void k_radix_upsweep(const K* keys, con
     constexpr uint32_t kStep = KV * kUpThreads;
     auto count_word = [&](uint32_t x) {
         if constexpr (sizeof(K) == 4) {
-            atomicAdd(&s_h[w][(x >> shift) & 255u], 1u);
+            atomicAdd(&h[(x >> shift) & 255u], 1u);
         } else {
-            atomicAdd(&s_h[w][((x & 0xffffu) >> shift) & 255u], 1u);
-            atomicAdd(&s_h[w][((x >> 16) >> shift) & 255u], 1u);
+            atomicAdd(&h[((x & 0xffffu) >> shift) & 255u], 1u);
+            atomicAdd(&h[((x >> 16) >> shift) & 255u], 1u);
         }
     };
     auto count_vec = [&](const uint4& q) {
@@ -187,9 +252,14 @@ __global__ __launch_bounds__(kUpThreads) void k_radix_upsweep(const K* keys, con
         for (int u = 0; u < 4; ++u) count_vec(q[u]);
     }
     for (; i < hiv; i += kStep) count_vec(k4[i / KV]);
-    for (uint32_t i = hiv + t; i < hi; i += kUpThreads) atomicAdd(&s_h[w][(uint32_t(keys[i]) >> shift) & 255u], 1u);
+    for (uint32_t i = hiv + t; i < hi; i += kUpThreads) atomicAdd(&h[(uint32_t(keys[i]) >> shift) & 255u], 1u);
     __syncthreads();
-    if (t < 256) hist[size_t(t) * nb + blockIdx.x] = (s_h[0][t] + s_h[1][t]) + (s_h[2][t] + s_h[3][t]);
+    if (t < 256) {
+        uint32_t c = 0;
+#pragma unroll
+        for (int q = 0; q < kUpSubs; ++q) c += s_h[q * kUpPitch + t];
+        hist[size_t(t) * nb + blockIdx.x] = c;
+    }
 }
 
 // Block d (256 blocks of 1024 threads): exclusive scan of digit d's counts over the nb <= 1024
@@ -246,7 +316,9 @@ __global__ __launch_bounds__(BT) void k_radix_downsweep(const K* keys_in, const 
     chunk_of(n, nb, blockIdx.x, lo, hi);
     const uint64_t lt = __lanemask_lt();
     // The next tile's keys are loaded into the same registers as soon as the current tile has
-    // been reordered into LDS, so its global loads overlap the current tile's write-out.
+    // been reordered into LDS, so its global loads overlap the current tile's write-out. (Two
+    // register sets, with the next tile's loads issued before the current tile is ranked, made
+    // both sorts slower on the bench step: the block is bound by its LDS work, not by HBM idling.)
     uint32_t key[R], val[R], aux[R], rank[R];
     auto load_tile = [&](uint32_t b) {
         const uint32_t wb = b + uint32_t(w) * (kSortTile / NW);
@@ -344,7 +416,7 @@ static int blocks_for(uint32_t capacity) {
 
 int scan_blocks_for(uint32_t capacity) { return blocks_for(capacity); }
 
-void launch_compact_visible(const uint32_t* flags, const uint4* meta, uint32_t n_slots, uint32_t* keys,
+void launch_compact_visible(const uint32_t* flags, const uint2* meta, uint32_t n_slots, uint32_t* keys,
                             uint32_t* vals, uint32_t* tiles, uint32_t* block_scratch, uint32_t* visible_out,
                             hipStream_t st) {
     const int nb = blocks_for(n_slots);
@@ -374,7 +446,9 @@ void launch_tile_offsets(const uint32_t* sorted_tiles, const uint32_t* visible_p
 
 int launch_radix_sort(uint32_t* keys[2], uint32_t* vals[2], uint32_t* aux[2], const uint32_t* n_ptr, uint32_t cap,
                       int bits, uint32_t* hist, uint32_t* digit_total, hipStream_t st, bool narrow) {
-    // 1024-thread blocks, one resident per CU: at most 256 chunks
+    // 1024-thread blocks, one resident per CU: at most 256 chunks. (Measured on the bench step:
+    // two 512-thread blocks per CU with 8192- or 4096-key tiles, or 256-thread blocks, made both
+    // sorts slower; the per-tile bookkeeping favours the largest tile that fits LDS.)
     const int nb = blocks_for(cap) < 256 ? blocks_for(cap) : 256;
     int cur = 0;
     for (int shift = 0; shift < bits; shift += 8) {

@@ -13,7 +13,7 @@ struct PreArgs {
     int n_views;
     int vp_log2;
     SplatRec* rec;
-    uint4* meta;
+    uint2* meta;
     uint32_t* vis_flag;
     uint32_t* view_mask;
     Counters* counters;
@@ -36,6 +36,7 @@ struct BlendArgs {
     int want_refine; // backward: accumulate the refine statistic (only when a caller reads it)
     const uint32_t* tile_order; // tile of each work item (heaviest first), or null for the XCD-contiguous map
     uint32_t* queue; // persistent work-queue counters [forward, backward] (zeroed per launch), or null: flat grid
+    float* loss_sum; // forward with the fused L1 loss: += sum over views of scale * sum |C - target|, or null
 };
 
 // The statistics of a batched backward's views, from its per-(gaussian, view) refine statistic.
@@ -56,7 +57,7 @@ struct GaussArgs {
     const DevView* views;
     int n_views;
     int vp_log2;
-    const uint4* meta;
+    const uint2* meta;
     const float* sgrad;
     const uint32_t* view_mask;
     const Counters* counters;
@@ -74,7 +75,7 @@ void launch_sh_adam_quant(float* param, const float* grad, uint16_t* first, uint
 // k_sort.hip
 int scan_blocks_for(uint32_t capacity);
 // Compacts the visible slots (slot order) into (depth key, slot, tile count) lists.
-void launch_compact_visible(const uint32_t* flags, const uint4* meta, uint32_t n_slots, uint32_t* keys,
+void launch_compact_visible(const uint32_t* flags, const uint2* meta, uint32_t n_slots, uint32_t* keys,
                             uint32_t* vals, uint32_t* tiles, uint32_t* block_scratch, uint32_t* visible_out,
                             hipStream_t st);
 // Exclusive scan of the depth-ordered tile counts into instance offsets; stores the instance
@@ -98,7 +99,7 @@ void launch_emit_instances(const uint32_t* sorted_slots, const uint32_t* sorted_
                            bool narrow, hipStream_t st);
 void launch_ranges(const uint32_t* keys, const Counters* counters, uint32_t inst_cap, uint32_t n_tiles,
                    uint2* ranges, bool narrow, hipStream_t st);
-void launch_debug_keys(const uint32_t* tile_keys, const uint32_t* slots, const uint4* meta,
+void launch_debug_keys(const uint32_t* tile_keys, const uint32_t* slots, const uint2* meta,
                        const Counters* counters, uint32_t inst_cap, int vp_log2, uint64_t* keys_out,
                        uint32_t* vals_out, bool narrow, hipStream_t st);
 
@@ -123,15 +124,6 @@ void launch_l1_ssim(const float* color, const float* target, const float* mask, 
                     float* loss_sum, float* scratch, int width, int height, float ssim_weight,
                     float photometric_weight, hipStream_t st);
 
-// The L1 term of several views in one launch (the captured step).
-struct L1Views {
-    const float* color[PHT_MAX_VIEWS];
-    const float* target[PHT_MAX_VIEWS];
-    float* grad[PHT_MAX_VIEWS];
-    int64_t pixels[PHT_MAX_VIEWS];
-    int n;
-};
-void launch_l1_views(const L1Views& v, float* loss_sum, float weight, hipStream_t st);
 void launch_l1_loss(const float* color, const float* target, float* grad, float* loss_sum, int64_t pixels,
                     float weight, hipStream_t st);
 void launch_zero_f32(float* p, int64_t n, hipStream_t st);

@@ -107,6 +107,8 @@ class TrainStep:
         self.dl = (_abi.pht_view_grads * nv)()
         for k in range(nv):
             self.dl[k].grad_color = self.grad_colors[k].data_ptr()
+        self._target_ptrs = (C.c_void_p * nv)(*[t.data_ptr() for t in self.targets])
+        self._grad_ptrs = (C.c_void_p * nv)(*[g.data_ptr() for g in self.grad_colors])
         self.loss_sum = torch.zeros((1,), device=dev)
         self.loss_kind = loss
         self.ssim_weight, self.photometric_weight = float(ssim_weight), float(photometric_weight)
@@ -260,35 +262,36 @@ class TrainStep:
             self.sh_adam.correction1, self.sh_adam.correction2 = c1, c2
         stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
         nv = len(self.cameras)
-        check(self.lib.pht_forward(self.ws.handle, C.byref(self.gauss), self.cams, nv, C.byref(self.settings),
-                                   self.outs, 0, stream))
         self.loss_sum.zero_()
-        for k in range(nv):
-            color, target, grad = (C.c_void_p(self.colors[k].data_ptr()), C.c_void_p(self.targets[k].data_ptr()),
-                                   C.c_void_p(self.grad_colors[k].data_ptr()))
-            if self.loss_kind == "l1_ssim":
+        if self.loss_kind == "l1":
+            # the L1 loss and its gradient come out of the forward's blend (pht_forward_l1)
+            check(self.lib.pht_forward_l1(self.ws.handle, C.byref(self.gauss), self.cams, nv, C.byref(self.settings),
+                                          self.outs, self._target_ptrs, self._grad_ptrs,
+                                          C.c_void_p(self.loss_sum.data_ptr()), self.photometric_weight, 0, stream))
+        else:
+            check(self.lib.pht_forward(self.ws.handle, C.byref(self.gauss), self.cams, nv, C.byref(self.settings),
+                                       self.outs, 0, stream))
+            for k in range(nv):
                 cam = self.cameras[k]
-                check(self.lib.pht_l1_ssim_loss(color, target, None, grad, 0, C.c_void_p(self.loss_sum.data_ptr()),
+                check(self.lib.pht_l1_ssim_loss(C.c_void_p(self.colors[k].data_ptr()),
+                                                C.c_void_p(self.targets[k].data_ptr()), None,
+                                                C.c_void_p(self.grad_colors[k].data_ptr()), 0,
+                                                C.c_void_p(self.loss_sum.data_ptr()),
                                                 C.c_void_p(self.ssim_scratch[k].data_ptr()), int(cam.width),
                                                 int(cam.height), self.ssim_weight, self.photometric_weight, stream))
-            else:
-                check(self.lib.pht_l1_loss(color, target, grad, C.c_void_p(self.loss_sum.data_ptr()),
-                                           self.colors[k][0].numel(), self.photometric_weight, stream))
         adam = make_adam(self.model, self.adam, self.sh_adam)
         self._backward(adam, stream)
         self._track("eager")
 
     def capture(self) -> None:
-        """Capture forward + L1 + backward + Adam as one hipGraph (hyper-parameters on device)."""
+        """Capture forward + loss + backward + Adam as one hipGraph (hyper-parameters on device)."""
         nv = len(self.cameras)
         self._adam_graph = make_adam(self.model, self.adam, self.sh_adam)
         self._adam_graph.device_hparams = self.hparams.data_ptr()
-        tgt = (C.c_void_p * nv)(*[t.data_ptr() for t in self.targets])
-        grd = (C.c_void_p * nv)(*[g.data_ptr() for g in self.grad_colors])
         handle = C.c_void_p()
         stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
         check(self.lib.pht_step_graph_create_densify(self.ws.handle, C.byref(self.gauss), self.cams, nv,
-                                                     C.byref(self.settings), self.outs, tgt, grd,
+                                                     C.byref(self.settings), self.outs, self._target_ptrs, self._grad_ptrs,
                                                      C.c_void_p(self.loss_sum.data_ptr()), C.byref(self.loss_cfg),
                                                      C.byref(self._adam_graph),
                                                      C.byref(self._dstep) if self.densify else None, stream,

@@ -97,6 +97,43 @@ def test_fused_quantised_sh_step_matches_oracle(cuda, oracle):
             assert np.abs(u_gpu - u_orc)[strong].max() <= 2.0 ** -10 * np.abs(u_orc)[strong].max()
 
 
+def test_forward_l1_matches_forward_then_l1_loss(cuda, oracle):
+    """pht_forward_l1 (the L1 loss fused into the blend's epilogue) against pht_forward followed by
+    pht_l1_loss per view, through the C ABI: the same colours and the same gradient bits; the
+    loss sums agree to float rounding (they are added in another order)."""
+    import ctypes as C
+    from photara_studio_amd import _abi
+    _, cams, _, _, _, step = _setup(cuda, oracle)
+    lib, nv, weight = step.lib, len(step.cameras), 0.8
+    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    loss_fused = torch.zeros((1,), device=cuda)
+    print("[train] pht_forward_l1 vs pht_forward + pht_l1_loss", flush=True)
+    assert lib.pht_forward_l1(step.ws.handle, C.byref(step.gauss), step.cams, nv, C.byref(step.settings), step.outs,
+                              step._target_ptrs, step._grad_ptrs, C.c_void_p(loss_fused.data_ptr()), weight, 1,
+                              stream) == _abi.PHT_OK
+    torch.cuda.synchronize()
+    colors_fused = [c.clone() for c in step.colors]
+    grads_fused = [g.clone() for g in step.grad_colors]
+    assert lib.pht_forward(step.ws.handle, C.byref(step.gauss), step.cams, nv, C.byref(step.settings), step.outs, 1,
+                           stream) == _abi.PHT_OK
+    loss_sep = torch.zeros((1,), device=cuda)
+    grads_sep = [torch.full_like(g, 7.0) for g in step.grad_colors]
+    for k in range(nv):
+        assert lib.pht_l1_loss(C.c_void_p(step.colors[k].data_ptr()), C.c_void_p(step.targets[k].data_ptr()),
+                               C.c_void_p(grads_sep[k].data_ptr()), C.c_void_p(loss_sep.data_ptr()),
+                               step.colors[k][0].numel(), weight, stream) == _abi.PHT_OK
+    torch.cuda.synchronize()
+    for k in range(nv):
+        assert torch.equal(step.colors[k], colors_fused[k]), k
+        assert torch.equal(grads_sep[k], grads_fused[k]), k
+    assert float(loss_sep) > 0.0
+    assert abs(float(loss_fused) - float(loss_sep)) <= 1e-6 * float(loss_sep), (float(loss_fused), float(loss_sep))
+    # a missing target is refused before any launch
+    bad = (C.c_void_p * nv)(*([None] * nv))
+    assert lib.pht_forward_l1(step.ws.handle, C.byref(step.gauss), step.cams, nv, C.byref(step.settings), step.outs,
+                              bad, step._grad_ptrs, None, weight, 0, stream) == _abi.PHT_EINVAL
+
+
 @pytest.mark.parametrize("sh_optimizer", ["fp32", "quant"])
 def test_graph_replays_match_eager_steps(cuda, oracle, sh_optimizer):
     _, _, _, _, _, eager = _setup(cuda, oracle, sh_optimizer=sh_optimizer)
