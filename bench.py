@@ -45,11 +45,11 @@ def algorithmic_bytes(n, views, visible, instances, pixels, tiles, sh_bytes, lp,
         "compact+depth_sort": n * views * 4 + visible * 12 + sort_passes_depth * 24 * visible,
         "tile_offsets+emit": visible * 8 + visible * 32 + 12 * instances,
         "tile_sort+ranges": sort_passes_tile * 24 * instances + 8 * instances + 8 * tiles,
-        "blend_forward": 40 * instances + 20 * pixels,
-        "loss": 36 * pixels,
-        "blend_backward": 40 * instances + 24 * pixels + 44 * visible,
+        # the L1 loss is fused into the forward's epilogue: + target read and gradient write
+        "blend_forward": 40 * instances + 20 * pixels + 24 * pixels,
+        "blend_backward": 40 * instances + 24 * pixels + 40 * visible,
         "gaussian_backward+adam": n * (59 * 4 * 2 + (11 * 8 + 48 * 3 + 16 if sh_quant else 59 * 8) * 2
-                                       + (52 * 2 if lp else 0)) + 44 * visible,
+                                       + (52 * 2 if lp else 0) + 4) + 40 * visible,
     }
     return out
 
