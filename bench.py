@@ -347,20 +347,27 @@ def main():
     targets = make_targets(n, cams, opts, dev, seed=1)
     arrays = synthetic_torch(n, 3.0, 0, dev)
 
-    # launch shape of the batched step (the plain step below takes the same shape)
+    # launch shapes: each step kind gets the fastest shape for its own workload, so the plain step
+    # is compared at its best (a one-view step favours other block shapes than an 8-view one)
     model = to_low_precision(model_from(arrays, dev), torch.bfloat16)
-    bopts = opts
-    tune_rows = None
+    bopts = popts = opts
+    tune_rows = plain_tune_rows = None
     if args.autotune:
         from photara_studio_amd.autotune import autotune
         bopts, tune_rows = autotune(model, cams, targets, opts, LearningRates(), steps=3, warmup=1,
                                     log=lambda m: print(m, flush=True))
-        print(f"[bench] autotuned: tile {bopts.tile_size}, batch {bopts.batch}, waves {bopts.blend_waves}, "
+        print(f"[bench] autotuned (batched): tile {bopts.tile_size}, batch {bopts.batch}, waves {bopts.blend_waves}, "
               f"{'queue' if bopts.blend_queue else 'grid'}", flush=True)
+        torch.cuda.empty_cache()
+        popts, plain_tune_rows = autotune(model_from(arrays, dev), [cams[0]], [targets[0]], opts, LearningRates(),
+                                          steps=6, warmup=2, log=lambda m: print(m.replace("[autotune]", "[autotune plain]"),
+                                                                                 flush=True))
+        print(f"[bench] autotuned (plain): tile {popts.tile_size}, batch {popts.batch}, waves {popts.blend_waves}, "
+              f"{'queue' if popts.blend_queue else 'grid'}", flush=True)
         torch.cuda.empty_cache()
 
     # plain FP32 step: one view per step, eager launches
-    plain = TrainStep(model_from(arrays, dev), [cams[0]], [targets[0]], bopts, LearningRates(), sh_optimizer="fp32")
+    plain = TrainStep(model_from(arrays, dev), [cams[0]], [targets[0]], popts, LearningRates(), sh_optimizer="fp32")
     plain.ensure_capacity()
     plain_steps = args.plain_steps or 4 * args.steps
     plain_s = timed(plain.step_eager, plain_steps, args.warmup)
@@ -368,6 +375,14 @@ def main():
     if pc.overflow:
         raise RuntimeError("plain step overflowed its instance capacity")
     print(f"[bench] plain fp32: {plain_s * 1e3:.3f} ms/step ({pc.instances} instances)", flush=True)
+    plain.ws.set_profiling(True)
+    plain_acc = {}
+    for _ in range(3):
+        plain.step_eager()
+        for k, v in plain.ws.stage_times().items():
+            plain_acc[k] = plain_acc.get(k, 0.0) + v / 3
+    plain.ws.set_profiling(False)
+    print(f"[bench] plain stages ms: " + ", ".join(f"{k}={v:.3f}" for k, v in plain_acc.items()), flush=True)
     del plain
     torch.cuda.empty_cache()
 
@@ -464,7 +479,12 @@ def main():
         "autotune": [[r.tile_size, r.batch, r.blend_waves, "queue" if r.blend_queue else "grid",
                       round(r.ms_per_step, 3)] for r in tune_rows] if tune_rows else None,
         "plain_fp32": {"ms_per_step": round(plain_s * 1e3, 4), "steps_per_s": round(1.0 / plain_s, 3),
-                       "instances": int(pc.instances)},
+                       "instances": int(pc.instances), "tile": int(popts.tile_size), "batch": int(popts.batch),
+                       "blend_waves": int(popts.blend_waves),
+                       "blend_launch": "persistent work queue" if popts.blend_queue else "flat grid",
+                       "stage_ms": {k: round(v, 4) for k, v in plain_acc.items()},
+                       "autotune": [[r.tile_size, r.batch, r.blend_waves, "queue" if r.blend_queue else "grid",
+                                     round(r.ms_per_step, 3)] for r in plain_tune_rows] if plain_tune_rows else None},
         "speedup_vs_plain_fp32": round((nv / step_s) * plain_s, 3),
         f"batched_{other_sh}_sh": None if alt_s is None else {
             "ms_per_step": round(alt_s * 1e3, 4), "view_steps_per_s": round(nv / alt_s, 3),

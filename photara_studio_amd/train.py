@@ -54,7 +54,8 @@ class TrainStep:
                  instance_capacity: Optional[int] = None, loss: str = "l1", ssim_weight: float = 0.2,
                  photometric_weight: float = 1.0, sh_optimizer: str = "quant",
                  sh_regularization_weight: float = 0.0, densify: bool = False,
-                 view_indices: Optional[Sequence[int]] = None, densify_screen_threshold: float = 0.5):
+                 view_indices: Optional[Sequence[int]] = None, densify_screen_threshold: float = 0.5,
+                 view_outputs: Optional[bool] = None):
         """loss: "l1" (loss_kernel's L1, cuda_ops.cu:886-898) or "l1_ssim" (the trainer's default
         (1 - w) L1 + w (1 - SSIM), fused_ssim.cu:473-560, w = ssim_weight).
         sh_optimizer: "quant", the trainer's SH Adam with fp16 update + 8-bit log-second state
@@ -65,7 +66,11 @@ class TrainStep:
         densify: accumulate the ADC-IGS densification statistics inside the step (every view of the
         batch observed as the trainer observes its render, trainer.cpp:2441-2449, with view ids
         ``view_indices`` and oversize threshold ``densify_screen_threshold``); ``refine()`` then
-        runs a refinement and re-binds the step to the new model."""
+        runs a refinement and re-binds the step to the new model.
+        view_outputs: also write each view's per-Gaussian radii and contribution visibility (the
+        render's radii / visibility outputs) into ``self.radii`` / ``self.visibility``; by default
+        only when ``densify`` is set, whose statistics read them (otherwise nothing in the step
+        does, and the forward skips those writes)."""
         if loss not in ("l1", "l1_ssim"):
             raise ValueError(f"unknown loss {loss!r}")
         if sh_optimizer not in ("quant", "fp32"):
@@ -97,13 +102,19 @@ class TrainStep:
         self.ws = Workspace(dev, n, nv, ts, self.tiles, self.pixels, self.capacity)
         self.colors = [torch.empty((3, c.height, c.width), device=dev) for c in self.cameras]
         self.alphas = [torch.empty((c.height, c.width), device=dev) for c in self.cameras]
-        self.radii = [torch.empty((n,), dtype=torch.int32, device=dev) for _ in self.cameras]
-        self.visibility = [torch.empty((n,), device=dev) for _ in self.cameras]
+        self.view_outputs = bool(densify) if view_outputs is None else bool(view_outputs)
+        if densify and not self.view_outputs:
+            raise ValueError("the densification statistics need the per-view radii and visibility")
+        self.radii = self.visibility = None
+        if self.view_outputs:
+            self.radii = [torch.empty((n,), dtype=torch.int32, device=dev) for _ in self.cameras]
+            self.visibility = [torch.empty((n,), device=dev) for _ in self.cameras]
         self.grad_colors = [torch.empty_like(c) for c in self.colors]
         self.outs = (_abi.pht_view_outputs * nv)()
         for k in range(nv):
             self.outs[k].color, self.outs[k].alpha = self.colors[k].data_ptr(), self.alphas[k].data_ptr()
-            self.outs[k].radii, self.outs[k].visibility = self.radii[k].data_ptr(), self.visibility[k].data_ptr()
+            if self.view_outputs:
+                self.outs[k].radii, self.outs[k].visibility = self.radii[k].data_ptr(), self.visibility[k].data_ptr()
         self.dl = (_abi.pht_view_grads * nv)()
         for k in range(nv):
             self.dl[k].grad_color = self.grad_colors[k].data_ptr()
