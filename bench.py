@@ -311,6 +311,8 @@ def main():
     p.add_argument("--batch", type=int, default=64)
     p.add_argument("--waves", type=int, default=1, help="blend waves per workgroup (1, 2, 4) without --autotune")
     p.add_argument("--queue", action="store_true", help="persistent work-queue blends without --autotune")
+    p.add_argument("--view-groups", type=int, default=1,
+                   help="split the batched step's views into this many groups on their own streams")
     p.add_argument("--no-autotune", dest="autotune", action="store_false",
                    help="skip the launch-shape sweep (tile {16,8} x batch {32..256}, then waves {1,2,4} x "
                         "grid/queue) that picks the batched step's shape")
@@ -387,7 +389,8 @@ def main():
     torch.cuda.empty_cache()
 
     # batched BF16 graph step: all views per step
-    batched = TrainStep(model, cams, targets, bopts, LearningRates(), sh_optimizer=args.sh_optimizer)
+    batched = TrainStep(model, cams, targets, bopts, LearningRates(), sh_optimizer=args.sh_optimizer,
+                        view_groups=args.view_groups)
     batched.ensure_capacity()
     batched.capture()
     step_s = timed(batched.step_graph, args.steps, args.warmup)
@@ -395,15 +398,21 @@ def main():
     assert not bc.overflow
     print(f"[bench] batched bf16 graph: {step_s * 1e3:.3f} ms/step, {nv / step_s:.2f} view-steps/s", flush=True)
 
-    # per-stage kernel times (hipEvents on the launch stream) over a few eager steps
-    batched.ws.set_profiling(True)
+    # per-stage kernel times (hipEvents on the launch streams) over a few eager steps; with view
+    # groups, each stage's times are summed over the groups (whose stages overlap in time)
+    for ws in batched.wss:
+        ws.set_profiling(True)
     acc = {}
     prof_steps = 3
     for _ in range(prof_steps):
         batched.step_eager()
-        for k, v in batched.ws.stage_times().items():
-            acc[k] = acc.get(k, 0.0) + v / prof_steps
-    batched.ws.set_profiling(False)
+        torch.cuda.synchronize()
+        for ws in batched.wss:
+            for k, v in ws.stage_times().items():
+                if v >= 0:
+                    acc[k] = acc.get(k, 0.0) + v / prof_steps
+    for ws in batched.wss:
+        ws.set_profiling(False)
     pixels = sum(c.width * c.height for c in cams)
     bt = int(bopts.tile_size)
     tiles = sum(((c.width + bt - 1) // bt) * ((c.height + bt - 1) // bt) for c in cams)

@@ -491,6 +491,26 @@ int pht_step_graph_create_densify(pht_workspace* ws, const pht_gaussians* g, con
                                   const pht_densify_step* densify, void* stream,
                                   pht_step_graph** out_graph);
 
+/* ---- view groups: a step's views split into n_groups (1..4) groups, each forwarded (pht_forward
+ * or pht_forward_l1) on its own workspace and stream, so that one group's sorts can run beside
+ * another group's blends. pht_backward_groups runs each group's blend backward on that group's
+ * stream (group_streams[k], which must already hold the group's forward), makes `stream` wait for
+ * all of them, then runs ONE per-Gaussian backward (+ fused Adam) over every group's views in
+ * group order on `stream`. dl lists the views of group 0, then group 1, ... No densification
+ * statistics (use one workspace for those). No reference counterpart: the reference renders one
+ * view per call (trainer.cpp:2024-2379); this is the batched step's scheduling. ---- */
+int pht_backward_groups(pht_workspace* const* ws, int n_groups, const pht_gaussians* g,
+                        const pht_view_grads* dl, pht_model_grads* grads, const pht_adam* adam,
+                        void* const* group_streams, void* stream);
+/* The whole grouped step as one hipGraph: the capture forks into one stream per group (group k
+ * takes the next group_views[k] cameras, outputs, targets and gradient buffers), each group runs
+ * its forward and loss, and the graph joins into pht_backward_groups. */
+int pht_step_graph_create_groups(pht_workspace* const* ws, int n_groups, const int* group_views,
+                                 const pht_gaussians* g, const pht_camera* cams, int n_views,
+                                 const pht_settings* s, pht_view_outputs* out, const float* const* targets,
+                                 float* const* grad_color_bufs, float* loss_sum, const pht_loss_config* loss,
+                                 const pht_adam* adam, void* stream, pht_step_graph** out_graph);
+
 /* Host only: count draws of a fresh std::normal_distribution<float>(0, 1) over the std::mt19937
  * whose state (624 words, position in [0, 624]) the caller holds and which is advanced in place:
  * the split offsets of grow_igs_random_gpu (densification_adc_plus.cpp:113-120). */

@@ -227,6 +227,15 @@ SIGNATURES = {
                                                 C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_void_p,
                                                 C.POINTER(pht_loss_config), C.POINTER(pht_adam),
                                                 C.POINTER(pht_densify_step), C.c_void_p, C.POINTER(C.c_void_p)]),
+    "pht_backward_groups": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(pht_gaussians),
+                                      C.POINTER(pht_view_grads), C.POINTER(pht_model_grads), C.POINTER(pht_adam),
+                                      C.POINTER(C.c_void_p), C.c_void_p]),
+    "pht_step_graph_create_groups": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_int),
+                                               C.POINTER(pht_gaussians), C.POINTER(pht_camera), C.c_int,
+                                               C.POINTER(pht_settings), C.POINTER(pht_view_outputs),
+                                               C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_void_p,
+                                               C.POINTER(pht_loss_config), C.POINTER(pht_adam), C.c_void_p,
+                                               C.POINTER(C.c_void_p)]),
     "pht_host_normal_floats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.c_void_p, C.c_int64]),
     "pht_adc_decay": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int64, C.c_float,
                                 C.c_float, C.c_void_p]),

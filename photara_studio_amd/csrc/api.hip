@@ -83,6 +83,7 @@ struct pht_workspace {
     uint32_t* hist = nullptr;
     uint32_t* digit_total = nullptr;
     Counters* host_counters = nullptr; // pinned
+    hipEvent_t join_ev = nullptr;      // end of this workspace's blend backward in a grouped backward
     // state of the last forward
     bool fwd_valid = false;
     // a backward has accumulated into sgrad since the forward cleared it: the next backward of
@@ -206,6 +207,7 @@ int pht_workspace_create(void* device_bytes, size_t nbytes, const pht_capacity* 
 void pht_workspace_destroy(pht_workspace* ws) {
     if (!ws) return;
     if (ws->host_counters) (void)hipHostFree(ws->host_counters);
+    if (ws->join_ev) (void)hipEventDestroy(ws->join_ev);
     for (auto& pair : ws->ev)
         for (hipEvent_t& e : pair)
             if (e) (void)hipEventDestroy(e);
@@ -387,9 +389,10 @@ int pht_backward(pht_workspace* ws, const pht_gaussians* g, const pht_view_grads
     return pht_backward_densify(ws, g, dl, grads, adam, nullptr, stream);
 }
 
-int pht_backward_densify(pht_workspace* ws, const pht_gaussians* g, const pht_view_grads* dl, pht_model_grads* grads,
-                         const pht_adam* adam, const pht_densify_step* densify, void* stream) {
-    if (!ws || !g || !dl) return fail(PHT_EINVAL, "null argument");
+// Argument checks shared by the backward entry points: a completed forward of this model on the
+// workspace, and either gradient outputs or a complete fused-Adam description.
+static int check_backward(const pht_workspace* ws, const pht_gaussians* g, const pht_model_grads* grads,
+                          const pht_adam* adam) {
     if (!ws->fwd_valid) return fail(PHT_ESTATE, "backward requires a completed forward on this workspace");
     if (g->count != ws->gaussians) return fail(PHT_EINVAL, "model size differs from the forward's");
     if (!grads && !adam) return fail(PHT_EINVAL, "either gradients or an Adam update are required");
@@ -404,6 +407,63 @@ int pht_backward_densify(pht_workspace* ws, const pht_gaussians* g, const pht_vi
             return fail(PHT_EINVAL, "quantised SH Adam requires sh_first, sh_packed and sh_bounds");
         if (adam->lp_format != g->lp_format) return fail(PHT_EINVAL, "Adam lp_format differs from the model's");
     }
+    return PHT_OK;
+}
+
+// The blend backward of one workspace's views into its splat-gradient records (dl: one entry per
+// view of the workspace's forward).
+static int blend_backward_part(pht_workspace* ws, const pht_view_grads* dl, bool want_refine, hipStream_t st) {
+    std::vector<DevView> hv = ws->host_views;
+    for (int v = 0; v < ws->n_views; ++v) {
+        if (!dl[v].grad_color) return fail(PHT_EINVAL, "grad_color is required for every view");
+        hv[size_t(v)].grad_color = dl[v].grad_color;
+        hv[size_t(v)].grad_alpha = dl[v].grad_alpha;
+    }
+    launch_upload_views(ws->views, hv.data(), ws->n_views, st);
+    BlendArgs ba;
+    ba.ranges = ws->ranges;
+    ba.tile_order = ws->tile_order;
+    ba.inst_vals = ws->tvals[ws->tile_sel];
+    ba.rec = ws->rec;
+    ba.views = ws->views;
+    ba.n_views = ws->n_views;
+    ba.vp_log2 = ws->vp_log2;
+    ba.alpha_clip = ws->settings.alpha_clip;
+    ba.final_T = ws->final_T;
+    ba.n_contrib = ws->n_contrib;
+    ba.sgrad = ws->sgrad;
+    ba.counters = ws->counters;
+    ba.slot_cap = ws->slot_cap;
+    ba.n_gauss = ws->gaussians;
+    ba.want_refine = want_refine ? 1 : 0;
+    ba.queue = ws->settings.blend_queue ? ws->counters->queue : nullptr;
+    ba.loss_sum = nullptr;
+    ws->mark(5, 0, st);
+    if (ws->bwd_done)
+        launch_zero_f32(ws->sgrad, int64_t(ws->n_views) * ws->gaussians * kGradStride, st);
+    ws->bwd_done = true;
+    launch_blend_backward(ba, int(ws->tiles_total), ws->settings.tile_size, ws->settings.batch,
+                          ws->settings.blend_waves, st);
+    ws->mark(5, 1, st);
+    return PHT_OK;
+}
+
+static GaussGroup gauss_group(const pht_workspace* ws) {
+    GaussGroup gr;
+    gr.views = ws->views;
+    gr.n_views = ws->n_views;
+    gr.vp_log2 = ws->vp_log2;
+    gr.meta = ws->meta;
+    gr.sgrad = ws->sgrad;
+    gr.view_mask = ws->view_mask;
+    gr.counters = ws->counters;
+    return gr;
+}
+
+int pht_backward_densify(pht_workspace* ws, const pht_gaussians* g, const pht_view_grads* dl, pht_model_grads* grads,
+                         const pht_adam* adam, const pht_densify_step* densify, void* stream) {
+    if (!ws || !g || !dl) return fail(PHT_EINVAL, "null argument");
+    if (int r = check_backward(ws, g, grads, adam)) return r;
     DensifyStepArgs da{};
     if (densify) {
         const pht_densify_stats& d = densify->stats;
@@ -430,47 +490,12 @@ int pht_backward_densify(pht_workspace* ws, const pht_gaussians* g, const pht_vi
         da.n = ws->gaussians;
     }
     hipStream_t st = static_cast<hipStream_t>(stream);
-    std::vector<DevView> hv = ws->host_views;
-    for (int v = 0; v < ws->n_views; ++v) {
-        if (!dl[v].grad_color) return fail(PHT_EINVAL, "grad_color is required for every view");
-        hv[size_t(v)].grad_color = dl[v].grad_color;
-        hv[size_t(v)].grad_alpha = dl[v].grad_alpha;
-    }
-    launch_upload_views(ws->views, hv.data(), ws->n_views, st);
-    BlendArgs ba;
-    ba.ranges = ws->ranges;
-    ba.tile_order = ws->tile_order;
-    ba.inst_vals = ws->tvals[ws->tile_sel];
-    ba.rec = ws->rec;
-    ba.views = ws->views;
-    ba.n_views = ws->n_views;
-    ba.vp_log2 = ws->vp_log2;
-    ba.alpha_clip = ws->settings.alpha_clip;
-    ba.final_T = ws->final_T;
-    ba.n_contrib = ws->n_contrib;
-    ba.sgrad = ws->sgrad;
-    ba.counters = ws->counters;
-    ba.slot_cap = ws->slot_cap;
-    ba.n_gauss = ws->gaussians;
-    ba.want_refine = (grads && grads->refine_weight) || densify ? 1 : 0;
-    ba.queue = ws->settings.blend_queue ? ws->counters->queue : nullptr;
-    ws->mark(5, 0, st);
-    if (ws->bwd_done)
-        launch_zero_f32(ws->sgrad, int64_t(ws->n_views) * ws->gaussians * kGradStride, st);
-    ws->bwd_done = true;
-    launch_blend_backward(ba, int(ws->tiles_total), ws->settings.tile_size, ws->settings.batch,
-                          ws->settings.blend_waves, st);
-    ws->mark(5, 1, st);
+    if (int r = blend_backward_part(ws, dl, (grads && grads->refine_weight) || densify, st)) return r;
     GaussArgs ga;
     ga.g = *g;
     ga.s = ws->settings;
-    ga.views = ws->views;
-    ga.n_views = ws->n_views;
-    ga.vp_log2 = ws->vp_log2;
-    ga.meta = ws->meta;
-    ga.sgrad = ws->sgrad;
-    ga.view_mask = ws->view_mask;
-    ga.counters = ws->counters;
+    ga.n_groups = 1;
+    ga.grp[0] = gauss_group(ws);
     std::memset(&ga.out, 0, sizeof(ga.out));
     if (grads) ga.out = *grads;
     std::memset(&ga.adam, 0, sizeof(ga.adam));
@@ -481,6 +506,52 @@ int pht_backward_densify(pht_workspace* ws, const pht_gaussians* g, const pht_vi
     launch_gaussian_backward(ga, adam != nullptr, st);
     if (densify && !adam) launch_densify_step(da, st);
     ws->mark(6, 1, st);
+    return check_hip(hipGetLastError(), "backward launch");
+}
+
+int pht_backward_groups(pht_workspace* const* ws, int n_groups, const pht_gaussians* g, const pht_view_grads* dl,
+                        pht_model_grads* grads, const pht_adam* adam, void* const* group_streams, void* stream) {
+    if (!ws || !g || !dl || !group_streams) return fail(PHT_EINVAL, "null argument");
+    if (n_groups < 1 || n_groups > kMaxViewGroups) return fail(PHT_EINVAL, "n_groups must be 1..4");
+    for (int k = 0; k < n_groups; ++k) {
+        if (!ws[k]) return fail(PHT_EINVAL, "null workspace");
+        if (int r = check_backward(ws[k], g, grads, adam)) return r;
+        if (ws[k]->settings.active_sh_degree != ws[0]->settings.active_sh_degree ||
+            ws[k]->settings.sh_bases != ws[0]->settings.sh_bases)
+            return fail(PHT_EINVAL, "view groups must share the SH settings");
+        for (int j = 0; j < k; ++j)
+            if (ws[j] == ws[k]) return fail(PHT_EINVAL, "each view group needs its own workspace");
+    }
+    const bool want_refine = grads && grads->refine_weight;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    GaussArgs ga;
+    ga.g = *g;
+    ga.s = ws[0]->settings;
+    ga.n_groups = n_groups;
+    int first_view = 0;
+    for (int k = 0; k < n_groups; ++k) {
+        hipStream_t gs = static_cast<hipStream_t>(group_streams[k]);
+        if (int r = blend_backward_part(ws[k], dl + first_view, want_refine, gs)) return r;
+        first_view += ws[k]->n_views;
+        // the per-Gaussian backward on `stream` waits for this group's blend backward
+        if (gs != st) {
+            if (!ws[k]->join_ev)
+                if (int r = check_hip(hipEventCreateWithFlags(&ws[k]->join_ev, hipEventDisableTiming), "event"))
+                    return r;
+            if (int r = check_hip(hipEventRecord(ws[k]->join_ev, gs), "join record")) return r;
+            if (int r = check_hip(hipStreamWaitEvent(st, ws[k]->join_ev, 0), "join wait")) return r;
+        }
+        ga.grp[k] = gauss_group(ws[k]);
+    }
+    std::memset(&ga.out, 0, sizeof(ga.out));
+    if (grads) ga.out = *grads;
+    std::memset(&ga.adam, 0, sizeof(ga.adam));
+    if (adam) ga.adam = *adam;
+    std::memset(&ga.dens, 0, sizeof(ga.dens));
+    ga.has_dens = 0;
+    ws[0]->mark(6, 0, st);
+    launch_gaussian_backward(ga, adam != nullptr, st);
+    ws[0]->mark(6, 1, st);
     return check_hip(hipGetLastError(), "backward launch");
 }
 
@@ -853,6 +924,91 @@ int pht_step_graph_create_densify(pht_workspace* ws, const pht_gaussians* g, con
     }
     (void)hipStreamDestroy(cap_stream);
     if (ws) ws->profiling = was_profiling;
+    if (rc != PHT_OK) {
+        pht_step_graph_destroy(sg);
+        return rc;
+    }
+    *out_graph = sg;
+    return PHT_OK;
+}
+
+int pht_step_graph_create_groups(pht_workspace* const* ws, int n_groups, const int* group_views,
+                                 const pht_gaussians* g, const pht_camera* cams, int n_views, const pht_settings* s,
+                                 pht_view_outputs* out, const float* const* targets, float* const* grad_color_bufs,
+                                 float* loss_sum, const pht_loss_config* loss, const pht_adam* adam, void* stream,
+                                 pht_step_graph** out_graph) {
+    if (!out_graph || !ws || !group_views || !cams || !out || !targets || !grad_color_bufs || !adam)
+        return fail(PHT_EINVAL, "null argument");
+    if (n_groups < 1 || n_groups > kMaxViewGroups) return fail(PHT_EINVAL, "n_groups must be 1..4");
+    int total = 0;
+    for (int k = 0; k < n_groups; ++k) {
+        if (!ws[k] || group_views[k] <= 0) return fail(PHT_EINVAL, "every view group needs a workspace and views");
+        total += group_views[k];
+    }
+    if (total != n_views) return fail(PHT_EINVAL, "the view groups must cover the step's views");
+    pht_loss_config lc{};
+    lc.kind = PHT_LOSS_L1;
+    lc.photometric_weight = 1.0f;
+    if (loss) lc = *loss;
+    if (lc.kind != PHT_LOSS_L1 && lc.kind != PHT_LOSS_L1_SSIM) return fail(PHT_EINVAL, "unknown loss kind");
+    if (lc.kind == PHT_LOSS_L1_SSIM && loss_sum && !lc.ssim_scratch)
+        return fail(PHT_EINVAL, "the L1+SSIM loss sum needs per-view scratch");
+    *out_graph = nullptr;
+    (void)stream;
+    // the capture stream forks into one stream per group (default priority) and joins back
+    hipStream_t cap_stream = nullptr;
+    hipStream_t gs[kMaxViewGroups] = {};
+    hipEvent_t fork = nullptr;
+    int rc = check_hip(hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking), "capture stream");
+    for (int k = 0; rc == PHT_OK && k < n_groups; ++k)
+        rc = check_hip(hipStreamCreateWithFlags(&gs[k], hipStreamNonBlocking), "group stream");
+    if (rc == PHT_OK) rc = check_hip(hipEventCreateWithFlags(&fork, hipEventDisableTiming), "fork event");
+    pht_step_graph* sg = new pht_step_graph();
+    bool was_profiling[kMaxViewGroups] = {};
+    for (int k = 0; k < n_groups; ++k) {
+        was_profiling[k] = ws[k]->profiling;
+        ws[k]->profiling = false;
+    }
+    if (rc == PHT_OK) rc = check_hip(hipStreamBeginCapture(cap_stream, hipStreamCaptureModeThreadLocal), "begin capture");
+    if (rc == PHT_OK) {
+        if (loss_sum) launch_zero_f32(loss_sum, 1, cap_stream);
+        rc = check_hip(hipEventRecord(fork, cap_stream), "fork record");
+        std::vector<pht_view_grads> dl(static_cast<size_t>(n_views));
+        int v0 = 0;
+        for (int k = 0; rc == PHT_OK && k < n_groups; ++k) {
+            rc = check_hip(hipStreamWaitEvent(gs[k], fork, 0), "fork wait");
+            const int nk = group_views[k];
+            if (rc == PHT_OK && lc.kind == PHT_LOSS_L1) {
+                const FusedL1 l1{targets + v0, grad_color_bufs + v0, loss_sum, lc.photometric_weight};
+                rc = forward_impl(ws[k], g, cams + v0, nk, s, out + v0, 0, gs[k], &l1);
+            } else if (rc == PHT_OK) {
+                rc = pht_forward(ws[k], g, cams + v0, nk, s, out + v0, 0, gs[k]);
+                for (int v = v0; rc == PHT_OK && v < v0 + nk; ++v)
+                    rc = pht_l1_ssim_loss(out[v].color, targets[v], nullptr, grad_color_bufs[v], 0, loss_sum,
+                                          loss_sum ? lc.ssim_scratch[v] : nullptr, cams[v].width, cams[v].height,
+                                          lc.ssim_weight, lc.photometric_weight, gs[k]);
+            }
+            for (int v = v0; v < v0 + nk; ++v) {
+                dl[size_t(v)].grad_color = grad_color_bufs[v];
+                dl[size_t(v)].grad_alpha = nullptr;
+            }
+            v0 += nk;
+        }
+        void* gsv[kMaxViewGroups];
+        for (int k = 0; k < n_groups; ++k) gsv[k] = gs[k];
+        if (rc == PHT_OK) rc = pht_backward_groups(ws, n_groups, g, dl.data(), nullptr, adam, gsv, cap_stream);
+        hipGraph_t graph = nullptr;
+        const int rc_end = check_hip(hipStreamEndCapture(cap_stream, &graph), "end capture");
+        if (rc == PHT_OK) rc = rc_end;
+        sg->graph = graph;
+        if (rc == PHT_OK) rc = check_hip(hipGraphInstantiate(&sg->exec, graph, nullptr, nullptr, 0), "instantiate");
+    }
+    for (int k = 0; k < n_groups; ++k) {
+        ws[k]->profiling = was_profiling[k];
+        if (gs[k]) (void)hipStreamDestroy(gs[k]);
+    }
+    if (fork) (void)hipEventDestroy(fork);
+    if (cap_stream) (void)hipStreamDestroy(cap_stream);
     if (rc != PHT_OK) {
         pht_step_graph_destroy(sg);
         return rc;

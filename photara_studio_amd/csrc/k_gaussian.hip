@@ -17,7 +17,7 @@ namespace pht {
 
 // SH clamp flags of Gaussian i in view v: from its view-mask word when there are at most
 // kMaskClampViews view slots, else from the slot's meta record.
-__device__ __forceinline__ uint32_t slot_clamp(const GaussArgs& a, uint32_t mask, int64_t i, int v) {
+__device__ __forceinline__ uint32_t slot_clamp(const GaussGroup& a, uint32_t mask, int64_t i, int v) {
     if (a.vp_log2 <= kMaskClampViewsLog2) return mask_clamp_flags(mask, v);
     return a.meta[(i << a.vp_log2) + v].y >> kMetaClampShift;
 }
@@ -235,7 +235,9 @@ __global__ __launch_bounds__(256) void k_gaussian_bwd(GaussArgs a) {
     constexpr int NB = (DEG + 1) * (DEG + 1);
     __shared__ float s_sh[kScanThreads * kShPitch];
     if constexpr (ADAM) {
-        if (a.counters->overflow) return; // uniform over the grid
+        // uniform over the grid: a group whose instance list overflowed skipped its blends
+        for (int gr = 0; gr < a.n_groups; ++gr)
+            if (a.grp[gr].counters->overflow) return;
     }
     const int64_t i0 = int64_t(blockIdx.x) * kScanThreads;
     const int64_t i = i0 + threadIdx.x;
@@ -285,7 +287,6 @@ __global__ __launch_bounds__(256) void k_gaussian_bwd(GaussArgs a) {
     float gls[3] = {0.0f, 0.0f, 0.0f}, gq[4] = {0.0f, 0.0f, 0.0f, 0.0f}, glogit = 0.0f;
     if (live) {
         const float mean[3] = {a.g.means[3 * i], a.g.means[3 * i + 1], a.g.means[3 * i + 2]};
-        const uint32_t mask = a.view_mask[i];
         // Pass 1: geometry (EWA chain) and the SH view-direction term of the mean gradient.
         // Pass 2 (below) accumulates the SH coefficient gradients; splitting the view loop keeps
         // the 48 coefficient accumulators out of the geometry pass's registers.
@@ -297,23 +298,27 @@ __global__ __launch_bounds__(256) void k_gaussian_bwd(GaussArgs a) {
             const bool hf = a.g.filter_3d != nullptr;
             const float filt = hf ? a.g.filter_3d[i] : 0.0f;
             const Activated<float> act = activate<float>(ls, q, logit, hf, filt);
-            for (int v = 0; v < a.n_views; ++v) {
-                if (!((mask >> v) & 1u)) continue;
-                const float2* gp = reinterpret_cast<const float2*>(a.sgrad + (v * n + i) * kGradStride);
-                const float2 g0 = gp[0], g1 = gp[1], g2 = gp[2], g3 = gp[3], g4 = gp[4];
-                const float dconic[4] = {g1.x, g1.y, g2.x, g2.y};
-                const float dm2[2] = {g0.x, g0.y};
-                const pht_camera& cam = a.views[v].vi.cam;
-                splat_backward_pinhole(mean, act, cam, a.s, dconic, dm2, sg);
-                const float dcol[3] = {g3.x, g3.y, g4.x};
-                if (precomp) {
-                    gcol[0] += dcol[0]; gcol[1] += dcol[1]; gcol[2] += dcol[2];
-                } else {
-                    float d[3], raw[3];
-                    view_dir(mean, cam.position, d, raw);
-                    sh_backward_dir(DEG, raw, d, my_sh, slot_clamp(a, mask, i, v), dcol, sg.mean);
+            for (int gr = 0; gr < a.n_groups; ++gr) {
+                const GaussGroup& G = a.grp[gr];
+                const uint32_t mask = G.view_mask[i];
+                for (int v = 0; v < G.n_views; ++v) {
+                    if (!((mask >> v) & 1u)) continue;
+                    const float2* gp = reinterpret_cast<const float2*>(G.sgrad + (v * n + i) * kGradStride);
+                    const float2 g0 = gp[0], g1 = gp[1], g2 = gp[2], g3 = gp[3], g4 = gp[4];
+                    const float dconic[4] = {g1.x, g1.y, g2.x, g2.y};
+                    const float dm2[2] = {g0.x, g0.y};
+                    const pht_camera& cam = G.views[v].vi.cam;
+                    splat_backward_pinhole(mean, act, cam, a.s, dconic, dm2, sg);
+                    const float dcol[3] = {g3.x, g3.y, g4.x};
+                    if (precomp) {
+                        gcol[0] += dcol[0]; gcol[1] += dcol[1]; gcol[2] += dcol[2];
+                    } else {
+                        float d[3], raw[3];
+                        view_dir(mean, cam.position, d, raw);
+                        sh_backward_dir(DEG, raw, d, my_sh, slot_clamp(G, mask, i, v), dcol, sg.mean);
+                    }
+                    refine += g4.y;
                 }
-                refine += g4.y;
             }
             chain_raw(act, hf, filt, sg, gls, gq, glogit);
         }
@@ -321,25 +326,32 @@ __global__ __launch_bounds__(256) void k_gaussian_bwd(GaussArgs a) {
         if constexpr (DENS) {
             // its own pass over the views (the records are in cache), so the statistics' registers
             // do not add to the geometry pass's
+            // (one view group: the host refuses densification statistics with several)
+            const GaussGroup& G = a.grp[0];
+            const uint32_t mask = G.view_mask[i];
             DensifyRow dr = densify_load_row(a.dens.stats, i);
-            for (int v = 0; v < a.n_views; ++v) {
+            for (int v = 0; v < G.n_views; ++v) {
                 if (!((mask >> v) & 1u)) continue;
-                const float rw = a.sgrad[(v * n + i) * kGradStride + 9];
-                const DevView& dv = a.views[v];
+                const float rw = G.sgrad[(v * n + i) * kGradStride + 9];
+                const DevView& dv = G.views[v];
                 densify_observe(dr, a.dens.rules, a.dens.view_index[v], a.dens.inverse_resolution[v], dv.radii[i],
                                 dv.visibility[i], rw, a.dens.geometry_from_refine != 0, rw, false, 0.0f, false, 0.0f);
             }
             densify_store_row(a.dens.stats, i, dr);
         }
         if (!precomp) {
-            for (int v = 0; v < a.n_views; ++v) {
-                if (!((mask >> v) & 1u)) continue;
-                const float2* gp = reinterpret_cast<const float2*>(a.sgrad + (v * n + i) * kGradStride);
-                const float2 g3 = gp[3], g4 = gp[4];
-                const float dcol[3] = {g3.x, g3.y, g4.x};
-                float d[3], raw[3];
-                view_dir(mean, a.views[v].vi.cam.position, d, raw);
-                sh_backward_coef(DEG, d, slot_clamp(a, mask, i, v), dcol, gsh);
+            for (int gr = 0; gr < a.n_groups; ++gr) {
+                const GaussGroup& G = a.grp[gr];
+                const uint32_t mask = G.view_mask[i];
+                for (int v = 0; v < G.n_views; ++v) {
+                    if (!((mask >> v) & 1u)) continue;
+                    const float2* gp = reinterpret_cast<const float2*>(G.sgrad + (v * n + i) * kGradStride);
+                    const float2 g3 = gp[3], g4 = gp[4];
+                    const float dcol[3] = {g3.x, g3.y, g4.x};
+                    float d[3], raw[3];
+                    view_dir(mean, G.views[v].vi.cam.position, d, raw);
+                    sh_backward_coef(DEG, d, slot_clamp(G, mask, i, v), dcol, gsh);
+                }
             }
         }
     }

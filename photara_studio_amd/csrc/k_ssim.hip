@@ -187,7 +187,9 @@ __global__ __launch_bounds__(256) void k_sum_partials(const float* partials, int
     acc = wave_sum_to_last(acc);
     if (lane_id() == 63) s_red[threadIdx.x >> 6] = acc;
     __syncthreads();
-    if (threadIdx.x == 0) *loss_sum += (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]);
+    // atomic: the views of different view groups are summed from concurrent streams (within one
+    // stream the kernels, and so the additions, still run in launch order)
+    if (threadIdx.x == 0) atomic_add_f32(loss_sum, (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]));
 }
 
 size_t l1_ssim_scratch_floats(int width, int height) {

@@ -51,9 +51,11 @@ struct DensifyStepArgs {
     const float* sgrad;
     int64_t n;
 };
-struct GaussArgs {
-    pht_gaussians g;
-    pht_settings s;
+// One workspace's views as the per-Gaussian backward reads them: a step's views may be split into
+// view groups, each forwarded and blended on its own workspace (and stream); the per-Gaussian
+// backward then sums over every group's views, in group order.
+constexpr int kMaxViewGroups = 4;
+struct GaussGroup {
     const DevView* views;
     int n_views;
     int vp_log2;
@@ -61,6 +63,12 @@ struct GaussArgs {
     const float* sgrad;
     const uint32_t* view_mask;
     const Counters* counters;
+};
+struct GaussArgs {
+    pht_gaussians g;
+    pht_settings s;
+    int n_groups;
+    GaussGroup grp[kMaxViewGroups];
     pht_model_grads out;
     pht_adam adam;
     DensifyStepArgs dens; // with the fused Adam: the step's densification statistics (has_dens)

@@ -55,7 +55,7 @@ class TrainStep:
                  photometric_weight: float = 1.0, sh_optimizer: str = "quant",
                  sh_regularization_weight: float = 0.0, densify: bool = False,
                  view_indices: Optional[Sequence[int]] = None, densify_screen_threshold: float = 0.5,
-                 view_outputs: Optional[bool] = None):
+                 view_outputs: Optional[bool] = None, view_groups: int = 1):
         """loss: "l1" (loss_kernel's L1, cuda_ops.cu:886-898) or "l1_ssim" (the trainer's default
         (1 - w) L1 + w (1 - SSIM), fused_ssim.cu:473-560, w = ssim_weight).
         sh_optimizer: "quant", the trainer's SH Adam with fp16 update + 8-bit log-second state
@@ -70,7 +70,11 @@ class TrainStep:
         view_outputs: also write each view's per-Gaussian radii and contribution visibility (the
         render's radii / visibility outputs) into ``self.radii`` / ``self.visibility``; by default
         only when ``densify`` is set, whose statistics read them (otherwise nothing in the step
-        does, and the forward skips those writes)."""
+        does, and the forward skips those writes).
+        view_groups: split the views into this many contiguous groups (1-4), each forwarded and
+        blended on its own workspace and stream, with one per-Gaussian backward + Adam over all of
+        them (pht_backward_groups): one group's sorts then run beside another group's blends. The
+        step computes the same update; only the launch schedule differs."""
         if loss not in ("l1", "l1_ssim"):
             raise ValueError(f"unknown loss {loss!r}")
         if sh_optimizer not in ("quant", "fp32"):
@@ -98,8 +102,23 @@ class TrainStep:
         ts = int(options.tile_size)
         self.tiles = tiles_of(self.cameras, ts)
         self.pixels = sum(int(c.width) * int(c.height) for c in self.cameras)
-        self.capacity = instance_capacity or default_instance_capacity(n, nv)
-        self.ws = Workspace(dev, n, nv, ts, self.tiles, self.pixels, self.capacity)
+        self.view_groups = int(view_groups)
+        if not 1 <= self.view_groups <= min(4, nv):
+            raise ValueError("view_groups must be between 1 and min(4, views)")
+        if densify and self.view_groups > 1:
+            raise ValueError("the densification statistics need a single view group")
+        sizes = [nv // self.view_groups + (1 if k < nv % self.view_groups else 0) for k in range(self.view_groups)]
+        self.groups = [(sum(sizes[:k]), sizes[k]) for k in range(self.view_groups)]  # (first view, views)
+        if self.view_groups == 1:
+            self.capacities = [instance_capacity or default_instance_capacity(n, nv)]
+        else:
+            self.capacities = [(int(instance_capacity) * nk // nv + 4096) if instance_capacity else
+                               default_instance_capacity(n, nk) for _, nk in self.groups]
+        self.wss = [Workspace(dev, n, nk, ts, tiles_of(self.cameras[v0:v0 + nk], ts),
+                              sum(int(c.width) * int(c.height) for c in self.cameras[v0:v0 + nk]), cap)
+                    for (v0, nk), cap in zip(self.groups, self.capacities)]
+        self.ws, self.capacity = self.wss[0], self.capacities[0]
+        self._streams = [torch.cuda.Stream(device=dev) for _ in self.groups] if self.view_groups > 1 else []
         self.colors = [torch.empty((3, c.height, c.width), device=dev) for c in self.cameras]
         self.alphas = [torch.empty((c.height, c.width), device=dev) for c in self.cameras]
         self.view_outputs = bool(densify) if view_outputs is None else bool(view_outputs)
@@ -144,7 +163,7 @@ class TrainStep:
         # Counters of launched steps, copied back asynchronously (pinned ring, one row per step):
         # a step whose instance list overflowed the workspace left the model unchanged and is
         # re-run after the workspace grows (see _poll).
-        self._counts_host = torch.zeros((_COUNT_RING, 4), dtype=torch.int32, pin_memory=True)
+        self._counts_host = torch.zeros((_COUNT_RING, self.view_groups, 4), dtype=torch.int32, pin_memory=True)
         self._pending = collections.deque()  # (step_index, ring row, event, "graph" | "eager")
         self._ring_next = 0
         self.overflow_recoveries = 0
@@ -204,10 +223,11 @@ class TrainStep:
         self._bind_densify()
         if not self.ws.fits(n, len(self.cameras), int(self.options.tile_size), self.tiles, self.pixels,
                             self.capacity):
-            self.ws = None
+            self.ws, self.wss = None, []
             torch.cuda.empty_cache()
             self.ws = Workspace(dev, int(n * 1.25), len(self.cameras), int(self.options.tile_size), self.tiles,
                                 self.pixels, self.capacity)
+            self.wss = [self.ws]
         self.ensure_capacity()  # the instance list grows with the model
         return counts
 
@@ -225,8 +245,9 @@ class TrainStep:
         row = self._ring_next % _COUNT_RING
         self._ring_next += 1
         stream = torch.cuda.current_stream()
-        check(self.lib.pht_get_counts_async(self.ws.handle, C.c_void_p(self._counts_host[row].data_ptr()),
-                                            C.c_void_p(stream.cuda_stream)))
+        for k, ws in enumerate(self.wss):
+            check(self.lib.pht_get_counts_async(ws.handle, C.c_void_p(self._counts_host[row, k].data_ptr()),
+                                                C.c_void_p(stream.cuda_stream)))
         ev = torch.cuda.Event()
         ev.record(stream)
         self._pending.append((self.step_index, row, ev, mode))
@@ -246,7 +267,7 @@ class TrainStep:
                 return
             self._pending.popleft()
             done += 1
-            if int(self._counts_host[row, 2]) != 0:
+            if bool((self._counts_host[row, :, 2] != 0).any()):
                 modes = [mode] + [p[3] for p in self._pending]
                 self._pending.clear()
                 torch.cuda.synchronize()
@@ -263,50 +284,82 @@ class TrainStep:
         """Waits for every launched step and re-runs any that overflowed the instance capacity."""
         self._poll(block=True)
 
+    @staticmethod
+    def _ptr(arr, first: int, elem):
+        """Typed pointer to element `first` of a ctypes array (a view group's slice)."""
+        return C.cast(C.addressof(arr) + first * C.sizeof(elem), C.POINTER(elem))
+
+    def _forward_loss(self, ws: Workspace, v0: int, nk: int, stream) -> None:
+        """Forward and loss of views v0 .. v0 + nk on `ws`, enqueued on `stream` (a c_void_p)."""
+        cams = self._ptr(self.cams, v0, _abi.pht_camera)
+        outs = self._ptr(self.outs, v0, _abi.pht_view_outputs)
+        if self.loss_kind == "l1":
+            # the L1 loss and its gradient come out of the forward's blend (pht_forward_l1)
+            check(self.lib.pht_forward_l1(ws.handle, C.byref(self.gauss), cams, nk, C.byref(self.settings), outs,
+                                          self._ptr(self._target_ptrs, v0, C.c_void_p),
+                                          self._ptr(self._grad_ptrs, v0, C.c_void_p),
+                                          C.c_void_p(self.loss_sum.data_ptr()), self.photometric_weight, 0, stream))
+            return
+        check(self.lib.pht_forward(ws.handle, C.byref(self.gauss), cams, nk, C.byref(self.settings), outs, 0, stream))
+        for k in range(v0, v0 + nk):
+            cam = self.cameras[k]
+            check(self.lib.pht_l1_ssim_loss(C.c_void_p(self.colors[k].data_ptr()),
+                                            C.c_void_p(self.targets[k].data_ptr()), None,
+                                            C.c_void_p(self.grad_colors[k].data_ptr()), 0,
+                                            C.c_void_p(self.loss_sum.data_ptr()),
+                                            C.c_void_p(self.ssim_scratch[k].data_ptr()), int(cam.width),
+                                            int(cam.height), self.ssim_weight, self.photometric_weight, stream))
+
     def step_eager(self) -> None:
-        """One iteration launched kernel by kernel on the current stream (no host sync)."""
+        """One iteration launched kernel by kernel on the current stream (no host sync); with view
+        groups, each group's forward and blend backward run on the group's stream."""
         self._poll()
         self.step_index += 1
         c1, c2 = self._corrections(self.step_index)
         self.adam.correction1, self.adam.correction2 = c1, c2
         if self.sh_adam is not None:
             self.sh_adam.correction1, self.sh_adam.correction2 = c1, c2
-        stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
-        nv = len(self.cameras)
+        main = torch.cuda.current_stream()
+        stream = C.c_void_p(main.cuda_stream)
         self.loss_sum.zero_()
-        if self.loss_kind == "l1":
-            # the L1 loss and its gradient come out of the forward's blend (pht_forward_l1)
-            check(self.lib.pht_forward_l1(self.ws.handle, C.byref(self.gauss), self.cams, nv, C.byref(self.settings),
-                                          self.outs, self._target_ptrs, self._grad_ptrs,
-                                          C.c_void_p(self.loss_sum.data_ptr()), self.photometric_weight, 0, stream))
-        else:
-            check(self.lib.pht_forward(self.ws.handle, C.byref(self.gauss), self.cams, nv, C.byref(self.settings),
-                                       self.outs, 0, stream))
-            for k in range(nv):
-                cam = self.cameras[k]
-                check(self.lib.pht_l1_ssim_loss(C.c_void_p(self.colors[k].data_ptr()),
-                                                C.c_void_p(self.targets[k].data_ptr()), None,
-                                                C.c_void_p(self.grad_colors[k].data_ptr()), 0,
-                                                C.c_void_p(self.loss_sum.data_ptr()),
-                                                C.c_void_p(self.ssim_scratch[k].data_ptr()), int(cam.width),
-                                                int(cam.height), self.ssim_weight, self.photometric_weight, stream))
         adam = make_adam(self.model, self.adam, self.sh_adam)
-        self._backward(adam, stream)
+        if self.view_groups == 1:
+            self._forward_loss(self.ws, 0, len(self.cameras), stream)
+            self._backward(adam, stream)
+        else:
+            for ws, gs, (v0, nk) in zip(self.wss, self._streams, self.groups):
+                gs.wait_stream(main)
+                self._forward_loss(ws, v0, nk, C.c_void_p(gs.cuda_stream))
+            wsp = (C.c_void_p * self.view_groups)(*[ws.handle.value for ws in self.wss])
+            gsp = (C.c_void_p * self.view_groups)(*[gs.cuda_stream for gs in self._streams])
+            check(self.lib.pht_backward_groups(wsp, self.view_groups, C.byref(self.gauss), self.dl, None,
+                                               C.byref(adam), gsp, stream))
         self._track("eager")
 
     def capture(self) -> None:
-        """Capture forward + loss + backward + Adam as one hipGraph (hyper-parameters on device)."""
+        """Capture forward + loss + backward + Adam as one hipGraph (hyper-parameters on device);
+        with view groups the graph forks into one branch per group and joins before the
+        per-Gaussian backward."""
         nv = len(self.cameras)
         self._adam_graph = make_adam(self.model, self.adam, self.sh_adam)
         self._adam_graph.device_hparams = self.hparams.data_ptr()
         handle = C.c_void_p()
         stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
-        check(self.lib.pht_step_graph_create_densify(self.ws.handle, C.byref(self.gauss), self.cams, nv,
-                                                     C.byref(self.settings), self.outs, self._target_ptrs, self._grad_ptrs,
-                                                     C.c_void_p(self.loss_sum.data_ptr()), C.byref(self.loss_cfg),
-                                                     C.byref(self._adam_graph),
-                                                     C.byref(self._dstep) if self.densify else None, stream,
-                                                     C.byref(handle)))
+        if self.view_groups == 1:
+            check(self.lib.pht_step_graph_create_densify(self.ws.handle, C.byref(self.gauss), self.cams, nv,
+                                                         C.byref(self.settings), self.outs, self._target_ptrs,
+                                                         self._grad_ptrs, C.c_void_p(self.loss_sum.data_ptr()),
+                                                         C.byref(self.loss_cfg), C.byref(self._adam_graph),
+                                                         C.byref(self._dstep) if self.densify else None, stream,
+                                                         C.byref(handle)))
+        else:
+            wsp = (C.c_void_p * self.view_groups)(*[ws.handle.value for ws in self.wss])
+            sizes = (C.c_int * self.view_groups)(*[nk for _, nk in self.groups])
+            check(self.lib.pht_step_graph_create_groups(wsp, self.view_groups, sizes, C.byref(self.gauss), self.cams,
+                                                        nv, C.byref(self.settings), self.outs, self._target_ptrs,
+                                                        self._grad_ptrs, C.c_void_p(self.loss_sum.data_ptr()),
+                                                        C.byref(self.loss_cfg), C.byref(self._adam_graph), stream,
+                                                        C.byref(handle)))
         self.graph = handle
 
     def step_graph(self) -> None:
@@ -322,29 +375,50 @@ class TrainStep:
         self._track("graph")
 
     def counts(self) -> _abi.pht_counts:
-        return self.ws.counts()
+        """The last forward's counts, summed over the view groups (overflow: any group's)."""
+        if self.view_groups == 1:
+            return self.ws.counts()
+        out = _abi.pht_counts()
+        for ws in self.wss:
+            c = ws.counts()
+            out.visible += c.visible
+            out.instances += c.instances
+            out.fp64_conics += c.fp64_conics
+            out.overflow = 1 if (out.overflow or c.overflow) else 0
+        return out
 
     def ensure_capacity(self, headroom: float = 1.25) -> int:
-        """Run forward passes until the instance list fits the workspace (growing it by the
-        measured count x headroom); returns the instance count. Parameters are not updated."""
+        """Run forward passes until every view group's instance list fits its workspace (growing
+        a workspace to its measured count x headroom); returns the instance count. Parameters are
+        not updated."""
         lib = self.lib
         stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
-        while True:
-            check(lib.pht_forward(self.ws.handle, C.byref(self.gauss), self.cams, len(self.cameras),
-                                  C.byref(self.settings), self.outs, 0, stream))
-            c = self.ws.counts()
-            if not c.overflow:
-                return int(c.instances)
-            if self.graph is not None:
-                lib.pht_step_graph_destroy(self.graph)
-                self.graph = None
-            self.capacity = int(c.instances * headroom) + 4096
-            dev = self.model.means.device
-            rows = max(self.model.size(), int(self.ws.cap.gaussians))
-            self.ws = None
-            torch.cuda.empty_cache()
-            self.ws = Workspace(dev, rows, len(self.cameras), int(self.options.tile_size), self.tiles, self.pixels,
-                                self.capacity)
+        total = 0
+        for k, (v0, nk) in enumerate(self.groups):
+            while True:
+                check(lib.pht_forward(self.wss[k].handle, C.byref(self.gauss), self._ptr(self.cams, v0, _abi.pht_camera),
+                                      nk, C.byref(self.settings), self._ptr(self.outs, v0, _abi.pht_view_outputs), 0,
+                                      stream))
+                c = self.wss[k].counts()
+                if not c.overflow:
+                    total += int(c.instances)
+                    break
+                if self.graph is not None:
+                    lib.pht_step_graph_destroy(self.graph)
+                    self.graph = None
+                self.capacities[k] = int(c.instances * headroom) + 4096
+                dev = self.model.means.device
+                rows = max(self.model.size(), int(self.wss[k].cap.gaussians))
+                cams = self.cameras[v0:v0 + nk]
+                ts = int(self.options.tile_size)
+                self.wss[k] = None
+                if k == 0:
+                    self.ws = None
+                torch.cuda.empty_cache()
+                self.wss[k] = Workspace(dev, rows, nk, ts, tiles_of(cams, ts),
+                                        sum(int(cm.width) * int(cm.height) for cm in cams), self.capacities[k])
+            self.ws, self.capacity = self.wss[0], self.capacities[0]
+        return total
 
     def loss(self) -> float:
         self.sync()
