@@ -311,7 +311,7 @@ def main():
     p.add_argument("--batch", type=int, default=64)
     p.add_argument("--waves", type=int, default=1, help="blend waves per workgroup (1, 2, 4) without --autotune")
     p.add_argument("--queue", action="store_true", help="persistent work-queue blends without --autotune")
-    p.add_argument("--view-groups", type=int, default=1,
+    p.add_argument("--view-groups", type=int, default=2,
                    help="split the batched step's views into this many groups on their own streams")
     p.add_argument("--no-autotune", dest="autotune", action="store_false",
                    help="skip the launch-shape sweep (tile {16,8} x batch {32..256}, then waves {1,2,4} x "
@@ -481,6 +481,7 @@ def main():
                    "gaussians": n, "views": nv, "resolution": res_label, "tile": int(bopts.tile_size),
                    "batch": int(bopts.batch), "blend_waves": int(bopts.blend_waves),
                    "blend_launch": "persistent work queue" if bopts.blend_queue else "flat grid",
+                   "view_groups": int(args.view_groups),
                    "storage": "bf16 sh/log_scales/opacity, fp32 masters",
                    "sh_optimizer": ("FP32 moments (as the plain step)" if args.sh_optimizer == "fp32" else
                                     "quantised (fp16 update + 8-bit log-second)") + ", fp32 structure moments",
