@@ -398,21 +398,25 @@ def main():
     assert not bc.overflow
     print(f"[bench] batched bf16 graph: {step_s * 1e3:.3f} ms/step, {nv / step_s:.2f} view-steps/s", flush=True)
 
-    # per-stage kernel times (hipEvents on the launch streams) over a few eager steps; with view
-    # groups, each stage's times are summed over the groups (whose stages overlap in time)
-    for ws in batched.wss:
-        ws.set_profiling(True)
+    # per-stage kernel times (hipEvents on the launch stream) over a few eager steps of the same
+    # workload in one view group: with several groups the stages of different groups overlap in
+    # time, so per-group events would not add up to kernel time
+    staged = batched
+    if args.view_groups > 1:
+        staged = TrainStep(to_low_precision(model_from(arrays, dev), torch.bfloat16), cams, targets, bopts,
+                           LearningRates(), sh_optimizer=args.sh_optimizer)
+        staged.ensure_capacity()
+    staged.ws.set_profiling(True)
     acc = {}
     prof_steps = 3
     for _ in range(prof_steps):
-        batched.step_eager()
-        torch.cuda.synchronize()
-        for ws in batched.wss:
-            for k, v in ws.stage_times().items():
-                if v >= 0:
-                    acc[k] = acc.get(k, 0.0) + v / prof_steps
-    for ws in batched.wss:
-        ws.set_profiling(False)
+        staged.step_eager()
+        for k, v in staged.ws.stage_times().items():
+            acc[k] = acc.get(k, 0.0) + v / prof_steps
+    staged.ws.set_profiling(False)
+    if staged is not batched:
+        del staged
+        torch.cuda.empty_cache()
     pixels = sum(c.width * c.height for c in cams)
     bt = int(bopts.tile_size)
     tiles = sum(((c.width + bt - 1) // bt) * ((c.height + bt - 1) // bt) for c in cams)
