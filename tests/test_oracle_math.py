@@ -33,8 +33,8 @@ def test_pexpf_error_bound(oracle):
 
 def test_pexp2f_blend_error_bound(oracle):
     """The blend's 2^x on the exponents that reach it ([-8, 0]): every 64th float32 bit pattern
-    of the interval (16M values) plus the ends and the integers, within 1.5 ulp of float64 2^x;
-    2^integer exact; NaN propagates."""
+    of the interval (16M values) plus the ends and the integers, within 2.1 ulp of float64 2^x (the
+    degree-5 fit's 2.03) and 2.5e-7 relative; 2^integer exact; NaN propagates."""
     lo_bits = np.float32(-8.0).view(np.uint32)  # negative floats: larger bits = more negative
     neg = np.arange(0x80000000, lo_bits + 1, 1 << 6, dtype=np.uint64).astype(np.uint32).view(np.float32)
     ints = -np.arange(0, 9, dtype=np.float32)
@@ -42,7 +42,8 @@ def test_pexp2f_blend_error_bound(oracle):
     y = oracle.pexp2f_blend(x).astype(np.float64)
     want = np.exp2(x.astype(np.float64))
     ulp = np.abs(y - want) / np.spacing(want.astype(np.float32)).astype(np.float64)
-    assert ulp.max() <= 1.5, float(ulp.max())
+    assert ulp.max() <= 2.1, float(ulp.max())
+    assert float((np.abs(y - want) / want).max()) <= 2.5e-7
     np.testing.assert_array_equal(oracle.pexp2f_blend(ints), np.exp2(ints))
     assert np.isnan(oracle.pexp2f_blend(np.array([np.nan], np.float32))[0])
 
