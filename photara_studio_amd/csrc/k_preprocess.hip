@@ -85,6 +85,54 @@ __device__ __forceinline__ void load_sh_row(const float* f, const uint16_t* lp, 
     }
 }
 
+// sh_eval on 16-bit stored coefficients kept packed two per 32-bit word (w[e / 2], element e in
+// the low half when e is even): each is widened where sh_eval reads it, with the same conversion
+// and the same per-channel fused multiply-add order, so the colours are sh_eval's bit for bit;
+// the row takes half the registers of the widened coefficients.
+template <int LP, int NC>
+__device__ __forceinline__ void load_sh_words(const uint16_t* lp, int64_t row, int row_elems, uint32_t (&w)[(NC + 1) / 2]) {
+    constexpr int NW = (NC + 1) / 2;
+    if ((row_elems * 2) % 16 == 0 && (reinterpret_cast<uintptr_t>(lp) & 15u) == 0) {
+        const uint4* src = reinterpret_cast<const uint4*>(lp + row);
+#pragma unroll
+        for (int c = 0; c < (NW + 3) / 4; ++c) {
+            const uint4 q = src[c];
+            const uint32_t v[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                if (4 * c + e < NW) w[4 * c + e] = v[e];
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < NW; ++k) {
+            const uint32_t lo = lp[row + 2 * k];
+            const uint32_t hi = 2 * k + 1 < NC ? uint32_t(lp[row + 2 * k + 1]) : 0u;
+            w[k] = lo | (hi << 16);
+        }
+    }
+}
+template <int LP, int NC>
+__device__ __forceinline__ float sh_word(const uint32_t (&w)[(NC + 1) / 2], int e) {
+    return lp_to_f32(uint16_t((e & 1) ? (w[e >> 1] >> 16) : (w[e >> 1] & 0xffffu)), LP);
+}
+template <int LP, int DEG>
+__device__ __forceinline__ void sh_eval_words(const float d[3], const uint32_t (&w)[((DEG + 1) * (DEG + 1) * 3 + 1) / 2],
+                                              float rgb[3], uint32_t& clamp_bits) {
+    constexpr int NB = (DEG + 1) * (DEG + 1), NC = NB * 3;
+    float Y[16];
+    sh_basis(DEG, d, Y);
+    clamp_bits = 0;
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+        float acc = Y[0] * sh_word<LP, NC>(w, ch);
+#pragma unroll
+        for (int k = 1; k < NB; ++k) acc = fma_(Y[k], sh_word<LP, NC>(w, k * 3 + ch), acc);
+        acc = acc + 0.5f;
+        if (acc < 0.0f) clamp_bits |= 1u << ch;
+        rgb[ch] = max_(acc, 0.0f);
+    }
+}
+
 template <int DEG, int LP>
 __global__ __launch_bounds__(256) void k_preprocess(PreArgs a, int64_t n_slots) {
     const int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x;
@@ -119,12 +167,18 @@ __global__ __launch_bounds__(256) void k_preprocess(PreArgs a, int64_t n_slots) 
                     for (int ch = 0; ch < 3; ++ch) rgb[ch] = a.g.colors_precomp[3 * gi + ch];
                 } else {
                     constexpr int NB = (DEG + 1) * (DEG + 1);
-                    float coef[NB * 3];
                     const int64_t row = gi * int64_t(a.s.sh_bases) * 3;
-                    load_sh_row<LP, NB * 3>(a.g.sh, a.g.sh_lp, row, a.s.sh_bases * 3, coef);
                     float d[3], raw[3];
                     view_dir(mean, dv.vi.cam.position, d, raw);
-                    sh_eval(DEG, d, coef, rgb, clamp);
+                    if constexpr (LP == PHT_LP_NONE) {
+                        float coef[NB * 3];
+                        load_sh_row<LP, NB * 3>(a.g.sh, a.g.sh_lp, row, a.s.sh_bases * 3, coef);
+                        sh_eval(DEG, d, coef, rgb, clamp);
+                    } else {
+                        uint32_t w[(NB * 3 + 1) / 2];
+                        load_sh_words<LP, NB * 3>(a.g.sh_lp, row, a.s.sh_bases * 3, w);
+                        sh_eval_words<LP, DEG>(d, w, rgb, clamp);
+                    }
                 }
                 float ex, ey, qmax;
                 patch_extent(p.conic[0], p.conic[1], p.conic[2], p.opacity, ex, ey, qmax);
