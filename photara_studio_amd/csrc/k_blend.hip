@@ -430,51 +430,51 @@ __device__ __forceinline__ void blend_bwd_item(const BlendArgs& a, int item, int
                 const float dx = A.x - pxf[k], dy = A.y - pyf[k];
                 float alpha = 0.0f, G = 0.0f;
                 const bool contrib =
-                    pair_alpha_flat<true>(A.z, A.w, B.x, B.y, dx, dy, clip, alpha, G) && idx < last[k];
+                    pair_alpha_flat<false>(A.z, A.w, B.x, B.y, dx, dy, clip, alpha, G) && idx < last[k];
                 if (__ballot(contrib) == 0) continue;
                 any = true;
-                // Branch-free over the wave: lanes without a contributing pair zero the factors
-                // every term is a product of (dL/dalpha, hence dL/dG, and the blend weight); G and
-                // the offsets are finite on every lane (a rejected pair's G is 1), so their terms
-                // are exact zeros and T, E keep their values.
+                // Branch-free over the wave: a lane without a contributing pair takes alpha = 0 and a
+                // zero s (below), so its blend weight and every gradient term are exact zeros and E
+                // keeps its value; its G (unspecified, possibly inf or NaN) is never used.
                 // (The recovered T and the gradients are not compared bit for bit: the hardware
                 // reciprocal / square root and fused multiply-adds are used throughout.)
-                const float ox = dx, oy = dy;
-                const float inv_1ma = __builtin_amdgcn_rcpf(1.0f - alpha);
+                const float ac = contrib ? alpha : 0.0f;
+                const float inv_1ma = __builtin_amdgcn_rcpf(1.0f - ac);
                 const float Tb = T[k] * inv_1ma;
-                const float wgt = contrib ? alpha * Tb : 0.0f;
+                const float wgt = ac * Tb;
                 const float c_dot = fmaf(cb, gp2[k], fmaf(B.w, gp1[k], B.z * gp0[k]));
-                const float dL_dalpha = contrib ? fmaf(Tb, c_dot, -E[k] * inv_1ma) : 0.0f;
-                const float dL_dG = B.y * dL_dalpha;
+                const float dL_dalpha = fmaf(Tb, c_dot, -E[k] * inv_1ma);
+                // s = G dL/dalpha, the pair's dL/dopacity; dL/dG = opacity * dL/dalpha, so
+                // dL/dG * G = opacity * s
+                const float s = contrib ? G * dL_dalpha : 0.0f;
                 if constexpr (REFINE) {
-                    const float gdx = G * ox, gdy = G * oy;
+                    const float wG = B.y * s;
                     // dL/dmean2d = dL/dG * dG/d(dx, dy), dG/ddx = -G (a dx + b dy), dG/ddy = -G (b dx + c dy)
-                    const float ddx = -dL_dG * fmaf(gdx, Cc.y, gdy * Cc.z);
-                    const float ddy = -dL_dG * fmaf(gdy, Cc.w, gdx * Cc.z);
+                    const float ddx = -wG * fmaf(dx, Cc.y, dy * Cc.z);
+                    const float ddy = -wG * fmaf(dy, Cc.w, dx * Cc.z);
                     r[0] += ddx;
                     r[1] += ddy;
                     // dL/dconic = -dL/dG * G * (dx^2, dx dy, dy^2) / 2
-                    const float h = -0.5f * dL_dG;
-                    const float hx = h * gdx;
-                    r[2] = fmaf(hx, ox, r[2]);
-                    r[3] = fmaf(hx, oy, r[3]);
-                    r[4] = fmaf(h * gdy, oy, r[4]);
+                    const float h = -0.5f * wG;
+                    const float hx = h * dx;
+                    r[2] = fmaf(hx, dx, r[2]);
+                    r[3] = fmaf(hx, dy, r[3]);
+                    r[4] = fmaf(h * dy, dy, r[4]);
                     // the refine statistic is a norm per pixel: it needs the per-pair dL/dmean2d
                     const float rx = ddx * fW, ry = ddy * fH;
                     r[9] = fmaf(__builtin_amdgcn_sqrtf(fmaf(rx, rx, ry * ry)), refine_den[k], r[9]);
                 } else {
-                    // Without it, both gradients are linear in the weighted moments of the offsets,
-                    // w = dL/dG * G: sum w (dx, dy) and sum w (dx^2, dx dy, dy^2). The conic and the
-                    // -1/2 are applied once per (instance, tile) after the reduction.
-                    const float wG = dL_dG * G;
-                    const float t = wG * ox, u = wG * oy;
+                    // Without it, both gradients are linear in the moments of the offsets weighted
+                    // by s: sum s (dx, dy) and sum s (dx^2, dx dy, dy^2). The opacity, the conic and
+                    // the -1/2 are applied once per (instance, tile) after the reduction.
+                    const float t = s * dx, u = s * dy;
                     r[0] += t;
                     r[1] += u;
-                    r[2] = fmaf(t, ox, r[2]);
-                    r[3] = fmaf(t, oy, r[3]);
-                    r[4] = fmaf(u, oy, r[4]);
+                    r[2] = fmaf(t, dx, r[2]);
+                    r[3] = fmaf(t, dy, r[3]);
+                    r[4] = fmaf(u, dy, r[4]);
                 }
-                r[5] = fmaf(G, dL_dalpha, r[5]);
+                r[5] += s;
                 r[6] = fmaf(wgt, gp0[k], r[6]);
                 r[7] = fmaf(wgt, gp1[k], r[7]);
                 r[8] = fmaf(wgt, gp2[k], r[8]);
@@ -484,15 +484,16 @@ __device__ __forceinline__ void blend_bwd_item(const BlendArgs& a, int item, int
             if (!any) continue;
             float tot = wave_scatter10(r, lane);
             if constexpr (!REFINE) {
-                // moments -> gradients: dL/dmean2d = -(conic . (sum w dx, sum w dy)),
-                // dL/dconic = -1/2 (sum w dx^2, sum w dx dy, sum w dy^2); the first-order moments
-                // sit in lanes 0 and 8 (kScatterSlot)
+                // moments -> gradients: dL/dmean2d = -opacity (conic . (sum s dx, sum s dy)),
+                // dL/dconic = -opacity / 2 (sum s dx^2, sum s dx dy, sum s dy^2); the first-order
+                // moments sit in lanes 0 and 8 (kScatterSlot)
                 const float sx = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, tot), 0));
                 const float sy = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, tot), 8));
                 const float4 Cc = sC[j];
-                if (slot_g == 0) tot = -fmaf(Cc.y, sx, Cc.z * sy);
-                else if (slot_g == 1) tot = -fmaf(Cc.w, sy, Cc.z * sx);
-                else if (slot_g >= 2 && slot_g <= 4) tot = -0.5f * tot;
+                const float op = sB[j].y;
+                if (slot_g == 0) tot = -op * fmaf(Cc.y, sx, Cc.z * sy);
+                else if (slot_g == 1) tot = -op * fmaf(Cc.w, sy, Cc.z * sx);
+                else if (slot_g >= 2 && slot_g <= 4) tot = (-0.5f * op) * tot;
             }
             if (lane < 16 && slot_g >= 0 && tot != 0.0f)
                 atomic_add_f32(a.sgrad + grad_offset(sS[j], a.vp_log2, a.n_gauss) + slot_g, tot);
