@@ -32,6 +32,22 @@ template <int LP> __device__ __forceinline__ float ld_lp(const float* f, const u
 // moments, reduced-precision mirror) are done cooperatively over the block's contiguous range,
 // one coalesced pass each, instead of one 192-byte row per lane.
 constexpr int kShPitch = 49;
+// With bf16 SH storage the backward stages the coefficient rows as loaded, two per 32-bit word
+// (odd pitch of 25 words), and the LDS this frees holds, per thread, the colour gradients of its
+// first kStashViews visible views, written by the geometry pass for the SH-coefficient pass. That
+// pass then reads no splat-gradient record a second time: by the time it runs, the records the
+// geometry pass read are long gone from L2, and re-reading them cost about one extra pass over
+// the records in HBM traffic.
+constexpr int kPackPitch = 25;
+constexpr int kStashViews = (kShPitch - kPackPitch) / 3;
+static_assert(kPackPitch * 2 >= 48 && kStashViews * 3 + kPackPitch <= kShPitch, "packed rows + stash fit the float rows");
+template <int LP> struct PackedCoef {
+    const uint32_t* w; // w[e / 2]: coefficient e in the low half when e is even
+    __device__ __forceinline__ float operator[](int e) const {
+        const uint32_t x = w[e >> 1];
+        return lp_to_f32(uint16_t((e & 1) ? (x >> 16) : (x & 0xffffu)), LP);
+    }
+};
 
 // Walks e = t, t + 256, ... over [0, rows * stride) keeping (row, k) = divmod(e, stride)
 // incrementally; calls f(e, row, k) for the active coefficients k < active.
@@ -248,9 +264,31 @@ __global__ __launch_bounds__(256) void k_gaussian_bwd(GaussArgs a) {
     const int64_t row0 = i0 * int64_t(stride);
     const bool precomp = a.g.colors_precomp != nullptr;
     const bool use_sh = !precomp;
-    if (use_sh) {
-        const bool chunked = (stride & 3) == 0 && (LP == PHT_LP_NONE ? aligned16(a.g.sh + row0)
-                                                                     : (reinterpret_cast<uintptr_t>(a.g.sh_lp + row0) & 7u) == 0);
+    // (bf16 only: the fp16 variant already spills, and the stash's registers would cost it a wave)
+    constexpr bool kPacked = LP == PHT_LP_BF16;
+    uint32_t* const s_pack = reinterpret_cast<uint32_t*>(s_sh);
+    float* const s_stash = s_sh + kScanThreads * kPackPitch; // [(u * 3 + ch) * 256 + thread]
+    if (use_sh && kPacked) {
+        if ((stride & 3) == 0 && (reinterpret_cast<uintptr_t>(a.g.sh_lp + row0) & 7u) == 0) {
+            uint2 v[4];
+            for_block_chunks<4>(
+                rows, stride,
+                [&](int u, int c, int, int k0) {
+                    if (k0 < NB * 3) v[u] = reinterpret_cast<const uint2*>(a.g.sh_lp + row0)[c];
+                },
+                [&](int u, int, int row, int k0) {
+                    if (k0 >= NB * 3) return;
+                    uint32_t* d = s_pack + row * kPackPitch + (k0 >> 1);
+                    d[0] = v[u].x;
+                    d[1] = v[u].y;
+                });
+        } else {
+            uint16_t* h = reinterpret_cast<uint16_t*>(s_pack);
+            for_block_rows(rows, stride, NB * 3,
+                           [&](int e, int row, int k) { h[row * 2 * kPackPitch + k] = a.g.sh_lp[row0 + e]; });
+        }
+    } else if (use_sh) {
+        const bool chunked = (stride & 3) == 0 && aligned16(a.g.sh + row0);
         if (chunked) {
             float4 v[4];
             for_block_chunks<4>(
@@ -274,6 +312,7 @@ __global__ __launch_bounds__(256) void k_gaussian_bwd(GaussArgs a) {
     }
     __syncthreads();
     float* my_sh = s_sh + threadIdx.x * kShPitch;
+    const PackedCoef<LP> my_pack{s_pack + threadIdx.x * kPackPitch};
 
     SplatGrad<float> sg;
     for (int k = 0; k < 3; ++k) { sg.mean[k] = 0.0f; sg.scale[k] = 0.0f; }
@@ -298,6 +337,7 @@ __global__ __launch_bounds__(256) void k_gaussian_bwd(GaussArgs a) {
             const bool hf = a.g.filter_3d != nullptr;
             const float filt = hf ? a.g.filter_3d[i] : 0.0f;
             const Activated<float> act = activate<float>(ls, q, logit, hf, filt);
+            int u = 0; // visible views so far: the stash slot of the next one
             for (int gr = 0; gr < a.n_groups; ++gr) {
                 const GaussGroup& G = a.grp[gr];
                 const uint32_t mask = G.view_mask[i];
@@ -315,9 +355,19 @@ __global__ __launch_bounds__(256) void k_gaussian_bwd(GaussArgs a) {
                     } else {
                         float d[3], raw[3];
                         view_dir(mean, cam.position, d, raw);
-                        sh_backward_dir(DEG, raw, d, my_sh, slot_clamp(G, mask, i, v), dcol, sg.mean);
+                        if constexpr (kPacked) {
+                            if (u < kStashViews) {
+#pragma unroll
+                                for (int ch = 0; ch < 3; ++ch)
+                                    s_stash[(u * 3 + ch) * kScanThreads + threadIdx.x] = dcol[ch];
+                            }
+                            sh_backward_dir(DEG, raw, d, my_pack, slot_clamp(G, mask, i, v), dcol, sg.mean);
+                        } else {
+                            sh_backward_dir(DEG, raw, d, my_sh, slot_clamp(G, mask, i, v), dcol, sg.mean);
+                        }
                     }
                     refine += g4.y;
+                    ++u;
                 }
             }
             chain_raw(act, hf, filt, sg, gls, gq, glogit);
@@ -340,14 +390,24 @@ __global__ __launch_bounds__(256) void k_gaussian_bwd(GaussArgs a) {
             densify_store_row(a.dens.stats, i, dr);
         }
         if (!precomp) {
+            int u = 0;
             for (int gr = 0; gr < a.n_groups; ++gr) {
                 const GaussGroup& G = a.grp[gr];
                 const uint32_t mask = G.view_mask[i];
                 for (int v = 0; v < G.n_views; ++v) {
                     if (!((mask >> v) & 1u)) continue;
-                    const float2* gp = reinterpret_cast<const float2*>(G.sgrad + (v * n + i) * kGradStride);
-                    const float2 g3 = gp[3], g4 = gp[4];
-                    const float dcol[3] = {g3.x, g3.y, g4.x};
+                    float dcol[3];
+                    if (kPacked && u < kStashViews) {
+#pragma unroll
+                        for (int ch = 0; ch < 3; ++ch) dcol[ch] = s_stash[(u * 3 + ch) * kScanThreads + threadIdx.x];
+                    } else {
+                        const float2* gp = reinterpret_cast<const float2*>(G.sgrad + (v * n + i) * kGradStride);
+                        const float2 g3 = gp[3], g4 = gp[4];
+                        dcol[0] = g3.x;
+                        dcol[1] = g3.y;
+                        dcol[2] = g4.x;
+                    }
+                    ++u;
                     float d[3], raw[3];
                     view_dir(mean, G.views[v].vi.cam.position, d, raw);
                     sh_backward_coef(DEG, d, slot_clamp(G, mask, i, v), dcol, gsh);
@@ -355,8 +415,10 @@ __global__ __launch_bounds__(256) void k_gaussian_bwd(GaussArgs a) {
             }
         }
     }
-    // own row only: the coefficients just read are replaced by their gradients
+    // own row only: the coefficients just read are replaced by their gradients (packed rows and
+    // stashes overlap other threads' float rows: all of them are read first)
     if (use_sh) {
+        if constexpr (kPacked) __syncthreads();
 #pragma unroll
         for (int k = 0; k < NB * 3; ++k) my_sh[k] = gsh[k];
     }

@@ -558,8 +558,10 @@ PHT_HD void sh_backward_coef(int degree, const T d[3], uint32_t clamp_bits, cons
         for (int ch = 0; ch < 3; ++ch) gsh[k * 3 + ch] += Y[k] * dl[ch];
 }
 
-template <class T>
-PHT_HD void sh_backward_dir(int degree, const T raw_dir[3], const T d[3], const T* coef, uint32_t clamp_bits,
+// coef: the coefficients, anything indexable as coef[k * 3 + ch] (a pointer, or a device-side view
+// of a packed reduced-precision row).
+template <class T, class C>
+PHT_HD void sh_backward_dir(int degree, const T raw_dir[3], const T d[3], const C& coef, uint32_t clamp_bits,
                             const T dl_rgb[3], T gmean[3]) {
     if (degree < 1) return;
     T dl[3];

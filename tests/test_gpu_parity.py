@@ -197,6 +197,37 @@ def test_reduced_precision_storage(cuda, oracle, lp):
     print(f"  {lp} image vs FP64 oracle of the FP32 parameters: {err:.2e}")
 
 
+@pytest.mark.parametrize("n_views", [3, 12])
+def test_reduced_precision_backward(cuda, oracle, n_views):
+    """bf16 storage through the backward, against the oracle on the same bf16 parameters. The
+    per-Gaussian backward keeps the colour gradients of a Gaussian's first eight visible views in
+    LDS for its SH-coefficient pass and reads those of any later view from the records again: with
+    12 ring views most Gaussians are visible in more than eight of them."""
+    from photara_studio_amd.rasterizer import Rasterizer
+    from photara_studio_amd.scenes import model_from, ring_cameras
+    from photara_studio_amd.train import to_low_precision
+    arrays = small_scene(2000, 1.0, 13)
+    cams = ring_cameras(n_views, 3.5, [(96, 72), (80, 64)])
+    opts = training_options()
+    print(f"[parity] bf16 backward, {n_views} views", flush=True)
+    model = to_low_precision(model_from(arrays, cuda), torch.bfloat16)
+    r = Rasterizer()
+    res = r.forward_views(model, cams, opts)
+    seen = sum((x.radii > 0).to(torch.int32) for x in res).cpu().numpy()
+    if n_views > 8:
+        assert (seen > 8).sum() > len(seen) // 2, np.bincount(seen)
+    orc = oracle.OracleRun(oracle.Scene(arrays, lp="bf16"), cams, opts, 32)
+    for k in range(n_views):
+        assert np.abs(res[k].color.cpu().numpy() - orc.color[k]).max() <= IMG_TOL, k
+    gimg = _grad_images(res, 17)
+    grads = r.backward_views(model, res, [torch.from_numpy(g).to(cuda) for g in gimg])
+    og = orc.backward(gimg)
+    for name in ("means", "log_scales", "quaternions", "opacity_logits", "sh"):
+        err = _rel(getattr(grads, name).cpu().numpy().reshape(og[name].shape), og[name])
+        print(f"  grad {name}: rel err {err:.2e}")
+        assert err <= GRAD_TOL, (name, err)
+
+
 def test_l1_loss_parity(cuda, oracle):
     from photara_studio_amd import _abi
     import ctypes as C
