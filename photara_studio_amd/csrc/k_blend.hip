@@ -126,6 +126,27 @@ __device__ __forceinline__ bool pair_alpha_flat(float a2, float b2, float c2, fl
     return ok && !(alpha < K<float>::alpha_floor());
 }
 
+// The backward's alpha of a pair: the same decision as pair_alpha_flat (and so as the forward),
+// with G from the hardware exponential (v_exp_f32, within a few ulp of the forward's polynomial)
+// instead of the nine-instruction polynomial. The decision compares alpha with the 1/255 floor, so
+// a lane whose alpha lies within kFloorMargin of it (about 64 ulp; the two exponentials differ by
+// far less) takes the forward's exponential and its exact decision; every other lane's decision
+// cannot differ from it. G and alpha feed only the gradients, which are compared to a tolerance.
+// A rejected pair's G and alpha are unspecified (possibly inf or NaN) and must not be used.
+constexpr float kFloorMargin = 3.0e-8f;
+__device__ __forceinline__ bool pair_alpha_bwd(float a2, float b2, float c2, float op, float dx, float dy,
+                                               float clip, float& alpha, float& G) {
+    // (non-short-circuit & on the predicates: lane masks, no branches)
+    const float p2 = pair_power2(a2, b2, c2, dx, dy);
+    G = __builtin_amdgcn_exp2f(p2);
+    alpha = min_(clip, op * G);
+    if (fabsf(alpha - K<float>::alpha_floor()) <= kFloorMargin) { // rare: skipped by the wave almost always
+        G = exp2_pair(p2);
+        alpha = min_(clip, op * G);
+    }
+    return !(p2 > 0.0f) & !(p2 < kSkipPower2) & !(alpha < K<float>::alpha_floor());
+}
+
 struct TileCoord {
     int tile, v, tx, ty;
 };
@@ -428,9 +449,8 @@ __device__ __forceinline__ void blend_bwd_item(const BlendArgs& a, int item, int
             for (int k = 0; k < Q; ++k) {
                 if (!((m >> k) & 1u)) continue;
                 const float dx = A.x - pxf[k], dy = A.y - pyf[k];
-                float alpha = 0.0f, G = 0.0f;
-                const bool contrib =
-                    pair_alpha_flat<false>(A.z, A.w, B.x, B.y, dx, dy, clip, alpha, G) && idx < last[k];
+                float alpha, G;
+                const bool contrib = pair_alpha_bwd(A.z, A.w, B.x, B.y, dx, dy, clip, alpha, G) & (idx < last[k]);
                 if (__ballot(contrib) == 0) continue;
                 any = true;
                 // Branch-free over the wave: a lane without a contributing pair takes alpha = 0 and a
