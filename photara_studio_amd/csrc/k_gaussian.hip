@@ -230,16 +230,31 @@ template <int W, int LP>
 __device__ __forceinline__ void struct_adam_pass(float* P, float* M, float* V, uint16_t* L, int rows, const float* s_g,
                                                  int first, float lr, const pht_adam& A, float c1, float c2, float lo,
                                                  float hi) {
-    for (int e = int(threadIdx.x); e < rows * W; e += kScanThreads) {
+    // rows <= 256, so a thread has at most W elements: all of their loads are issued before any
+    // update (a load per array per iteration, behind the previous iteration's stores, left one
+    // round trip per element)
+    float p[W], m[W], v[W];
+#pragma unroll
+    for (int it = 0; it < W; ++it) {
+        const int e = int(threadIdx.x) + it * kScanThreads;
+        if (e < rows * W) {
+            p[it] = P[e];
+            m[it] = M[e];
+            v[it] = V[e];
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < W; ++it) {
+        const int e = int(threadIdx.x) + it * kScanThreads;
+        if (e >= rows * W) break;
         const int row = e / W, k = e - row * W;
-        float p = P[e], m = M[e], v = V[e];
-        adam_update(p, m, v, s_g[row * kStructGrads + first + k], lr, A.beta1, A.beta2, c1, c2, A.epsilon, lo, hi,
-                    false);
-        P[e] = p;
-        M[e] = m;
-        V[e] = v;
+        adam_update(p[it], m[it], v[it], s_g[row * kStructGrads + first + k], lr, A.beta1, A.beta2, c1, c2, A.epsilon,
+                    lo, hi, false);
+        P[e] = p[it];
+        M[e] = m[it];
+        V[e] = v[it];
         if constexpr (LP != PHT_LP_NONE) {
-            if (L) L[e] = f32_to_lp(p, LP);
+            if (L) L[e] = f32_to_lp(p[it], LP);
         }
     }
 }
