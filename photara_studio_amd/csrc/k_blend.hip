@@ -12,8 +12,8 @@
 // Backward : the closed-form per-pair terms of blend_bucket_backward (render_backward.cu:383-489),
 //            walked back to front per pixel from the stored final transmittance and contributor
 //            count (T_before = T_after / (1 - alpha)), with no per-bucket snapshots. Each lane
-//            sums its pixels' terms, the wave reduce-scatters the ten values (wave_scatter10),
-//            and ten lanes commit them with one global atomic each per (instance, tile).
+//            sums its pixels' terms, the wave sums the ten values through LDS (wave_reduce10_lds),
+//            and ten lanes commit them with one global atomic instruction per (instance, tile).
 #include <algorithm>
 
 #include "launch.h"
@@ -51,46 +51,6 @@ __device__ __forceinline__ uint32_t quadrant_mask(const SplatRec& r, int x0, int
         m |= 1u << k;
     }
     return m;
-}
-
-// Reduce-scatter of ten per-lane values over the wave (a transposed butterfly). Each of the
-// stages on lane bits 0-3 keeps half of the lane's values and adds the partner lane's copy of
-// them (the partner keeps the other half), so the ten values shrink to one per lane in
-// 5 + 3 + 2 + 1 adds; two cross-row exchanges then complete the sums. Every lane ends with the wave
-// total of value kScatterSlot[lane & 15] (-1: a padding slot). Every lane must be active.
-__constant__ int8_t kScatterSlot[16] = {0, 5, 3, 8, 2, 7, -1, -1, 1, 6, 4, 9, -1, -1, -1, -1};
-
-__device__ __forceinline__ float lane_xor4(float v) {
-    // lane ^ 4 within each row: banks 0 and 2 take lane + 4 (row_shl:4), banks 1 and 3 lane - 4
-    const int x = __builtin_bit_cast(int, v);
-    const int t = __builtin_amdgcn_update_dpp(0, x, 0x104, 0xf, 0x5, false);
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(t, x, 0x114, 0xf, 0xa, false));
-}
-
-__device__ __forceinline__ float wave_scatter10(const float (&r)[10], int lane) {
-    const bool b0 = lane & 1, b1 = lane & 2, b2 = lane & 4, b3 = lane & 8;
-    float a[6], c[4], d[2];
-#pragma unroll
-    for (int i = 0; i < 5; ++i) { // lane ^ 1: quad_perm [1, 0, 3, 2]
-        const float keep = b0 ? r[5 + i] : r[i], send = b0 ? r[i] : r[5 + i];
-        a[i] = keep + dpp_move<0xb1, 0xf>(send);
-    }
-    a[5] = 0.0f;
-#pragma unroll
-    for (int i = 0; i < 3; ++i) { // lane ^ 2: quad_perm [2, 3, 0, 1]
-        const float keep = b1 ? a[3 + i] : a[i], send = b1 ? a[i] : a[3 + i];
-        c[i] = keep + dpp_move<0x4e, 0xf>(send);
-    }
-    c[3] = 0.0f;
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-        const float keep = b2 ? c[2 + i] : c[i], send = b2 ? c[i] : c[2 + i];
-        d[i] = keep + lane_xor4(send);
-    }
-    const float keep = b3 ? d[1] : d[0], send = b3 ? d[0] : d[1];
-    float e = keep + dpp_move<0x128, 0xf>(send); // lane ^ 8: row_ror:8
-    e += __shfl_xor(e, 16, 64);
-    return e + __shfl_xor(e, 32, 64);
 }
 
 // Round-to-integer float to int32 by v_cvt_i32_f32, which saturates out-of-range values and maps
@@ -358,17 +318,42 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(8, 8))
     });
 }
 
+// Sum over the wave of ten per-lane values, through the wave's own LDS region red[10 * kRedPitch]:
+// each lane stores its values value-major (red[g * kRedPitch + lane]); lane 4g + q (g < 10,
+// q < 4) adds value g of source lanes 16q .. 16q + 15 (four 16-byte reads), and two quad-permute
+// adds complete the sum. Lane 4g ends with the total of value g (slot_of_lane). The stores and
+// reads go to the LDS pipe, which the backward leaves nearly idle: 17 VALU instructions against
+// the butterfly's ~45. Every lane must be active.
+constexpr int kRedPitch = 68; // 64 lanes + 4: value rows start on different banks
+__device__ __forceinline__ int slot_of_lane(int lane) { return (lane & 3) == 0 && lane < 40 ? lane >> 2 : -1; }
+__device__ __forceinline__ float wave_reduce10_lds(const float (&r)[10], int lane, float* red) {
+#pragma unroll
+    for (int g = 0; g < 10; ++g) red[g * kRedPitch + lane] = r[g];
+    wave_lds_sync();
+    float s = 0.0f;
+    if (lane < 40) {
+        const float4* p = reinterpret_cast<const float4*>(red + (lane >> 2) * kRedPitch + (lane & 3) * 16);
+        const float4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3];
+        s = (((q0.x + q0.y) + (q0.z + q0.w)) + ((q1.x + q1.y) + (q1.z + q1.w))) +
+            (((q2.x + q2.y) + (q2.z + q2.w)) + ((q3.x + q3.y) + (q3.z + q3.w)));
+    }
+    wave_lds_sync(); // the region is free for the next instance's values
+    s += dpp_move<0xb1, 0xf>(s); // quad_perm [1, 0, 3, 2]
+    s += dpp_move<0x4e, 0xf>(s); // quad_perm [2, 3, 0, 1]
+    return s;
+}
+
 // One (view, tile) work item of the backward: back-to-front from the stored final transmittance.
 template <int TS, int BATCH, bool REFINE>
 __device__ __forceinline__ void blend_bwd_item(const BlendArgs& a, int item, int n_items, float4* sA, float4* sB,
-                                               float4* sC, uint32_t* sM, uint32_t* sS) {
+                                               float4* sC, uint32_t* sM, uint32_t* sS, float* sR) {
     constexpr int Q = (TS / 8) * (TS / 8);
     constexpr int NG = 10;
     const DevView* dvp;
     const TileCoord tc = tile_of_item(a, item, n_items, dvp);
     const DevView& dv = *dvp;
     const int lane = int(threadIdx.x) & 63;
-    const int slot_g = kScatterSlot[lane & 15];
+    const int slot_g = slot_of_lane(lane);
     const int W = dv.vi.cam.width, H = dv.vi.cam.height;
     const uint2 range = a.ranges[tc.tile];
 
@@ -502,20 +487,20 @@ __device__ __forceinline__ void blend_bwd_item(const BlendArgs& a, int item, int
                 T[k] = contrib ? Tb : T[k];
             }
             if (!any) continue;
-            float tot = wave_scatter10(r, lane);
+            float tot = wave_reduce10_lds(r, lane, sR);
             if constexpr (!REFINE) {
                 // moments -> gradients: dL/dmean2d = -opacity (conic . (sum s dx, sum s dy)),
                 // dL/dconic = -opacity / 2 (sum s dx^2, sum s dx dy, sum s dy^2); the first-order
-                // moments sit in lanes 0 and 8 (kScatterSlot)
+                // moments sit in lanes 0 and 4 (slot_of_lane)
                 const float sx = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, tot), 0));
-                const float sy = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, tot), 8));
+                const float sy = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, tot), 4));
                 const float4 Cc = sC[j];
                 const float op = sB[j].y;
                 if (slot_g == 0) tot = -op * fmaf(Cc.y, sx, Cc.z * sy);
                 else if (slot_g == 1) tot = -op * fmaf(Cc.w, sy, Cc.z * sx);
                 else if (slot_g >= 2 && slot_g <= 4) tot = (-0.5f * op) * tot;
             }
-            if (lane < 16 && slot_g >= 0 && tot != 0.0f)
+            if (slot_g >= 0 && tot != 0.0f)
                 atomic_add_f32(a.sgrad + grad_offset(sS[j], a.vp_log2, a.n_gauss) + slot_g, tot);
         }
     }
@@ -532,9 +517,10 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(REFINE
     __shared__ float4 sC[WPB][BATCH]; // (colour b, conic a, b, c): the gradient terms take the conic itself
     __shared__ uint32_t sM[WPB][BATCH];
     __shared__ uint32_t sS[WPB][BATCH];
+    __shared__ __align__(16) float sR[WPB][10 * kRedPitch];
     const int w = int(__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)); // wave-uniform
     for_each_item<WPB>(a, a.queue ? a.queue + 1 : nullptr, n_items, [&](int item) {
-        blend_bwd_item<TS, BATCH, REFINE>(a, item, n_items, sA[w], sB[w], sC[w], sM[w], sS[w]);
+        blend_bwd_item<TS, BATCH, REFINE>(a, item, n_items, sA[w], sB[w], sC[w], sM[w], sS[w], sR[w]);
     });
 }
 
