@@ -353,6 +353,8 @@ __global__ __launch_bounds__(256) void k_gaussian_bwd(GaussArgs a) {
             const float filt = hf ? a.g.filter_3d[i] : 0.0f;
             const Activated<float> act = activate<float>(ls, q, logit, hf, filt);
             int u = 0; // visible views so far: the stash slot of the next one
+            // the views' d(3D covariance) is summed and chained to scale and rotation once
+            float dcov[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
             for (int gr = 0; gr < a.n_groups; ++gr) {
                 const GaussGroup& G = a.grp[gr];
                 const uint32_t mask = G.view_mask[i];
@@ -363,7 +365,7 @@ __global__ __launch_bounds__(256) void k_gaussian_bwd(GaussArgs a) {
                     const float dconic[4] = {g1.x, g1.y, g2.x, g2.y};
                     const float dm2[2] = {g0.x, g0.y};
                     const pht_camera& cam = G.views[v].vi.cam;
-                    splat_backward_pinhole(mean, act, cam, a.s, dconic, dm2, sg);
+                    splat_backward_pinhole_view(mean, act, cam, a.s, dconic, dm2, sg, dcov);
                     const float dcol[3] = {g3.x, g3.y, g4.x};
                     if (precomp) {
                         gcol[0] += dcol[0]; gcol[1] += dcol[1]; gcol[2] += dcol[2];
@@ -385,6 +387,7 @@ __global__ __launch_bounds__(256) void k_gaussian_bwd(GaussArgs a) {
                     ++u;
                 }
             }
+            splat_backward_cov_chain(act, a.s, dcov, sg);
             chain_raw(act, hf, filt, sg, gls, gq, glogit);
         }
         if (a.out.refine_weight) a.out.refine_weight[i] = refine;

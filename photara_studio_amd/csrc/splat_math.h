@@ -700,10 +700,62 @@ template <class T> struct SplatGrad {
     T opacity;    // wrt activated opacity
 };
 
+// The d(transform) -> d(mean3d) terms of one view and the mean2d -> mean3d term
+// (geometry.cuh:826-903 pinhole, 385-399).
 template <class T>
-PHT_HD void splat_backward_pinhole(const T mean[3], const Activated<T>& act, const pht_camera& cam,
-                                   const pht_settings& s, const T d_conic[4], const T d_mean2d[2],
-                                   SplatGrad<T>& io) {
+PHT_HD void splat_backward_pinhole_mean(const T mean[3], const pht_camera& cam, const float* view, T hx, T hy,
+                                        const T t[3], T txc, T tyc, T x_mul, T y_mul, const M3<T>& Tm,
+                                        const M3<T>& Vrk, const M3<T>& W, T dL_da, T dL_db, T dL_dc,
+                                        const T d_mean2d[2], SplatGrad<T>& io) {
+    (void)mean;
+    (void)cam;
+    const T dT00 = T(2) * ((Tm.at(0, 0) * Vrk.at(0, 0) + Tm.at(0, 1) * Vrk.at(0, 1)) + Tm.at(0, 2) * Vrk.at(0, 2)) * dL_da +
+                   ((Tm.at(1, 0) * Vrk.at(0, 0) + Tm.at(1, 1) * Vrk.at(0, 1)) + Tm.at(1, 2) * Vrk.at(0, 2)) * dL_db;
+    const T dT01 = T(2) * ((Tm.at(0, 0) * Vrk.at(1, 0) + Tm.at(0, 1) * Vrk.at(1, 1)) + Tm.at(0, 2) * Vrk.at(1, 2)) * dL_da +
+                   ((Tm.at(1, 0) * Vrk.at(1, 0) + Tm.at(1, 1) * Vrk.at(1, 1)) + Tm.at(1, 2) * Vrk.at(1, 2)) * dL_db;
+    const T dT02 = T(2) * ((Tm.at(0, 0) * Vrk.at(2, 0) + Tm.at(0, 1) * Vrk.at(2, 1)) + Tm.at(0, 2) * Vrk.at(2, 2)) * dL_da +
+                   ((Tm.at(1, 0) * Vrk.at(2, 0) + Tm.at(1, 1) * Vrk.at(2, 1)) + Tm.at(1, 2) * Vrk.at(2, 2)) * dL_db;
+    const T dT10 = T(2) * ((Tm.at(1, 0) * Vrk.at(0, 0) + Tm.at(1, 1) * Vrk.at(0, 1)) + Tm.at(1, 2) * Vrk.at(0, 2)) * dL_dc +
+                   ((Tm.at(0, 0) * Vrk.at(0, 0) + Tm.at(0, 1) * Vrk.at(0, 1)) + Tm.at(0, 2) * Vrk.at(0, 2)) * dL_db;
+    const T dT11 = T(2) * ((Tm.at(1, 0) * Vrk.at(1, 0) + Tm.at(1, 1) * Vrk.at(1, 1)) + Tm.at(1, 2) * Vrk.at(1, 2)) * dL_dc +
+                   ((Tm.at(0, 0) * Vrk.at(1, 0) + Tm.at(0, 1) * Vrk.at(1, 1)) + Tm.at(0, 2) * Vrk.at(1, 2)) * dL_db;
+    const T dT12 = T(2) * ((Tm.at(1, 0) * Vrk.at(2, 0) + Tm.at(1, 1) * Vrk.at(2, 1)) + Tm.at(1, 2) * Vrk.at(2, 2)) * dL_dc +
+                   ((Tm.at(0, 0) * Vrk.at(2, 0) + Tm.at(0, 1) * Vrk.at(2, 1)) + Tm.at(0, 2) * Vrk.at(2, 2)) * dL_db;
+    const T dJ00 = (W.at(0, 0) * dT00 + W.at(0, 1) * dT01) + W.at(0, 2) * dT02;
+    const T dJ02 = (W.at(2, 0) * dT00 + W.at(2, 1) * dT01) + W.at(2, 2) * dT02;
+    const T dJ11 = (W.at(1, 0) * dT10 + W.at(1, 1) * dT11) + W.at(1, 2) * dT12;
+    const T dJ12 = (W.at(2, 0) * dT10 + W.at(2, 1) * dT11) + W.at(2, 2) * dT12;
+    const T tz = T(1) / t[2];
+    const T tz2 = tz * tz;
+    const T tz3 = tz2 * tz;
+    const T dt[3] = {
+        x_mul * (-hx * tz2 * dJ02),
+        y_mul * (-hy * tz2 * dJ12),
+        ((-hx * tz2 * dJ00 - hy * tz2 * dJ11) + ((T(1) + x_mul) * hx * txc) * tz3 * dJ02) +
+            ((T(1) + y_mul) * hy * tyc) * tz3 * dJ12};
+    T gm[3];
+    xform_dir_transpose(dt, view, gm);
+    io.mean[0] += gm[0]; io.mean[1] += gm[1]; io.mean[2] += gm[2];
+
+    // mean2d -> mean3d (geometry.cuh:385-399), unclamped point
+    {
+        const T rz = T(1) / (t[2] + T(1.0e-7));
+        const T sx = t[0] * rz, sy = t[1] * rz;
+        const T dmx = d_mean2d[0], dmy = d_mean2d[1];
+        io.mean[0] += (hx * (T(view[0]) - sx * T(view[2])) * dmx + hy * (T(view[1]) - sy * T(view[2])) * dmy) * rz;
+        io.mean[1] += (hx * (T(view[4]) - sx * T(view[6])) * dmx + hy * (T(view[5]) - sy * T(view[6])) * dmy) * rz;
+        io.mean[2] += (hx * (T(view[8]) - sx * T(view[10])) * dmx + hy * (T(view[9]) - sy * T(view[10])) * dmy) * rz;
+    }
+}
+
+// splat_backward_pinhole in two parts. The per-view part adds the view's d(2D conic) and
+// d(mean2d) terms to io.mean / io.opacity and its d(3D covariance) (six values, dcov) to dcov_sum;
+// splat_backward_cov_chain takes a d(3D covariance) to d(scale) and d(rotation). The chain is
+// linear, so a Gaussian seen in several views can sum dcov over the views and chain once.
+template <class T>
+PHT_HD void splat_backward_pinhole_view(const T mean[3], const Activated<T>& act, const pht_camera& cam,
+                                        const pht_settings& s, const T d_conic[4], const T d_mean2d[2],
+                                        SplatGrad<T>& io, T dcov_sum[6]) {
     const float* view = cam.world_to_camera;
     const T hx = T(cam.fx), hy = T(cam.fy);
     T t[3];
@@ -774,7 +826,21 @@ PHT_HD void splat_backward_pinhole(const T mean[3], const Activated<T>& act, con
         dcov[4] = (T(2) * t02 * t01 * dL_da + (t01 * t12 + t02 * t11) * dL_db) + T(2) * t11 * t12 * dL_dc;
     }
 
-    // dSigma -> d(scale, rotation) (geometry.cuh:317-379), dL_dr = 0
+    for (int k = 0; k < 6; ++k) dcov_sum[k] = dcov_sum[k] + dcov[k];
+    (void)min_id;
+
+    // dL/dT -> dL/dJ -> dL/dt (geometry.cuh:826-903, pinhole)
+    splat_backward_pinhole_mean(mean, cam, view, hx, hy, t, txc, tyc, x_mul, y_mul, Tm, Vrk, W, dL_da, dL_db, dL_dc,
+                                d_mean2d, io);
+}
+
+// dSigma -> d(scale, rotation) (geometry.cuh:317-379), dL_dr = 0
+template <class T>
+PHT_HD void splat_backward_cov_chain(const Activated<T>& act, const pht_settings& s, const T dcov[6],
+                                     SplatGrad<T>& io) {
+    const T sm = T(s.scale_modifier);
+    const T sl[3] = {sm * act.scale[0], sm * act.scale[1], sm * act.scale[2]};
+    const M3<T> R = quat_rotation<T>(act.rot);
     {
         const T dsig[3][3] = {{dcov[0], T(0.5) * dcov[1], T(0.5) * dcov[2]},
                               {T(0.5) * dcov[1], dcov[3], T(0.5) * dcov[4]},
@@ -801,7 +867,6 @@ PHT_HD void splat_backward_pinhole(const T mean[3], const Activated<T>& act, con
             dMt.m[r][1] = dMt.m[r][1] * sl[1];
             dMt.m[r][2] = dMt.m[r][2] * sl[2];
         }
-        (void)min_id;
         const T r_ = act.rot[0], x = act.rot[1], y = act.rot[2], z = act.rot[3];
         const T a01 = dMt.at(0, 1), a10 = dMt.at(1, 0);
         const T a02 = dMt.at(0, 2), a20 = dMt.at(2, 0);
@@ -815,45 +880,16 @@ PHT_HD void splat_backward_pinhole(const T mean[3], const Activated<T>& act, con
         io.rot[3] += ((T(2) * r_ * (a01 - a10) + T(2) * x * (a20 + a02)) + T(2) * y * (a12 + a21)) -
                      T(4) * z * (a11 + a00);
     }
+}
 
-    // dL/dT -> dL/dJ -> dL/dt (geometry.cuh:826-903, pinhole)
-    const T dT00 = T(2) * ((Tm.at(0, 0) * Vrk.at(0, 0) + Tm.at(0, 1) * Vrk.at(0, 1)) + Tm.at(0, 2) * Vrk.at(0, 2)) * dL_da +
-                   ((Tm.at(1, 0) * Vrk.at(0, 0) + Tm.at(1, 1) * Vrk.at(0, 1)) + Tm.at(1, 2) * Vrk.at(0, 2)) * dL_db;
-    const T dT01 = T(2) * ((Tm.at(0, 0) * Vrk.at(1, 0) + Tm.at(0, 1) * Vrk.at(1, 1)) + Tm.at(0, 2) * Vrk.at(1, 2)) * dL_da +
-                   ((Tm.at(1, 0) * Vrk.at(1, 0) + Tm.at(1, 1) * Vrk.at(1, 1)) + Tm.at(1, 2) * Vrk.at(1, 2)) * dL_db;
-    const T dT02 = T(2) * ((Tm.at(0, 0) * Vrk.at(2, 0) + Tm.at(0, 1) * Vrk.at(2, 1)) + Tm.at(0, 2) * Vrk.at(2, 2)) * dL_da +
-                   ((Tm.at(1, 0) * Vrk.at(2, 0) + Tm.at(1, 1) * Vrk.at(2, 1)) + Tm.at(1, 2) * Vrk.at(2, 2)) * dL_db;
-    const T dT10 = T(2) * ((Tm.at(1, 0) * Vrk.at(0, 0) + Tm.at(1, 1) * Vrk.at(0, 1)) + Tm.at(1, 2) * Vrk.at(0, 2)) * dL_dc +
-                   ((Tm.at(0, 0) * Vrk.at(0, 0) + Tm.at(0, 1) * Vrk.at(0, 1)) + Tm.at(0, 2) * Vrk.at(0, 2)) * dL_db;
-    const T dT11 = T(2) * ((Tm.at(1, 0) * Vrk.at(1, 0) + Tm.at(1, 1) * Vrk.at(1, 1)) + Tm.at(1, 2) * Vrk.at(1, 2)) * dL_dc +
-                   ((Tm.at(0, 0) * Vrk.at(1, 0) + Tm.at(0, 1) * Vrk.at(1, 1)) + Tm.at(0, 2) * Vrk.at(1, 2)) * dL_db;
-    const T dT12 = T(2) * ((Tm.at(1, 0) * Vrk.at(2, 0) + Tm.at(1, 1) * Vrk.at(2, 1)) + Tm.at(1, 2) * Vrk.at(2, 2)) * dL_dc +
-                   ((Tm.at(0, 0) * Vrk.at(2, 0) + Tm.at(0, 1) * Vrk.at(2, 1)) + Tm.at(0, 2) * Vrk.at(2, 2)) * dL_db;
-    const T dJ00 = (W.at(0, 0) * dT00 + W.at(0, 1) * dT01) + W.at(0, 2) * dT02;
-    const T dJ02 = (W.at(2, 0) * dT00 + W.at(2, 1) * dT01) + W.at(2, 2) * dT02;
-    const T dJ11 = (W.at(1, 0) * dT10 + W.at(1, 1) * dT11) + W.at(1, 2) * dT12;
-    const T dJ12 = (W.at(2, 0) * dT10 + W.at(2, 1) * dT11) + W.at(2, 2) * dT12;
-    const T tz = T(1) / t[2];
-    const T tz2 = tz * tz;
-    const T tz3 = tz2 * tz;
-    const T dt[3] = {
-        x_mul * (-hx * tz2 * dJ02),
-        y_mul * (-hy * tz2 * dJ12),
-        ((-hx * tz2 * dJ00 - hy * tz2 * dJ11) + ((T(1) + x_mul) * hx * txc) * tz3 * dJ02) +
-            ((T(1) + y_mul) * hy * tyc) * tz3 * dJ12};
-    T gm[3];
-    xform_dir_transpose(dt, view, gm);
-    io.mean[0] += gm[0]; io.mean[1] += gm[1]; io.mean[2] += gm[2];
-
-    // mean2d -> mean3d (geometry.cuh:385-399), unclamped point
-    {
-        const T rz = T(1) / (t[2] + T(1.0e-7));
-        const T sx = t[0] * rz, sy = t[1] * rz;
-        const T dmx = d_mean2d[0], dmy = d_mean2d[1];
-        io.mean[0] += (hx * (T(view[0]) - sx * T(view[2])) * dmx + hy * (T(view[1]) - sy * T(view[2])) * dmy) * rz;
-        io.mean[1] += (hx * (T(view[4]) - sx * T(view[6])) * dmx + hy * (T(view[5]) - sy * T(view[6])) * dmy) * rz;
-        io.mean[2] += (hx * (T(view[8]) - sx * T(view[10])) * dmx + hy * (T(view[9]) - sy * T(view[10])) * dmy) * rz;
-    }
+// One view, whole: the per-view part and the chain of its own d(3D covariance).
+template <class T>
+PHT_HD void splat_backward_pinhole(const T mean[3], const Activated<T>& act, const pht_camera& cam,
+                                   const pht_settings& s, const T d_conic[4], const T d_mean2d[2],
+                                   SplatGrad<T>& io) {
+    T dcov[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+    splat_backward_pinhole_view(mean, act, cam, s, d_conic, d_mean2d, io, dcov);
+    splat_backward_cov_chain(act, s, dcov, io);
 }
 
 // Activation chain to raw parameters (update_structure_adam_row grad part,
