@@ -506,10 +506,14 @@ __device__ __forceinline__ void blend_bwd_item(const BlendArgs& a, int item, int
     }
 }
 
+// Waves per SIMD the backward is compiled for. Without the refine statistic it fits 80 VGPRs
+// (6 waves) without spilling. With a 32-deep LDS batch a wave's LDS (staging plus the 2.7 KB wave
+// sum region, 4.5 KB) leaves room for 7 waves: there it is held to 72 VGPRs at the cost of a few
+// spilled dwords, which measured faster (blend backward 2.83 -> 2.74 ms, bench step). With deeper
+// batches the LDS allows no more than 6 anyway. With the refine statistic, 5.
+constexpr int bwd_waves(int batch, bool refine) { return refine ? 5 : (batch <= 32 ? 7 : 6); }
 template <int TS, int BATCH, int WPB, bool REFINE>
-// Without the refine statistic the kernel fits 80 VGPRs (6 waves per SIMD) without spilling; with it,
-// 5 waves.
-__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(REFINE ? 5 : 6))) void k_blend_bwd(BlendArgs a,
+__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(bwd_waves(BATCH, REFINE)))) void k_blend_bwd(BlendArgs a,
                                                                                                       int n_items) {
     if (a.counters->overflow) return;
     __shared__ float4 sA[WPB][BATCH];
