@@ -63,14 +63,15 @@ struct pht_workspace {
     uint32_t slot_cap = 0, inst_cap = 0;
     DevView* views = nullptr;
     Counters* counters = nullptr;
+    uint32_t* key_parts = nullptr; // partial ranges of the visible depth keys (kKeyParts pairs)
     SplatRec* rec = nullptr;
     uint2* meta = nullptr;
     uint32_t* vis_flag = nullptr;
     float* sgrad = nullptr;
     uint32_t* view_mask = nullptr;
-    uint32_t* dkeys[2] = {nullptr, nullptr};
-    uint32_t* dvals[2] = {nullptr, nullptr};
-    uint32_t* dtiles[2] = {nullptr, nullptr};
+    uint32_t* dkeys[3] = {nullptr, nullptr, nullptr}; // depth sort: two ping-pong buffers and the result
+    uint32_t* dvals[3] = {nullptr, nullptr, nullptr};
+    uint32_t* dtiles[3] = {nullptr, nullptr, nullptr};
     uint32_t* offsets = nullptr;
     uint32_t* tkeys[2] = {nullptr, nullptr};
     uint32_t* tvals[2] = {nullptr, nullptr};
@@ -117,12 +118,13 @@ static void carve_workspace(pht_workspace& w, char* base, const pht_capacity& ca
     const size_t inst = size_t(cap.instances);
     w.views = c.take<DevView>(PHT_MAX_VIEWS);
     w.counters = c.take<Counters>(1);
+    w.key_parts = c.take<uint32_t>(2 * kKeyParts);
     w.rec = c.take<SplatRec>(slots);
     w.meta = c.take<uint2>(slots);
     w.vis_flag = c.take<uint32_t>(slots);
     w.sgrad = c.take<float>(slots * kGradStride);
     w.view_mask = c.take<uint32_t>(size_t(cap.gaussians));
-    for (int k = 0; k < 2; ++k) {
+    for (int k = 0; k < 3; ++k) {
         w.dkeys[k] = c.take<uint32_t>(slots);
         w.dvals[k] = c.take<uint32_t>(slots);
         w.dtiles[k] = c.take<uint32_t>(slots);
@@ -289,7 +291,7 @@ static int forward_impl(pht_workspace* ws, const pht_gaussians* g, const pht_cam
 
     launch_upload_views(ws->views, hv.data(), n_views, st);
     ws->mark(0, 0, st);
-    launch_clear(ws->counters, ws->ranges, uint32_t(tiles), st);
+    launch_clear(ws->counters, ws->ranges, uint32_t(tiles), ws->key_parts, st);
     PreArgs pa;
     pa.g = *g;
     pa.s = *s;
@@ -301,14 +303,17 @@ static int forward_impl(pht_workspace* ws, const pht_gaussians* g, const pht_cam
     pa.vis_flag = ws->vis_flag;
     pa.view_mask = ws->view_mask;
     pa.counters = ws->counters;
+    pa.key_parts = ws->key_parts;
     const int64_t n_slots = g->count << ws->vp_log2;
     launch_preprocess(pa, n_slots, st);
+    launch_key_range(ws->key_parts, ws->counters, st);
     ws->mark(0, 1, st);
     ws->mark(1, 0, st);
     launch_compact_visible(ws->vis_flag, ws->meta, uint32_t(n_slots), ws->dkeys[0], ws->dvals[0], ws->dtiles[0],
                            ws->scan_scratch, &ws->counters->visible, st);
-    const int dsel = launch_radix_sort(ws->dkeys, ws->dvals, ws->dtiles, &ws->counters->visible, uint32_t(n_slots),
-                                       32, ws->hist, ws->digit_total, st);
+    launch_depth_sort(ws->dkeys, ws->dvals, ws->dtiles, &ws->counters->visible, uint32_t(n_slots),
+                      &ws->counters->key_nmin, ws->hist, ws->digit_total, st);
+    const int dsel = 2; // the depth sort's result buffer
     ws->mark(1, 1, st);
     ws->mark(2, 0, st);
     launch_tile_offsets(ws->dtiles[dsel], &ws->counters->visible, uint32_t(n_slots), ws->offsets, ws->scan_scratch,

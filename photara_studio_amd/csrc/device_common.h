@@ -40,7 +40,9 @@ struct Counters {
     uint32_t fp64;        // FP64-routed conics
     uint32_t scan_total;  // scratch total of the last chunked scan
     uint32_t queue[2];    // persistent blend work-queue heads (forward, backward)
-    uint32_t pad;
+    uint32_t key_nmin;    // max over visible slots of ~(depth key): the least key is ~key_nmin (k_key_range)
+    uint32_t key_max;     // max over visible slots of the depth key (k_key_range)
+    uint32_t pad[2];
 };
 
 // Blend record, one per (gaussian, view) slot (slot = g * vp + v), 64 bytes so that every

@@ -19,10 +19,33 @@ __global__ void k_upload_views(DevView* dst, ViewBatch b, int n) {
     if (i < n) dst[i] = b.v[i];
 }
 
-__global__ __launch_bounds__(256) void k_clear(Counters* counters, uint2* ranges, uint32_t n_tiles) {
+__global__ __launch_bounds__(256) void k_clear(Counters* counters, uint2* ranges, uint32_t n_tiles,
+                                               uint32_t* key_parts) {
     const uint32_t i0 = blockIdx.x * 256u + threadIdx.x;
     if (i0 < sizeof(Counters) / 4) reinterpret_cast<uint32_t*>(counters)[i0] = 0u;
     for (uint32_t i = i0; i < n_tiles; i += gridDim.x * 256u) ranges[i] = make_uint2(0u, 0u);
+    for (uint32_t i = i0; i < 2u * kKeyParts; i += gridDim.x * 256u) key_parts[i] = 0u;
+}
+
+// Folds the partial key ranges into the counters (one block of kKeyParts threads).
+__global__ __launch_bounds__(kKeyParts) void k_key_range(const uint32_t* key_parts, Counters* counters) {
+    __shared__ uint32_t s[2][kKeyParts / 64];
+    const int t = threadIdx.x, lane = lane_id(), w = t >> 6;
+    const uint32_t a = wave_max_u32(key_parts[2 * t]), b = wave_max_u32(key_parts[2 * t + 1]);
+    if (lane == 0) {
+        s[0][w] = a;
+        s[1][w] = b;
+    }
+    __syncthreads();
+    if (t == 0) {
+        uint32_t x = 0, y = 0;
+        for (int k = 0; k < kKeyParts / 64; ++k) {
+            x = s[0][k] > x ? s[0][k] : x;
+            y = s[1][k] > y ? s[1][k] : y;
+        }
+        counters->key_nmin = x;
+        counters->key_max = y;
+    }
 }
 
 // Half extents of the box around the alpha >= 1/255 ellipse {q(d) <= 2 log(255 op)} of the
@@ -142,7 +165,7 @@ __global__ __launch_bounds__(256) void k_preprocess(PreArgs a, int64_t n_slots) 
     const bool in_range = t < n_slots;
     const bool valid = in_range && v < a.n_views;
     bool visible = false, fp64 = false;
-    uint32_t clamp_flags = 0;
+    uint32_t clamp_flags = 0, dkey = 0;
     if (valid) {
         const DevView& dv = a.views[v];
         const float mean[3] = {a.g.means[3 * gi], a.g.means[3 * gi + 1], a.g.means[3 * gi + 2]};
@@ -189,7 +212,8 @@ __global__ __launch_bounds__(256) void k_preprocess(PreArgs a, int64_t n_slots) 
                 r.b = make_float4(c2, p.opacity, rgb[0], rgb[1]);
                 r.c = make_float4(rgb[2], ex, ey, qmax);
                 r.d = make_float4(p.conic[0], p.conic[1], p.conic[2], p.opacity);
-                a.meta[t] = make_uint2(f2u(p.depth), touched | (clamp << kMetaClampShift));
+                dkey = f2u(p.depth);
+                a.meta[t] = make_uint2(dkey, touched | (clamp << kMetaClampShift));
                 clamp_flags = clamp;
             }
         }
@@ -217,6 +241,16 @@ __global__ __launch_bounds__(256) void k_preprocess(PreArgs a, int64_t n_slots) 
     }
     const uint64_t fb = __ballot(fp64);
     if (lane == 0 && fb) atomicAdd(&a.counters->fp64, uint32_t(__popcll(fb)));
+    // the range of the visible depth keys: the depth sort ranks keys relative to the least, and
+    // skips its top pass when the range fits 24 bits
+    if (bal != 0) {
+        const uint32_t kmax = wave_max_u32(visible ? dkey : 0u), knmin = wave_max_u32(visible ? ~dkey : 0u);
+        if (lane == 0) {
+            uint32_t* part = a.key_parts + 2 * ((blockIdx.x * 4u + (threadIdx.x >> 6)) % uint32_t(kKeyParts));
+            atomicMax(part, knmin);
+            atomicMax(part + 1, kmax);
+        }
+    }
 }
 
 // Emits (global tile, slot) pairs for the depth-sorted visible splats at their scanned offsets
@@ -446,8 +480,12 @@ void launch_upload_views(DevView* dst, const DevView* host_views, int n_views, h
     }
 }
 
-void launch_clear(Counters* counters, uint2* ranges, uint32_t n_tiles, hipStream_t st) {
-    hipLaunchKernelGGL(k_clear, dim3(grid_stride_blocks(n_tiles)), dim3(256), 0, st, counters, ranges, n_tiles);
+void launch_clear(Counters* counters, uint2* ranges, uint32_t n_tiles, uint32_t* key_parts, hipStream_t st) {
+    hipLaunchKernelGGL(k_clear, dim3(grid_stride_blocks(n_tiles)), dim3(256), 0, st, counters, ranges, n_tiles,
+                       key_parts);
+}
+void launch_key_range(const uint32_t* key_parts, Counters* counters, hipStream_t st) {
+    hipLaunchKernelGGL(k_key_range, dim3(1), dim3(kKeyParts), 0, st, key_parts, counters);
 }
 
 template <int DEG, int LP> static void pre_dispatch(const PreArgs& a, int64_t n_slots, hipStream_t st) {

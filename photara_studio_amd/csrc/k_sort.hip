@@ -199,6 +199,16 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_write(Src src, Sink sink,
 // array in the write-out, and half the per-tile bookkeeping of 8192-key tiles); the depth sort,
 // with a third array in LDS, 1024 x 8.
 
+// Depth sort: its passes rank key - base, base = the least visible key, from the preprocess's
+// range counters (krange = {max of ~key, max of key}), so a pass whose digit is then the same for
+// every key can be skipped. Its pass roles: 0 = an ordinary pass; 1 = the pass before the top
+// one, which writes to the final (alternate) buffers when the top pass is not needed; 2 = the
+// top pass, which does nothing when the range fits 24 bits. The tile sort passes krange = null.
+__device__ __forceinline__ uint32_t key_base(const uint32_t* krange) { return krange ? ~krange[0] : 0u; }
+__device__ __forceinline__ bool top_pass_needed(const uint32_t* krange) {
+    return !krange || ((krange[1] - ~krange[0]) >> 24) != 0u; // no visible key: 0 - ~0 = 1, not needed
+}
+
 // 1024 threads per chunk (16 waves sharing the block's LDS sub-histograms): the chunks are the
 // downsweep's, one per CU, so the block's width sets how many loads a CU has in
 // flight (four times as many as 256-thread blocks; the six passes of the bench workload went
@@ -209,8 +219,10 @@ constexpr int kUpThreads = 1024;
 template <class K>
 __global__ __launch_bounds__(kUpThreads) void k_radix_upsweep(const K* keys, const uint32_t* n_ptr,
                                                               uint32_t n_const, uint32_t cap, int shift, int nb,
-                                                              uint32_t* hist) {
+                                                              uint32_t* hist, const uint32_t* krange, int role) {
     constexpr uint32_t KV = 16u / sizeof(K);
+    if (role == 2 && !top_pass_needed(krange)) return; // uniform over the grid
+    const uint32_t kb = key_base(krange);
     // kUpSubs sub-histograms, lane l counting into sub-histogram l % kUpSubs. Their rows are 257
     // words apart, so the lanes of one instruction that hold the same digit hit kUpSubs different
     // banks: a pass whose keys share one or two digit values (the top byte of depth keys) no
@@ -231,7 +243,7 @@ __global__ __launch_bounds__(kUpThreads) void k_radix_upsweep(const K* keys, con
     constexpr uint32_t kStep = KV * kUpThreads;
     auto count_word = [&](uint32_t x) {
         if constexpr (sizeof(K) == 4) {
-            atomicAdd(&h[(x >> shift) & 255u], 1u);
+            atomicAdd(&h[((x - kb) >> shift) & 255u], 1u);
         } else {
             atomicAdd(&h[((x & 0xffffu) >> shift) & 255u], 1u);
             atomicAdd(&h[((x >> 16) >> shift) & 255u], 1u);
@@ -252,7 +264,7 @@ __global__ __launch_bounds__(kUpThreads) void k_radix_upsweep(const K* keys, con
         for (int u = 0; u < 4; ++u) count_vec(q[u]);
     }
     for (; i < hiv; i += kStep) count_vec(k4[i / KV]);
-    for (uint32_t i = hiv + t; i < hi; i += kUpThreads) atomicAdd(&h[(uint32_t(keys[i]) >> shift) & 255u], 1u);
+    for (uint32_t i = hiv + t; i < hi; i += kUpThreads) atomicAdd(&h[((uint32_t(keys[i]) - kb) >> shift) & 255u], 1u);
     __syncthreads();
     if (t < 256) {
         uint32_t c = 0;
@@ -282,13 +294,20 @@ __global__ __launch_bounds__(1024) void k_radix_scan(uint32_t* hist, int nb, uin
     if (t == 0) digit_total[blockIdx.x] = total;
 }
 
+// The final buffers of a depth sort whose top pass may be skipped (role 1 writes there then).
+template <class K> struct SortOut {
+    K* keys;
+    uint32_t* vals;
+    uint32_t* aux;
+};
 template <class K, bool AUX, int BT, int R>
 __global__ __launch_bounds__(BT) void k_radix_downsweep(const K* keys_in, const uint32_t* vals_in,
                                                                   const uint32_t* aux_in, K* keys_out,
                                                                   uint32_t* vals_out, uint32_t* aux_out,
                                                                   const uint32_t* n_ptr, uint32_t n_const,
                                                                   uint32_t cap, int shift, int nb,
-                                                                  const uint32_t* hist, const uint32_t* digit_total) {
+                                                                  const uint32_t* hist, const uint32_t* digit_total,
+                                                                  const uint32_t* krange, int role, SortOut<K> alt) {
     constexpr int NW = BT / 64;
     constexpr int kSortTile = BT * R;
     static_assert(kSortTile * 4 >= BT / 64 * 256 * 8, "the digit-match tables live in s_key");
@@ -303,6 +322,16 @@ __global__ __launch_bounds__(BT) void k_radix_downsweep(const K* keys_in, const 
     __shared__ uint32_t s_w[NW];
     const int t = threadIdx.x, lane = lane_id(), w = t >> 6;
     const bool digit_thread = t < 256; // thread t < 256 owns digit t in the per-tile bookkeeping
+    if (role != 0) { // uniform over the grid
+        const bool top = top_pass_needed(krange);
+        if (role == 2 && !top) return;
+        if (role == 1 && !top) {
+            keys_out = alt.keys;
+            vals_out = alt.vals;
+            aux_out = alt.aux;
+        }
+    }
+    const uint32_t kb = key_base(krange);
     // global digit base of this block = exclusive digit prefix + this block's offset in the digit
     uint32_t run = 0;
     {
@@ -345,7 +374,7 @@ __global__ __launch_bounds__(BT) void k_radix_downsweep(const K* keys_in, const 
         for (int r = 0; r < R; ++r) {
             const uint32_t i = wbase + uint32_t(r) * 64u + uint32_t(lane);
             const bool valid = i < hi;
-            const uint32_t d = (key[r] >> shift) & 255u;
+            const uint32_t d = ((key[r] - kb) >> shift) & 255u;
             // lanes of this row holding digit d: each lane ORs its bit into the wave's table entry,
             // reads the entry back and clears it (the wave's LDS operations complete in order;
             // OR is order-free, so the mask and the ranks below do not depend on atomic order)
@@ -385,7 +414,7 @@ __global__ __launch_bounds__(BT) void k_radix_downsweep(const K* keys_in, const 
         for (int r = 0; r < R; ++r) {
             const uint32_t i = wbase + uint32_t(r) * 64u + uint32_t(lane);
             if (i < hi) {
-                const uint32_t d = (key[r] >> shift) & 255u;
+                const uint32_t d = ((key[r] - kb) >> shift) & 255u;
                 const uint32_t p = s_start[d] + s_cnt[w][d] + rank[r];
                 s_key[p] = key[r];
                 s_val[p] = val[r];
@@ -397,7 +426,7 @@ __global__ __launch_bounds__(BT) void k_radix_downsweep(const K* keys_in, const 
         const uint32_t tile_n = hi - base < uint32_t(kSortTile) ? hi - base : uint32_t(kSortTile);
         for (uint32_t j = t; j < tile_n; j += BT) {
             const uint32_t k = s_key[j];
-            const uint32_t d = (k >> shift) & 255u;
+            const uint32_t d = ((k - kb) >> shift) & 255u;
             const uint32_t dst = s_shift[d] + j;
             if (dst >= n) continue; // unreachable for consistent ranks; keeps every store in bounds
             keys_out[dst] = K(k);
@@ -456,27 +485,45 @@ int launch_radix_sort(uint32_t* keys[2], uint32_t* vals[2], uint32_t* aux[2], co
             uint16_t* kin = reinterpret_cast<uint16_t*>(keys[cur]);
             uint16_t* kout = reinterpret_cast<uint16_t*>(keys[cur ^ 1]);
             hipLaunchKernelGGL(k_radix_upsweep<uint16_t>, dim3(nb), dim3(kUpThreads), 0, st, kin, n_ptr, 0u, cap,
-                               shift, nb, hist);
+                               shift, nb, hist, nullptr, 0);
             hipLaunchKernelGGL(k_radix_scan, dim3(256), dim3(1024), 0, st, hist, nb, digit_total);
             hipLaunchKernelGGL((k_radix_downsweep<uint16_t, false, 1024, 16>), dim3(nb), dim3(1024), 0, st, kin,
                                vals[cur], nullptr, kout, vals[cur ^ 1], nullptr, n_ptr, 0u, cap, shift, nb, hist,
-                               digit_total);
+                               digit_total, nullptr, 0, SortOut<uint16_t>{nullptr, nullptr, nullptr});
         } else {
             hipLaunchKernelGGL(k_radix_upsweep<uint32_t>, dim3(nb), dim3(kUpThreads), 0, st, keys[cur], n_ptr, 0u,
-                               cap, shift, nb, hist);
+                               cap, shift, nb, hist, nullptr, 0);
             hipLaunchKernelGGL(k_radix_scan, dim3(256), dim3(1024), 0, st, hist, nb, digit_total);
             if (aux)
                 hipLaunchKernelGGL((k_radix_downsweep<uint32_t, true, 1024, 8>), dim3(nb), dim3(1024), 0, st,
                                    keys[cur], vals[cur], aux[cur], keys[cur ^ 1], vals[cur ^ 1], aux[cur ^ 1], n_ptr,
-                                   0u, cap, shift, nb, hist, digit_total);
+                                   0u, cap, shift, nb, hist, digit_total, nullptr, 0,
+                                   SortOut<uint32_t>{nullptr, nullptr, nullptr});
             else
                 hipLaunchKernelGGL((k_radix_downsweep<uint32_t, false, 1024, 16>), dim3(nb), dim3(1024), 0, st,
                                    keys[cur], vals[cur], nullptr, keys[cur ^ 1], vals[cur ^ 1], nullptr, n_ptr, 0u,
-                                   cap, shift, nb, hist, digit_total);
+                                   cap, shift, nb, hist, digit_total, nullptr, 0,
+                                   SortOut<uint32_t>{nullptr, nullptr, nullptr});
         }
         cur ^= 1;
     }
     return cur;
+}
+
+void launch_depth_sort(uint32_t* keys[3], uint32_t* vals[3], uint32_t* aux[3], const uint32_t* n_ptr, uint32_t cap,
+                       const uint32_t* krange, uint32_t* hist, uint32_t* digit_total, hipStream_t st) {
+    const int nb = blocks_for(cap) < 256 ? blocks_for(cap) : 256;
+    // buffers 0 -> 1 -> 0 -> 1 -> 2, or 0 -> 1 -> 0 -> 2 when the top pass is not needed
+    const int src[4] = {0, 1, 0, 1}, dst[4] = {1, 0, 1, 2}, role[4] = {0, 0, 1, 2};
+    for (int p = 0; p < 4; ++p) {
+        const int a = src[p], b = dst[p];
+        hipLaunchKernelGGL(k_radix_upsweep<uint32_t>, dim3(nb), dim3(kUpThreads), 0, st, keys[a], n_ptr, 0u, cap,
+                           8 * p, nb, hist, krange, role[p]);
+        hipLaunchKernelGGL(k_radix_scan, dim3(256), dim3(1024), 0, st, hist, nb, digit_total);
+        hipLaunchKernelGGL((k_radix_downsweep<uint32_t, true, 1024, 8>), dim3(nb), dim3(1024), 0, st, keys[a],
+                           vals[a], aux[a], keys[b], vals[b], aux[b], n_ptr, 0u, cap, 8 * p, nb, hist, digit_total,
+                           krange, role[p], SortOut<uint32_t>{keys[2], vals[2], aux[2]});
+    }
 }
 
 } // namespace pht

@@ -17,7 +17,12 @@ struct PreArgs {
     uint32_t* vis_flag;
     uint32_t* view_mask;
     Counters* counters;
+    uint32_t* key_parts; // [kKeyParts][2] partial (max of ~key, max of key) of the visible depth keys
 };
+// The visible depth keys' range is gathered in kKeyParts partial pairs (wave w of block b adds
+// to pair (4b + w) mod kKeyParts), so that no single address takes every wave's atomics, and
+// folded into the counters by launch_key_range.
+constexpr int kKeyParts = 256;
 
 struct BlendArgs {
     const uint2* ranges;
@@ -96,10 +101,16 @@ void launch_tile_offsets(const uint32_t* sorted_tiles, const uint32_t* visible_p
 // digit_total 256.
 int launch_radix_sort(uint32_t* keys[2], uint32_t* vals[2], uint32_t* aux[2], const uint32_t* n_ptr, uint32_t cap,
                       int bits, uint32_t* hist, uint32_t* digit_total, hipStream_t st, bool narrow = false);
+// Stable sort of the visible (depth key, slot, tile count) lists by depth key, ranking key - the
+// least visible key (krange: the preprocess's {max of ~key, max of key} counters); the top pass
+// is skipped when the keys' range fits 24 bits. The result is in buffer 2 either way.
+void launch_depth_sort(uint32_t* keys[3], uint32_t* vals[3], uint32_t* aux[3], const uint32_t* n_ptr, uint32_t cap,
+                       const uint32_t* krange, uint32_t* hist, uint32_t* digit_total, hipStream_t st);
 
 // k_preprocess.hip
 void launch_upload_views(DevView* dst, const DevView* host_views, int n_views, hipStream_t st);
-void launch_clear(Counters* counters, uint2* ranges, uint32_t n_tiles, hipStream_t st);
+void launch_clear(Counters* counters, uint2* ranges, uint32_t n_tiles, uint32_t* key_parts, hipStream_t st);
+void launch_key_range(const uint32_t* key_parts, Counters* counters, hipStream_t st);
 void launch_preprocess(const PreArgs& a, int64_t n_slots, hipStream_t st);
 void launch_emit_instances(const uint32_t* sorted_slots, const uint32_t* sorted_tiles, const uint32_t* offsets,
                            const Counters* counters, uint32_t slot_cap, const SplatRec* rec, const DevView* views,
