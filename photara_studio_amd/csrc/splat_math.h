@@ -415,8 +415,10 @@ PHT_HD TileWalk<T> tile_walk_setup(T mx, T my, const T conic[3], T op, int grid_
     w.threshold = threshold;
     w.tsf = T(ts);
     w.inv_ts = inv_ts;
-    w.eu = sqrt_(threshold * w.av / det);
-    w.ev = sqrt_(threshold * au / det);
+    // the extents along u and v are ex and ey in walk order (the same operations: av is c when
+    // walking x, so threshold * av / det is threshold * c / det)
+    w.eu = iterate_x ? ex : ey;
+    w.ev = iterate_x ? ey : ex;
     int u_min = int(floor_((w.cu - w.eu) * inv_ts));
     u_min = u_min < w.gu ? u_min : w.gu;
     u_min = u_min > 0 ? u_min : 0;
