@@ -349,7 +349,7 @@ __device__ __forceinline__ float wave_reduce10_lds(const float (&r)[10], int lan
 // One (view, tile) work item of the backward: back-to-front from the stored final transmittance.
 template <int TS, int BATCH, bool REFINE>
 __device__ __forceinline__ void blend_bwd_item(const BlendArgs& a, int item, int n_items, float4* sA, float4* sB,
-                                               float4* sC, uint32_t* sS, float* sR) {
+                                               float4* sC, uint32_t* sM, uint32_t* sS, float* sR) {
     constexpr int Q = (TS / 8) * (TS / 8);
     constexpr int NG = 10;
     const DevView* dvp;
@@ -398,23 +398,16 @@ __device__ __forceinline__ void blend_bwd_item(const BlendArgs& a, int item, int
     // canonical once, so the per-pair min(clip, .) needs no canonicalisation of its own
     const float clip = __builtin_canonicalizef(a.alpha_clip);
     const float fW = float(W), fH = float(H);
-    constexpr int NQ = (BATCH + 63) / 64; // 64-instance rows of an LDS batch
     for (int hi = int(tile_max); hi > 0; hi -= BATCH) {
         const int lo = hi - BATCH > 0 ? hi - BATCH : 0;
         const int cnt = hi - lo;
         wave_lds_sync(); // the previous batch's LDS reads are complete
-        // The batch's quadrant masks as lane ballots, in scalar registers: bit i of qb[q][k] says
-        // that instance lo + 64 q + i reaches quadrant k before the quadrant's last contributor.
-        // The pair loop visits the set bits from the highest down, so an instance that reaches no
-        // such quadrant costs no LDS read. (In the forward, whose per-instance work is shorter,
-        // the same loop measured slower than its LDS mask read.)
-        uint64_t qb[NQ][Q];
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
+        for (int q = 0; q < (BATCH + 63) / 64; ++q) {
             const int j = q * 64 + lane;
-            uint32_t m = 0;
             if (j < cnt) {
                 const uint32_t slot = a.inst_vals[range.x + lo + j];
+                uint32_t m = 0;
                 if (slot < a.slot_cap) {
                     const SplatRec& r = a.rec[slot];
                     sA[j] = r.a;
@@ -422,101 +415,96 @@ __device__ __forceinline__ void blend_bwd_item(const BlendArgs& a, int item, int
                     sC[j] = make_float4(r.c.x, r.d.x, r.d.y, r.d.z);
                     m = quadrant_mask<Q>(r, tc.tx * TS, tc.ty * TS);
                 }
+                sM[j] = m;
                 sS[j] = slot;
             }
-#pragma unroll
-            for (int k = 0; k < Q; ++k) qb[q][k] = __ballot(((m >> k) & 1u) & (uint32_t(lo + j) < qmax[k]));
         }
         wave_lds_sync();
+        for (int j = cnt - 1; j >= 0; --j) {
+            const uint32_t idx = uint32_t(lo + j);
+            uint32_t m = __builtin_amdgcn_readfirstlane(sM[j]);
 #pragma unroll
-        for (int q = NQ - 1; q >= 0; --q) {
-            uint64_t todo = 0;
+            for (int k = 0; k < Q; ++k)
+                if (idx >= qmax[k]) m &= ~(1u << k);
+            if (m == 0) continue;
+            const float4 A = sA[j], B = sB[j], Cc = sC[j];
+            const float cb = Cc.x;
+            float r[NG];
 #pragma unroll
-            for (int k = 0; k < Q; ++k) todo |= qb[q][k];
-            while (todo != 0) { // back to front: the highest set bit first
-                const int i = 63 - int(__builtin_clzll(todo));
-                todo &= ~(1ull << i);
-                const int j = q * 64 + i;
-                const uint32_t idx = uint32_t(lo + j);
-                const float4 A = sA[j], B = sB[j], Cc = sC[j];
-                const float cb = Cc.x;
-                float r[NG];
+            for (int g = 0; g < NG; ++g) r[g] = 0.0f;
+            bool any = false;
 #pragma unroll
-                for (int g = 0; g < NG; ++g) r[g] = 0.0f;
-                bool any = false;
-#pragma unroll
-                for (int k = 0; k < Q; ++k) {
-                    if (!((qb[q][k] >> i) & 1u)) continue;
-                    const float dx = A.x - pxf[k], dy = A.y - pyf[k];
-                    float alpha, G;
-                    const bool contrib = pair_alpha_bwd(A.z, A.w, B.x, B.y, dx, dy, clip, alpha, G) & (idx < last[k]);
-                    if (__ballot(contrib) == 0) continue;
-                    any = true;
-                    // Branch-free over the wave: a lane without a contributing pair takes alpha = 0 and a
-                    // zero s (below), so its blend weight and every gradient term are exact zeros and E
-                    // keeps its value; its G (unspecified, possibly inf or NaN) is never used.
-                    // (The recovered T and the gradients are not compared bit for bit: the hardware
-                    // reciprocal / square root and fused multiply-adds are used throughout.)
-                    const float ac = contrib ? alpha : 0.0f;
-                    const float inv_1ma = __builtin_amdgcn_rcpf(1.0f - ac);
-                    const float Tb = T[k] * inv_1ma;
-                    const float wgt = ac * Tb;
-                    const float c_dot = fmaf(cb, gp2[k], fmaf(B.w, gp1[k], B.z * gp0[k]));
-                    const float dL_dalpha = fmaf(Tb, c_dot, -E[k] * inv_1ma);
-                    // s = G dL/dalpha, the pair's dL/dopacity; dL/dG = opacity * dL/dalpha, so
-                    // dL/dG * G = opacity * s
-                    const float s = contrib ? G * dL_dalpha : 0.0f;
-                    if constexpr (REFINE) {
-                        const float wG = B.y * s;
-                        // dL/dmean2d = dL/dG * dG/d(dx, dy), dG/ddx = -G (a dx + b dy), dG/ddy = -G (b dx + c dy)
-                        const float ddx = -wG * fmaf(dx, Cc.y, dy * Cc.z);
-                        const float ddy = -wG * fmaf(dy, Cc.w, dx * Cc.z);
-                        r[0] += ddx;
-                        r[1] += ddy;
-                        // dL/dconic = -dL/dG * G * (dx^2, dx dy, dy^2) / 2
-                        const float h = -0.5f * wG;
-                        const float hx = h * dx;
-                        r[2] = fmaf(hx, dx, r[2]);
-                        r[3] = fmaf(hx, dy, r[3]);
-                        r[4] = fmaf(h * dy, dy, r[4]);
-                        // the refine statistic is a norm per pixel: it needs the per-pair dL/dmean2d
-                        const float rx = ddx * fW, ry = ddy * fH;
-                        r[9] = fmaf(__builtin_amdgcn_sqrtf(fmaf(rx, rx, ry * ry)), refine_den[k], r[9]);
-                    } else {
-                        // Without it, both gradients are linear in the moments of the offsets weighted
-                        // by s: sum s (dx, dy) and sum s (dx^2, dx dy, dy^2). The opacity, the conic and
-                        // the -1/2 are applied once per (instance, tile) after the reduction.
-                        const float t = s * dx, u = s * dy;
-                        r[0] += t;
-                        r[1] += u;
-                        r[2] = fmaf(t, dx, r[2]);
-                        r[3] = fmaf(t, dy, r[3]);
-                        r[4] = fmaf(u, dy, r[4]);
-                    }
-                    r[5] += s;
-                    r[6] = fmaf(wgt, gp0[k], r[6]);
-                    r[7] = fmaf(wgt, gp1[k], r[7]);
-                    r[8] = fmaf(wgt, gp2[k], r[8]);
-                    E[k] = fmaf(c_dot, wgt, E[k]);
-                    T[k] = contrib ? Tb : T[k];
+            for (int k = 0; k < Q; ++k) {
+                if (!((m >> k) & 1u)) continue;
+                const float dx = A.x - pxf[k], dy = A.y - pyf[k];
+                float alpha, G;
+                const bool contrib = pair_alpha_bwd(A.z, A.w, B.x, B.y, dx, dy, clip, alpha, G) & (idx < last[k]);
+                if (__ballot(contrib) == 0) continue;
+                any = true;
+                // Branch-free over the wave: a lane without a contributing pair takes alpha = 0 and a
+                // zero s (below), so its blend weight and every gradient term are exact zeros and E
+                // keeps its value; its G (unspecified, possibly inf or NaN) is never used.
+                // (The recovered T and the gradients are not compared bit for bit: the hardware
+                // reciprocal / square root and fused multiply-adds are used throughout.)
+                const float ac = contrib ? alpha : 0.0f;
+                const float inv_1ma = __builtin_amdgcn_rcpf(1.0f - ac);
+                const float Tb = T[k] * inv_1ma;
+                const float wgt = ac * Tb;
+                const float c_dot = fmaf(cb, gp2[k], fmaf(B.w, gp1[k], B.z * gp0[k]));
+                const float dL_dalpha = fmaf(Tb, c_dot, -E[k] * inv_1ma);
+                // s = G dL/dalpha, the pair's dL/dopacity; dL/dG = opacity * dL/dalpha, so
+                // dL/dG * G = opacity * s
+                const float s = contrib ? G * dL_dalpha : 0.0f;
+                if constexpr (REFINE) {
+                    const float wG = B.y * s;
+                    // dL/dmean2d = dL/dG * dG/d(dx, dy), dG/ddx = -G (a dx + b dy), dG/ddy = -G (b dx + c dy)
+                    const float ddx = -wG * fmaf(dx, Cc.y, dy * Cc.z);
+                    const float ddy = -wG * fmaf(dy, Cc.w, dx * Cc.z);
+                    r[0] += ddx;
+                    r[1] += ddy;
+                    // dL/dconic = -dL/dG * G * (dx^2, dx dy, dy^2) / 2
+                    const float h = -0.5f * wG;
+                    const float hx = h * dx;
+                    r[2] = fmaf(hx, dx, r[2]);
+                    r[3] = fmaf(hx, dy, r[3]);
+                    r[4] = fmaf(h * dy, dy, r[4]);
+                    // the refine statistic is a norm per pixel: it needs the per-pair dL/dmean2d
+                    const float rx = ddx * fW, ry = ddy * fH;
+                    r[9] = fmaf(__builtin_amdgcn_sqrtf(fmaf(rx, rx, ry * ry)), refine_den[k], r[9]);
+                } else {
+                    // Without it, both gradients are linear in the moments of the offsets weighted
+                    // by s: sum s (dx, dy) and sum s (dx^2, dx dy, dy^2). The opacity, the conic and
+                    // the -1/2 are applied once per (instance, tile) after the reduction.
+                    const float t = s * dx, u = s * dy;
+                    r[0] += t;
+                    r[1] += u;
+                    r[2] = fmaf(t, dx, r[2]);
+                    r[3] = fmaf(t, dy, r[3]);
+                    r[4] = fmaf(u, dy, r[4]);
                 }
-                if (!any) continue;
-                float tot = wave_reduce10_lds(r, lane, sR);
-                if constexpr (!REFINE) {
-                    // moments -> gradients: dL/dmean2d = -opacity (conic . (sum s dx, sum s dy)),
-                    // dL/dconic = -opacity / 2 (sum s dx^2, sum s dx dy, sum s dy^2); the first-order
-                    // moments sit in lanes 0 and 4 (slot_of_lane)
-                    const float sx = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, tot), 0));
-                    const float sy = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, tot), 4));
-                    const float4 Cc = sC[j];
-                    const float op = sB[j].y;
-                    if (slot_g == 0) tot = -op * fmaf(Cc.y, sx, Cc.z * sy);
-                    else if (slot_g == 1) tot = -op * fmaf(Cc.w, sy, Cc.z * sx);
-                    else if (slot_g >= 2 && slot_g <= 4) tot = (-0.5f * op) * tot;
-                }
-                if (slot_g >= 0 && tot != 0.0f)
-                    atomic_add_f32(a.sgrad + grad_offset(sS[j], a.vp_log2, a.n_gauss) + slot_g, tot);
+                r[5] += s;
+                r[6] = fmaf(wgt, gp0[k], r[6]);
+                r[7] = fmaf(wgt, gp1[k], r[7]);
+                r[8] = fmaf(wgt, gp2[k], r[8]);
+                E[k] = fmaf(c_dot, wgt, E[k]);
+                T[k] = contrib ? Tb : T[k];
             }
+            if (!any) continue;
+            float tot = wave_reduce10_lds(r, lane, sR);
+            if constexpr (!REFINE) {
+                // moments -> gradients: dL/dmean2d = -opacity (conic . (sum s dx, sum s dy)),
+                // dL/dconic = -opacity / 2 (sum s dx^2, sum s dx dy, sum s dy^2); the first-order
+                // moments sit in lanes 0 and 4 (slot_of_lane)
+                const float sx = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, tot), 0));
+                const float sy = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, tot), 4));
+                const float4 Cc = sC[j];
+                const float op = sB[j].y;
+                if (slot_g == 0) tot = -op * fmaf(Cc.y, sx, Cc.z * sy);
+                else if (slot_g == 1) tot = -op * fmaf(Cc.w, sy, Cc.z * sx);
+                else if (slot_g >= 2 && slot_g <= 4) tot = (-0.5f * op) * tot;
+            }
+            if (slot_g >= 0 && tot != 0.0f)
+                atomic_add_f32(a.sgrad + grad_offset(sS[j], a.vp_log2, a.n_gauss) + slot_g, tot);
         }
     }
 }
@@ -534,11 +522,12 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(bwd_wa
     __shared__ float4 sA[WPB][BATCH];
     __shared__ float4 sB[WPB][BATCH];
     __shared__ float4 sC[WPB][BATCH]; // (colour b, conic a, b, c): the gradient terms take the conic itself
+    __shared__ uint32_t sM[WPB][BATCH];
     __shared__ uint32_t sS[WPB][BATCH];
     __shared__ __align__(16) float sR[WPB][10 * kRedPitch];
     const int w = int(__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)); // wave-uniform
     for_each_item<WPB>(a, a.queue ? a.queue + 1 : nullptr, n_items, [&](int item) {
-        blend_bwd_item<TS, BATCH, REFINE>(a, item, n_items, sA[w], sB[w], sC[w], sS[w], sR[w]);
+        blend_bwd_item<TS, BATCH, REFINE>(a, item, n_items, sA[w], sB[w], sC[w], sM[w], sS[w], sR[w]);
     });
 }
 
