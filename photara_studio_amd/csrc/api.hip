@@ -346,9 +346,6 @@ static int forward_impl(pht_workspace* ws, const pht_gaussians* g, const pht_cam
     ba.slot_cap = ws->slot_cap;
     ba.n_gauss = g->count;
     ba.want_refine = 0;
-    ba.want_visibility = 0;
-    for (int v = 0; v < n_views; ++v)
-        if (out[v].visibility) ba.want_visibility = 1;
     ba.queue = s->blend_queue ? ws->counters->queue : nullptr;
     ba.loss_sum = l1 ? l1->loss_sum : nullptr;
     ws->mark(4, 0, st);
@@ -444,7 +441,6 @@ static int blend_backward_part(pht_workspace* ws, const pht_view_grads* dl, bool
     ba.slot_cap = ws->slot_cap;
     ba.n_gauss = ws->gaussians;
     ba.want_refine = want_refine ? 1 : 0;
-    ba.want_visibility = 0;
     ba.queue = ws->settings.blend_queue ? ws->counters->queue : nullptr;
     ba.loss_sum = nullptr;
     ws->mark(5, 0, st);

@@ -154,10 +154,8 @@ __device__ __forceinline__ void for_each_item(const BlendArgs& a, uint32_t* queu
     }
 }
 
-// One (view, tile) work item of the forward: front-to-back blend of the tile's sorted list. VIS:
-// some view of the launch writes contribution visibility (the per-instance bookkeeping it needs
-// is compiled out of the other launches).
-template <int TS, int BATCH, bool VIS>
+// One (view, tile) work item of the forward: front-to-back blend of the tile's sorted list.
+template <int TS, int BATCH>
 __device__ __forceinline__ void blend_fwd_item(const BlendArgs& a, int item, int n_items, float4* sA, float4* sB,
                                                float* sC, uint32_t* sM, uint32_t* sV) {
     constexpr int Q = (TS / 8) * (TS / 8);
@@ -200,7 +198,7 @@ __device__ __forceinline__ void blend_fwd_item(const BlendArgs& a, int item, int
 #pragma unroll
         for (int q = 0; q < (BATCH + 63) / 64; ++q) {
             const uint32_t j = uint32_t(q * 64 + lane);
-            if constexpr (VIS) my_slot[q] = 0xffffffffu;
+            my_slot[q] = 0xffffffffu;
             if (j < cnt) {
                 const uint32_t slot = a.inst_vals[base + j];
                 uint32_t m = 0;
@@ -210,10 +208,10 @@ __device__ __forceinline__ void blend_fwd_item(const BlendArgs& a, int item, int
                     sB[j] = r.b;
                     sC[j] = r.c.x;
                     m = quadrant_mask<Q>(r, tc.tx * TS, tc.ty * TS);
-                    if constexpr (VIS) my_slot[q] = slot;
+                    my_slot[q] = slot;
                 }
                 sM[j] = m;
-                if constexpr (VIS) sV[j] = 0u;
+                sV[j] = 0u;
             }
         }
         wave_lds_sync();
@@ -238,18 +236,17 @@ __device__ __forceinline__ void blend_fwd_item(const BlendArgs& a, int item, int
                         C2[k] = fmaf(cb, a_t, C2[k]);
                         T[k] = test_t;
                         last[k] = base - range.x + j + 1;
-                        if constexpr (VIS) blended = true;
+                        blended = true;
                     }
                 }
             }
-            if constexpr (VIS)
-                if (__ballot(blended) != 0 && lane == 0) sV[j] = 1u;
+            if (__ballot(blended) != 0 && lane == 0) sV[j] = 1u;
         }
 #pragma unroll
         for (int k = 0; k < Q; ++k)
             if (__ballot(T[k] > 0.0f) == 0) live &= ~(1u << k);
         wave_lds_sync();
-        if (VIS && dv.visibility) {
+        if (dv.visibility) {
 #pragma unroll
             for (int q = 0; q < (BATCH + 63) / 64; ++q) {
                 const uint32_t j = uint32_t(q * 64 + lane);
@@ -292,7 +289,7 @@ __device__ __forceinline__ void blend_fwd_item(const BlendArgs& a, int item, int
     }
 }
 
-template <int TS, int BATCH, int WPB, bool VIS>
+template <int TS, int BATCH, int WPB>
 // 8 waves per SIMD: the kernel fits 64 VGPRs without spilling (69 unconstrained, 7 waves), and the
 // extra wave hides more of the per-instance LDS reads (measured: 1.92 -> 1.83 ms on the batched
 // step). The backward does not fit its bound without spilling and keeps the default.
@@ -314,10 +311,10 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(8, 8))
     __shared__ float4 sB[WPB][BATCH];
     __shared__ float sC[WPB][BATCH];
     __shared__ uint32_t sM[WPB][BATCH];
-    __shared__ uint32_t sV[WPB][VIS ? BATCH : 1];
+    __shared__ uint32_t sV[WPB][BATCH];
     const int w = int(__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)); // wave-uniform
     for_each_item<WPB>(a, a.queue ? a.queue : nullptr, n_items, [&](int item) {
-        blend_fwd_item<TS, BATCH, VIS>(a, item, n_items, sA[w], sB[w], sC[w], sM[w], sV[w]);
+        blend_fwd_item<TS, BATCH>(a, item, n_items, sA[w], sB[w], sC[w], sM[w], sV[w]);
     });
 }
 
@@ -549,7 +546,7 @@ template <class K> static unsigned grid_for(K kernel, const BlendArgs& a, int n_
 }
 
 template <int TS, int BATCH, int WPB> static void fwd_launch(const BlendArgs& a, int n_items, hipStream_t st) {
-    const auto k = a.want_visibility ? k_blend_fwd<TS, BATCH, WPB, true> : k_blend_fwd<TS, BATCH, WPB, false>;
+    const auto k = k_blend_fwd<TS, BATCH, WPB>;
     hipLaunchKernelGGL(k, dim3(grid_for(k, a, n_items, WPB)), dim3(64 * WPB), 0, st, a, n_items);
 }
 template <int TS, int BATCH> static void fwd_shape(const BlendArgs& a, int n_items, int waves, hipStream_t st) {

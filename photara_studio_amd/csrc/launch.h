@@ -39,7 +39,6 @@ struct BlendArgs {
     uint32_t slot_cap;
     int64_t n_gauss;
     int want_refine; // backward: accumulate the refine statistic (only when a caller reads it)
-    int want_visibility; // forward: some view writes per-Gaussian contribution visibility
     const uint32_t* tile_order; // tile of each work item (heaviest first), or null for the XCD-contiguous map
     uint32_t* queue; // persistent work-queue counters [forward, backward] (zeroed per launch), or null: flat grid
     float* loss_sum; // forward with the fused L1 loss: += sum over views of scale * sum |C - target|, or null
